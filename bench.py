@@ -1,0 +1,180 @@
+"""Benchmark: Mrays/s of the 1024^2 Kerr Stokes render on one MI355X.
+
+Workload: a=0.94, i=17 deg, 1024x1024 pixels over +-20 M from r=1000, adaptive
+RK2 h = 0.05 (r - r_+) capped at 4096 steps, 128x64x128 11-channel brick with
+a thin disk.  A "step" of this benchmark is one full render of that image with
+the brick already resident in HBM.  Each path's error is measured against the
+full-FP64 GPU render of the same scene (itself checked against the CPU oracle
+by the tests).  The CPU baseline is the NumPy FP64 oracle on a bounded,
+evenly spaced row sample, run on all host cores.
+
+Prints ONE JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "blackhole-sim_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+import numpy as np  # noqa: E402
+
+# Algorithmic FP32 work of the fused render, counted from bh_render.hip
+# (FMA = 2 FLOP; sin, cos, rcp, sqrt and log2 count 1 each):
+#   one RK2 geodesic step (step size, two Carter-form right-hand sides, midpoint
+#   and update, transfer midpoint): F_GEO; one in-brick transfer segment
+#   (three brackets, trilinear weights, 11-channel interpolation, ds, RK2 Stokes
+#   step, optical depth): F_SAMPLE.  The kernel counts steps and samples exactly.
+F_GEO = 155.0
+F_SAMPLE = 298.0
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed); scalar issue caps at half of it
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel_l1(a, b) -> float:
+    return float(np.sum(np.abs(a - b)) / max(float(np.sum(np.abs(b))), 1e-300))
+
+
+def cpu_baseline(scene, brick, seconds: float) -> dict:
+    """Time the FP64 NumPy oracle on evenly spaced image rows using every host core."""
+    from oracle import render as orender
+
+    cores = len(os.sched_getaffinity(0))
+    W = scene.width
+    t = time.perf_counter()
+    calib_rows = np.array([scene.height // 2])
+    orender.render_rays(scene, brick, (calib_rows[:, None] * W + np.arange(W)).ravel())
+    per_row = time.perf_counter() - t
+    n_rows = int(max(cores, min(scene.height, seconds * cores / max(per_row, 1e-6))))
+    n_rows = max(cores, min(n_rows, scene.height))
+    rows = np.unique(np.linspace(0, scene.height - 1, n_rows).round().astype(np.int64))
+    pix = (rows[:, None] * W + np.arange(W)).ravel()
+    t = time.perf_counter()
+    orender.render_rays_parallel(scene, brick, pix, processes=cores)
+    dt = time.perf_counter() - t
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": pix.size / dt / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 NumPy oracle, {rows.size} evenly spaced rows of {scene.height} "
+                      f"({pix.size} rays), {dt:.1f} s on {cores} processes; CPU: {cpu}"}
+
+
+def time_path(rr, scene, db, precision, bufs, steps: int, warmup: int) -> dict:
+    d_iquv, d_steps, d_status = bufs
+    for _ in range(warmup):
+        rr.render_device(scene, db, precision, d_iquv, d_steps, d_status)
+        rr.synchronize()
+    kms = []
+    work = []
+    rr.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rr.render_device(scene, db, precision, d_iquv, d_steps, d_status)
+        kms.append(rr.last_kernel_ms())
+        work.append(rr.last_work())
+    rr.synchronize()
+    t1 = time.perf_counter()
+    return {"ms_per_step": (t1 - t0) * 1e3 / steps, "kernel_ms": float(np.mean(kms)), "work": work[-1]}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=int, default=1024)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+
+    from bhgrrt import scenes
+    from bhgrrt.render import PREC_FP32, PREC_FP64, Renderer
+
+    scene = scenes.scale_plain_scene(args.size, args.size)
+    brick = scenes.scale_brick()
+    n = scene.rays
+    log(f"[bench] scene {scene.width}x{scene.height}, brick {brick.shape}")
+    rr = Renderer(0)
+    db32 = rr.upload(brick, "f32")
+    d_iquv = rr.alloc(n * 4 * 8)
+    d_steps = rr.alloc(n * 4)
+    d_status = rr.alloc(n)
+    bufs = (d_iquv, d_steps, d_status)
+
+    log("[bench] FP64 reference render")
+    ref = rr.render(scene, db32, PREC_FP64)
+    ok = (ref["status"] & 8) == 0
+    I64 = ref["iquv"][..., 0]
+
+    log("[bench] plain FP32 path")
+    plain = time_path(rr, scene, db32, PREC_FP32, bufs, args.steps, args.warmup)
+    out32 = rr.render(scene, db32, PREC_FP32)
+    err32 = rel_l1(out32["iquv"][..., 0][ok].astype(np.float64), I64[ok])
+    quv32 = float(np.max(np.abs(out32["iquv"][..., 1:][ok].astype(np.float64) - ref["iquv"][..., 1:][ok])))
+    steps_total, samples_total = plain["work"]
+    flops = steps_total * F_GEO + samples_total * F_SAMPLE
+    achieved = flops / (plain["kernel_ms"] * 1e-3) / 1e12
+    value = n / (plain["ms_per_step"] * 1e-3) / 1e6
+    log(f"[bench] plain fp32: {plain['ms_per_step']:.3f} ms/render, kernel {plain['kernel_ms']:.3f} ms, "
+        f"{value:.1f} Mrays/s, rel L1 I {err32:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (FP64 NumPy oracle)")
+        cpu = cpu_baseline(scene, brick, args.cpu_seconds)
+
+    result = {
+        "metric": "Mrays/s, 1024² Kerr Stokes render: surrogate+mixed vs plain RK2 at ≤1e-3 I err",
+        "value": value,
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": plain["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP32 RK2 Kerr Stokes render, a=0.94, i=17deg, {scene.width}x{scene.height} "
+                               f"over +-20M, h=0.05(r-r+), <=4096 steps, brick 128x64x128x11 f32 (thin disk)",
+                   "path": "plain_fp32", "rays": n},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "work": {"rk2_steps": steps_total, "brick_samples": samples_total, "F_geo": F_GEO,
+                              "F_sample": F_SAMPLE, "kernel_ms": plain["kernel_ms"]}},
+        "cpu_baseline": cpu,
+        "error": {"plain_fp32": {"rel_l1_I": err32, "max_abs_QUV": quv32,
+                                 "pole_flagged_frac": float(1 - ok.mean())}},
+        "host": platform.node(),
+    }
+    print(json.dumps(result), flush=True)
+    rr.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

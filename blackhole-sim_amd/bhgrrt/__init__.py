@@ -1,0 +1,11 @@
+"""bhgrrt: MI355X-native Kerr Stokes GRRT hot path (RK2 geodesics, brick sampler, IQUV transfer).
+
+Host code in Python over a ctypes C ABI (include/bhgrrt.h) to hand-written HIP
+kernels for gfx950.  Submodules:
+
+* ``scenes``         -- camera / scene / brick descriptions and seeded generators
+* ``native_kernels`` -- mirror of the reference's native_kernels.py API (FP64 micro-kernels)
+* ``render``         -- fused render API (FP64, FP32, mixed precision)
+"""
+
+__version__ = "0.9.0"

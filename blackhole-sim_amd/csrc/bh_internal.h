@@ -1,0 +1,86 @@
+// Internal declarations shared by the bhgrrt HIP sources (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bhgrrt.h"
+
+namespace bh {
+
+constexpr int kCoeffs = 11;
+constexpr int kStokes = 4;
+constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+constexpr double kEpsTheta = 1.0e-7;   // kerr.py:27
+
+// Thread-local last-error text (bh_last_error).
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+
+#define BH_HIP(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::bh::set_error(BH_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct GridHint {      // index guess u = scale * f(x) + offset, f = log2 or identity
+    float scale;
+    float offset;
+    int log;
+};
+
+struct BrickDev {
+    bool live = false;
+    int storage = BH_STORE_F32;
+    int nr = 0, nt = 0, np = 0;
+    size_t plane = 0;          // nr * nt * np elements
+    void *coeffs = nullptr;    // SoA [11][nr][nt][np]
+    float *grids = nullptr;    // r | theta | phi, float32
+    GridHint hr{}, ht{}, hp{};
+};
+
+constexpr int kMaxQueues = 8;
+constexpr int kCounterWords = 64;
+constexpr int kWorkOffset = 16;   // two uint64 work counters after the queue counters
+
+struct Tuning {
+    int block_threads = 256;
+    int blocks_per_cu = 0;
+    int steps_per_sync = 8;
+    int refill_min = 16;
+    int queues = 8;
+};
+
+}  // namespace bh
+
+struct bh_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    int timed_launches = 0;
+    unsigned int *d_counters = nullptr;   // ray-queue counters, kCounterWords words
+    std::vector<bh::BrickDev> bricks;
+    bh::Tuning tuning[3];
+};
+
+namespace bh {
+int ctx_check(bh_ctx *ctx);
+// Records ev_start on the context stream; end_timing records ev_stop.
+void begin_timing(bh_ctx *ctx);
+void end_timing(bh_ctx *ctx, int launches);
+// Fill n words of device memory with v on the context stream (graph-safe, no memset node).
+int fill_u32(bh_ctx *ctx, unsigned int *dst, unsigned int v, int n);
+// Process-wide context on the current device used by the host-buffer entry points.
+bh_ctx *default_ctx(int *status);
+std::mutex &default_ctx_mutex();
+// Grow-only device staging buffer owned by the default context.
+int staging(size_t bytes, void **ptr);
+}  // namespace bh

@@ -1,0 +1,419 @@
+// FP64 drop-in micro-kernels: stokes_rk2_brick, sample_brick_trilinear,
+// sample_and_step_stokes (+ invalid_policy) and sample_brick_valid_mask.
+//
+// Arithmetic follows native/core/src/lib.rs:37-366 and
+// python/blackhole_sim/native_kernels.py:133-222: searchsorted-right brackets
+// clamped to [0, n-2], the x == grid[-1] special case, the periodic phi seam
+// cell anchored at phi_grid[0], trilinear lerps (1-w)a + wb in phi, theta, r
+// order, and the RK2 Stokes step k1 = f(S), k2 = f(S + ds/2 k1), S + ds k2 with
+// f = j - K S.  One thread per point/cell; the validation mirrors lib.rs
+// (messages included) and runs on the host before anything is copied.
+
+#include "bh_internal.h"
+
+#include <cmath>
+
+namespace bh {
+
+// ---- device arithmetic ----------------------------------------------------
+
+__device__ __forceinline__ int search_right_d(const double *g, int n, double x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (x < g[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// native_kernels.py:137-146; false when outside the closed range.
+__device__ __forceinline__ bool bracket_linear_d(const double *g, int n, double x, int &i0, double &w) {
+    if (x < g[0] || x > g[n - 1]) return false;
+    if (x == g[n - 1]) {
+        i0 = n - 2;
+        w = 1.0;
+        return true;
+    }
+    int i = search_right_d(g, n, x) - 1;
+    i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+    double den = g[i + 1] - g[i];
+    den = (1.0e-300 > den) ? 1.0e-300 : den;
+    i0 = i;
+    w = (x - g[i]) / den;
+    return true;
+}
+
+// Python float %: fmod, shifted into [0, 2 pi) when negative (native_kernels.py:133-134).
+__device__ __forceinline__ double wrap_phi_d(double phi, double base) {
+    double m = fmod(phi - base, kTwoPi);
+    if (m < 0.0) m += kTwoPi;
+    return m + base;
+}
+
+// native_kernels.py:149-161.
+__device__ __forceinline__ void bracket_phi_d(const double *g, int n, double phi, int &i0, int &i1, double &w) {
+    double p = wrap_phi_d(phi, g[0]);
+    int i = search_right_d(g, n, p) - 1;
+    if (i < 0) i = n - 1;
+    if (i >= n) i = n - 1;
+    int j = (i + 1) % n;
+    double hi = (j > i) ? g[j] : g[0] + kTwoPi;
+    double pp = (p >= g[i]) ? p : p + kTwoPi;
+    double den = hi - g[i];
+    den = (1.0e-300 > den) ? 1.0e-300 : den;
+    i0 = i;
+    i1 = j;
+    w = (pp - g[i]) / den;
+}
+
+// lib.rs:171-208 over an AoS FP64 brick.
+__device__ __forceinline__ bool sample_aos_d(const double *__restrict__ c, const double *rg, int nr, const double *tg,
+                                             int nt, const double *pg, int np, double r, double th, double ph,
+                                             double out[kCoeffs]) {
+    int r0, t0, p0, p1;
+    double wr, wt, wp;
+    if (!bracket_linear_d(rg, nr, r, r0, wr)) return false;
+    if (!bracket_linear_d(tg, nt, th, t0, wt)) return false;
+    bracket_phi_d(pg, np, ph, p0, p1, wp);
+    const int r1 = r0 + 1, t1 = t0 + 1;
+    const size_t o000 = ((size_t(r0) * nt + t0) * np + p0) * kCoeffs;
+    const size_t o001 = ((size_t(r0) * nt + t0) * np + p1) * kCoeffs;
+    const size_t o010 = ((size_t(r0) * nt + t1) * np + p0) * kCoeffs;
+    const size_t o011 = ((size_t(r0) * nt + t1) * np + p1) * kCoeffs;
+    const size_t o100 = ((size_t(r1) * nt + t0) * np + p0) * kCoeffs;
+    const size_t o101 = ((size_t(r1) * nt + t0) * np + p1) * kCoeffs;
+    const size_t o110 = ((size_t(r1) * nt + t1) * np + p0) * kCoeffs;
+    const size_t o111 = ((size_t(r1) * nt + t1) * np + p1) * kCoeffs;
+#pragma unroll
+    for (int k = 0; k < kCoeffs; ++k) {
+        double c00 = (1.0 - wp) * c[o000 + k] + wp * c[o001 + k];
+        double c01 = (1.0 - wp) * c[o010 + k] + wp * c[o011 + k];
+        double c10 = (1.0 - wp) * c[o100 + k] + wp * c[o101 + k];
+        double c11 = (1.0 - wp) * c[o110 + k] + wp * c[o111 + k];
+        double c0 = (1.0 - wt) * c00 + wt * c01;
+        double c1 = (1.0 - wt) * c10 + wt * c11;
+        out[k] = (1.0 - wr) * c0 + wr * c1;
+    }
+    return true;
+}
+
+// lib.rs:37-59: j - K S.
+__device__ __forceinline__ void stokes_rhs_d(const double s[4], const double *c, double o[4]) {
+    o[0] = c[0] - (c[4] * s[0] + c[5] * s[1] + c[6] * s[2] + c[7] * s[3]);
+    o[1] = c[1] - (c[5] * s[0] + c[4] * s[1] + c[8] * s[2] - c[10] * s[3]);
+    o[2] = c[2] - (c[6] * s[0] - c[8] * s[1] + c[4] * s[2] + c[9] * s[3]);
+    o[3] = c[3] - (c[7] * s[0] + c[10] * s[1] - c[9] * s[2] + c[4] * s[3]);
+}
+
+// lib.rs:210-225.
+__device__ __forceinline__ void stokes_rk2_d(double s[4], const double *c, double ds) {
+    double k1[4], mid[4], k2[4];
+    stokes_rhs_d(s, c, k1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mid[i] = s[i] + 0.5 * ds * k1[i];
+    stokes_rhs_d(mid, c, k2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = s[i] + ds * k2[i];
+}
+
+// ---- kernels ---------------------------------------------------------------
+
+__global__ void stokes_rk2_kernel(const double *__restrict__ coeffs, int64_t cells, double ds,
+                                  const double *__restrict__ init, int broadcast, double *__restrict__ out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const double *ip = init + (broadcast ? 0 : i * kStokes);
+    double s[4] = {ip[0], ip[1], ip[2], ip[3]};
+    stokes_rk2_d(s, coeffs + i * kCoeffs, ds);
+    double *op = out + i * kStokes;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) op[k] = s[k];
+}
+
+__global__ void sample_kernel(const double *__restrict__ coeffs, const double *__restrict__ rg, int nr,
+                              const double *__restrict__ tg, int nt, const double *__restrict__ pg, int np,
+                              const double *__restrict__ pts, int64_t n, double *__restrict__ out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v[kCoeffs];
+    const double *p = pts + i * 3;
+    double *o = out + i * kCoeffs;
+    if (sample_aos_d(coeffs, rg, nr, tg, nt, pg, np, p[0], p[1], p[2], v)) {
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) o[k] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) o[k] = __builtin_nan("");
+    }
+}
+
+__global__ void sample_step_kernel(const double *__restrict__ coeffs, const double *__restrict__ rg, int nr,
+                                   const double *__restrict__ tg, int nt, const double *__restrict__ pg, int np,
+                                   const double *__restrict__ pts, int64_t n, double ds,
+                                   const double *__restrict__ init, int broadcast, int policy,
+                                   double *__restrict__ out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *p = pts + i * 3;
+    const double *ip = init + (broadcast ? 0 : i * kStokes);
+    double s[4] = {ip[0], ip[1], ip[2], ip[3]};
+    double v[kCoeffs];
+    double *o = out + i * kStokes;
+    if (sample_aos_d(coeffs, rg, nr, tg, nt, pg, np, p[0], p[1], p[2], v)) {
+        stokes_rk2_d(s, v, ds);
+    } else if (policy == BH_POLICY_ZERO) {
+        s[0] = s[1] = s[2] = s[3] = 0.0;
+    } else if (policy == BH_POLICY_NAN) {
+        s[0] = s[1] = s[2] = s[3] = __builtin_nan("");
+    }  // BH_POLICY_INITIAL keeps the initial row
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = s[k];
+}
+
+__global__ void valid_mask_kernel(double r0, double r1, double t0, double t1, const double *__restrict__ pts,
+                                  int64_t n, uint8_t *__restrict__ out) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r = pts[i * 3], th = pts[i * 3 + 1];
+    out[i] = uint8_t((r >= r0) & (r <= r1) & (th >= t0) & (th <= t1));
+}
+
+// ---- host validation (lib.rs:112-169, 228-256, 318-334) --------------------
+
+static bool all_finite(const double *v, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+static int validate_grid(const char *name, const double *g, size_t n) {
+    if (!g || n < 2) return set_error(BH_ERR_VALUE, "%s grid must contain at least 2 values", name);
+    if (!all_finite(g, n)) return set_error(BH_ERR_VALUE, "%s grid must contain only finite values", name);
+    for (size_t i = 1; i < n; ++i)
+        if (!(g[i] > g[i - 1])) return set_error(BH_ERR_VALUE, "%s grid must be strictly increasing", name);
+    if (n > size_t(INT32_MAX)) return set_error(BH_ERR_VALUE, "%s grid is too large", name);
+    return BH_OK;
+}
+
+static int validate_ds(double ds) {
+    if (!std::isfinite(ds) || ds < 0.0) return set_error(BH_ERR_VALUE, "ds_cm must be finite and non-negative");
+    return BH_OK;
+}
+
+static int validate_sampler(const double *coeffs, size_t n_coeffs, const double *rg, size_t nr, const double *tg,
+                            size_t nt, const double *pg, size_t np, const double *pts, size_t n_pts_flat,
+                            size_t *n_points) {
+    int st;
+    if ((st = validate_grid("r", rg, nr))) return st;
+    if ((st = validate_grid("theta", tg, nt))) return st;
+    if ((st = validate_grid("phi", pg, np))) return st;
+    if (n_coeffs && (!coeffs || !all_finite(coeffs, n_coeffs)))
+        return set_error(BH_ERR_VALUE, "coeffs must contain only finite values");
+    size_t expected = 0;
+    if (__builtin_mul_overflow(nr, nt, &expected) || __builtin_mul_overflow(expected, np, &expected) ||
+        __builtin_mul_overflow(expected, size_t(kCoeffs), &expected))
+        return set_error(BH_ERR_VALUE, "coefficient brick shape is too large");
+    if (n_coeffs != expected)
+        return set_error(BH_ERR_VALUE, "coeffs must contain exactly r*theta*phi*11 values; expected %zu, got %zu",
+                         expected, n_coeffs);
+    if (n_pts_flat % 3 != 0) return set_error(BH_ERR_VALUE, "points must contain flat r, theta, phi triples");
+    if (n_pts_flat && (!pts || !all_finite(pts, n_pts_flat)))
+        return set_error(BH_ERR_VALUE, "points must contain only finite values");
+    *n_points = n_pts_flat / 3;
+    return BH_OK;
+}
+
+static inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
+
+// Copies host arrays into one staging allocation; returns device pointers in order.
+struct Stage {
+    char *base = nullptr;
+    size_t off = 0;
+    template <typename T>
+    T *put(bh_ctx *ctx, const T *src, size_t count, int *st) {
+        T *d = reinterpret_cast<T *>(base + off);
+        size_t bytes = count * sizeof(T);
+        off += (bytes + 255) & ~size_t(255);
+        if (count && src) {
+            hipError_t e = hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+            if (e != hipSuccess) *st = set_error(BH_ERR_DEVICE, "hipMemcpyAsync failed: %s", hipGetErrorString(e));
+        }
+        return d;
+    }
+};
+
+static size_t rounded(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+int bh_stokes_rk2_brick(const double *coeffs, size_t n_coeffs, double ds_cm, const double *initial, size_t n_initial,
+                        double *out) {
+    clear_error();
+    int st;
+    if ((st = validate_ds(ds_cm))) return st;
+    if (n_coeffs == 0 || n_coeffs % kCoeffs != 0 || !coeffs)
+        return set_error(BH_ERR_VALUE, "coeffs must contain one or more flat 11-coefficient cells");
+    if (!all_finite(coeffs, n_coeffs)) return set_error(BH_ERR_VALUE, "coeffs must contain only finite values");
+    size_t cells = n_coeffs / kCoeffs;
+    bool bcast = n_initial == size_t(kStokes);
+    if (!bcast && n_initial != cells * kStokes)
+        return set_error(BH_ERR_VALUE, "initial must contain either 4 values or 4 values per coefficient cell");
+    if (!initial || !all_finite(initial, n_initial))
+        return set_error(BH_ERR_VALUE, "initial must contain only finite values");
+    if (!out) return set_error(BH_ERR_VALUE, "out is NULL");
+    std::lock_guard<std::mutex> lock(default_ctx_mutex());
+    bh_ctx *ctx = default_ctx(&st);
+    if (!ctx) return st;
+    void *buf = nullptr;
+    size_t need = rounded(n_coeffs * 8) + rounded(n_initial * 8) + rounded(cells * kStokes * 8);
+    if ((st = staging(need, &buf))) return st;
+    Stage sg{static_cast<char *>(buf)};
+    st = BH_OK;
+    double *dc = sg.put(ctx, coeffs, n_coeffs, &st);
+    double *di = sg.put(ctx, initial, n_initial, &st);
+    double *dout = sg.put<double>(ctx, nullptr, cells * kStokes, &st);
+    if (st) return st;
+    begin_timing(ctx);
+    hipLaunchKernelGGL(stokes_rk2_kernel, dim3(blocks_for(int64_t(cells), 256)), dim3(256), 0, ctx->stream, dc,
+                       int64_t(cells), ds_cm, di, int(bcast), dout);
+    BH_HIP(hipGetLastError());
+    end_timing(ctx, 1);
+    BH_HIP(hipMemcpyAsync(out, dout, cells * kStokes * 8, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    return BH_OK;
+}
+
+int bh_sample_brick_trilinear(const double *coeffs, size_t n_coeffs, const double *r_grid, size_t nr,
+                              const double *theta_grid, size_t nt, const double *phi_grid, size_t np,
+                              const double *points, size_t n_points_flat, double *out) {
+    clear_error();
+    size_t n = 0;
+    int st = validate_sampler(coeffs, n_coeffs, r_grid, nr, theta_grid, nt, phi_grid, np, points, n_points_flat, &n);
+    if (st) return st;
+    if (n == 0) return BH_OK;
+    if (!out) return set_error(BH_ERR_VALUE, "out is NULL");
+    std::lock_guard<std::mutex> lock(default_ctx_mutex());
+    bh_ctx *ctx = default_ctx(&st);
+    if (!ctx) return st;
+    void *buf = nullptr;
+    size_t need = rounded(n_coeffs * 8) + rounded(nr * 8) + rounded(nt * 8) + rounded(np * 8) +
+                  rounded(n_points_flat * 8) + rounded(n * kCoeffs * 8);
+    if ((st = staging(need, &buf))) return st;
+    Stage sg{static_cast<char *>(buf)};
+    double *dc = sg.put(ctx, coeffs, n_coeffs, &st);
+    double *dr = sg.put(ctx, r_grid, nr, &st);
+    double *dt = sg.put(ctx, theta_grid, nt, &st);
+    double *dp = sg.put(ctx, phi_grid, np, &st);
+    double *dpts = sg.put(ctx, points, n_points_flat, &st);
+    double *dout = sg.put<double>(ctx, nullptr, n * kCoeffs, &st);
+    if (st) return st;
+    begin_timing(ctx);
+    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(int64_t(n), 256)), dim3(256), 0, ctx->stream, dc, dr, int(nr),
+                       dt, int(nt), dp, int(np), dpts, int64_t(n), dout);
+    BH_HIP(hipGetLastError());
+    end_timing(ctx, 1);
+    BH_HIP(hipMemcpyAsync(out, dout, n * kCoeffs * 8, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    return BH_OK;
+}
+
+int bh_sample_and_step_stokes(const double *coeffs, size_t n_coeffs, const double *r_grid, size_t nr,
+                              const double *theta_grid, size_t nt, const double *phi_grid, size_t np,
+                              const double *points, size_t n_points_flat, double ds_cm, const double *initial,
+                              size_t n_initial, int invalid_policy, double *out) {
+    clear_error();
+    int st;
+    if ((st = validate_ds(ds_cm))) return st;
+    size_t n = 0;
+    if ((st = validate_sampler(coeffs, n_coeffs, r_grid, nr, theta_grid, nt, phi_grid, np, points, n_points_flat, &n)))
+        return st;
+    bool bcast = n_initial == size_t(kStokes);
+    if (!bcast && n_initial != n * kStokes)
+        return set_error(BH_ERR_VALUE, "initial must contain either 4 values or 4 values per sample point");
+    if (n_initial && (!initial || !all_finite(initial, n_initial)))
+        return set_error(BH_ERR_VALUE, "initial must contain only finite values");
+    if (invalid_policy != BH_POLICY_NAN && invalid_policy != BH_POLICY_INITIAL && invalid_policy != BH_POLICY_ZERO)
+        return set_error(BH_ERR_VALUE, "invalid_policy must be 'nan', 'initial', or 'zero'");
+    if (n == 0) return BH_OK;
+    if (!out) return set_error(BH_ERR_VALUE, "out is NULL");
+    std::lock_guard<std::mutex> lock(default_ctx_mutex());
+    bh_ctx *ctx = default_ctx(&st);
+    if (!ctx) return st;
+    void *buf = nullptr;
+    size_t need = rounded(n_coeffs * 8) + rounded(nr * 8) + rounded(nt * 8) + rounded(np * 8) +
+                  rounded(n_points_flat * 8) + rounded(n_initial * 8) + rounded(n * kStokes * 8);
+    if ((st = staging(need, &buf))) return st;
+    Stage sg{static_cast<char *>(buf)};
+    double *dc = sg.put(ctx, coeffs, n_coeffs, &st);
+    double *dr = sg.put(ctx, r_grid, nr, &st);
+    double *dt = sg.put(ctx, theta_grid, nt, &st);
+    double *dp = sg.put(ctx, phi_grid, np, &st);
+    double *dpts = sg.put(ctx, points, n_points_flat, &st);
+    double *di = sg.put(ctx, initial, n_initial, &st);
+    double *dout = sg.put<double>(ctx, nullptr, n * kStokes, &st);
+    if (st) return st;
+    begin_timing(ctx);
+    hipLaunchKernelGGL(sample_step_kernel, dim3(blocks_for(int64_t(n), 256)), dim3(256), 0, ctx->stream, dc, dr,
+                       int(nr), dt, int(nt), dp, int(np), dpts, int64_t(n), ds_cm, di, int(bcast), invalid_policy,
+                       dout);
+    BH_HIP(hipGetLastError());
+    end_timing(ctx, 1);
+    BH_HIP(hipMemcpyAsync(out, dout, n * kStokes * 8, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    return BH_OK;
+}
+
+int bh_sample_brick_valid_mask(const double *r_grid, size_t nr, const double *theta_grid, size_t nt,
+                               const double *points, size_t n_points_flat, uint8_t *out) {
+    clear_error();
+    int st;
+    if ((st = validate_grid("r", r_grid, nr))) return st;
+    if ((st = validate_grid("theta", theta_grid, nt))) return st;
+    if (n_points_flat % 3 != 0) return set_error(BH_ERR_VALUE, "points must contain r/theta/phi triples");
+    if (n_points_flat && (!points || !all_finite(points, n_points_flat)))
+        return set_error(BH_ERR_VALUE, "points must contain only finite values");
+    size_t n = n_points_flat / 3;
+    if (n == 0) return BH_OK;
+    if (!out) return set_error(BH_ERR_VALUE, "out is NULL");
+    std::lock_guard<std::mutex> lock(default_ctx_mutex());
+    bh_ctx *ctx = default_ctx(&st);
+    if (!ctx) return st;
+    void *buf = nullptr;
+    if ((st = staging(rounded(n_points_flat * 8) + rounded(n), &buf))) return st;
+    Stage sg{static_cast<char *>(buf)};
+    double *dpts = sg.put(ctx, points, n_points_flat, &st);
+    uint8_t *dout = sg.put<uint8_t>(ctx, nullptr, n, &st);
+    if (st) return st;
+    begin_timing(ctx);
+    hipLaunchKernelGGL(valid_mask_kernel, dim3(blocks_for(int64_t(n), 256)), dim3(256), 0, ctx->stream, r_grid[0],
+                       r_grid[nr - 1], theta_grid[0], theta_grid[nt - 1], dpts, int64_t(n), dout);
+    BH_HIP(hipGetLastError());
+    end_timing(ctx, 1);
+    BH_HIP(hipMemcpyAsync(out, dout, n, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    return BH_OK;
+}
+
+int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs, const double *d_r, int32_t nr,
+                                     const double *d_theta, int32_t nt, const double *d_phi, int32_t np,
+                                     const double *d_points, int64_t n, double *d_out) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (nr < 2 || nt < 2 || np < 2) return set_error(BH_ERR_VALUE, "grids must contain at least 2 values");
+    if (n < 0) return set_error(BH_ERR_VALUE, "n must be non-negative");
+    if (n == 0) return BH_OK;
+    begin_timing(ctx);
+    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, d_coeffs, d_r, int(nr),
+                       d_theta, int(nt), d_phi, int(np), d_points, n, d_out);
+    BH_HIP(hipGetLastError());
+    end_timing(ctx, 1);
+    return BH_OK;
+}
+
+}  // extern "C"

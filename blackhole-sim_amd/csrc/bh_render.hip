@@ -1,0 +1,826 @@
+// Fused per-ray Kerr Stokes render: RK2 midpoint geodesic + SoA brick sampler +
+// RK2 Stokes IQUV transfer, one lane per ray, persistent ray queues.
+//
+// Behaviour follows the reference's CPU emulation of the GPU hot loop
+// (python/blackhole_sim/accelerated_renderer.py:100-158) with the RK2 midpoint
+// scheme and termination tests of native/rocm/kerr_stokes_kernel.hip:13, 20
+// and python/blackhole_sim/kerr.py:357-377; the camera launch is kerr.py:188-233.
+//
+// Design (gfx950):
+//  * Hamiltonian in separated Carter form: with E = -p_t, L = p_phi,
+//    P = (r^2 + a^2) E - a L and T = L^2/sin^2 - 2 a E L + a^2 E^2 sin^2,
+//    2 Sigma H = Delta p_r^2 + p_theta^2 + T - P^2/Delta.  Its gradients are
+//    algebraically identical to the reference's -1/2 p (d g^{mu nu}) p
+//    (kerr.py:236-303) at a fraction of the operations: one sincos and three
+//    reciprocals per right-hand side.
+//  * The brick is SoA [11][nr][ntheta][nphi] (f32, bf16 or f16).  Loads are raw
+//    buffer loads: the eight corner offsets are computed once per sample and the
+//    channel plane offset rides in the scalar soffset operand, so a channel costs
+//    one load instruction and no address arithmetic.  Out-of-range offsets read 0.
+//  * The r/theta/phi grids are staged in LDS.  Brackets take an O(1) index guess
+//    (log2 for geometric r grids, affine otherwise) and correct it against the
+//    grid with compares, which reproduces searchsorted(side='right') - 1 exactly.
+//  * Ray scheduling: persistent waves pull rays from per-XCD queues (block b
+//    serves queue b % queues and steals from the others when it runs dry).  Ray
+//    ids are ordered in 8x8 pixel tiles so a wave's lanes trace neighbouring
+//    rays and share cache lines.  Every steps_per_sync steps a wave ballots its
+//    idle lanes, one lane reserves that many ids with a single atomicAdd, and the
+//    idle lanes take consecutive ids by mbcnt rank (wave-level compaction).
+
+#include "bh_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace bh {
+
+constexpr int kMaxGrid = 2048;   // r + theta + phi grid values staged in LDS
+constexpr unsigned kRsrcWord3 = 0x00020000u;
+
+struct CamConst {
+    double tan_x, tan_y, cos_roll, sin_roll;
+    int roll;
+    double e_t0, e_t3, e_r1, e_th2, e_ph3;   // ZAMO tetrad non-zeros (kerr.py:144-148)
+    double g00, g03, g11, g22, g33;          // g_{mu nu} at the camera (kerr.py:67-83)
+    double r, th, ph;
+};
+
+struct RenderArgs {
+    CamConst cam;
+    double a, r_plus, r_cap, step, r_esc, L, tau_max;
+    int W, H, max_steps, adaptive;
+    const void *coeffs;
+    int nr, nt, np;
+    unsigned plane_bytes, total_bytes;
+    const float *grids;
+    GridHint hr, ht, hp;
+    void *iquv;
+    int *steps;
+    uint8_t *status;
+    unsigned *counters;
+    unsigned long long *work;   // [0] RK2 steps, [1] in-brick samples of this launch
+    int tiles_x, n_ids, queues, q_len;
+    int steps_per_sync, refill_min;
+};
+
+// ---- precision traits ------------------------------------------------------
+
+template <typename R>
+struct Mth;
+
+template <>
+struct Mth<float> {
+    static __device__ __forceinline__ void sincos(float x, float &s, float &c) {
+        s = __sinf(x);
+        c = __cosf(x);
+    }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float two_pi() { return 6.28318530717958648f; }
+    static __device__ __forceinline__ float pi_minus_eps() { return float(3.14159265358979323846 - kEpsTheta); }
+    static __device__ __forceinline__ float wrap(float phi, float base) {
+        float p = phi - base;
+        p = p - two_pi() * floorf(p * 0.159154943091895336f);
+        if (p < 0.f) p += two_pi();
+        if (p >= two_pi()) p -= two_pi();
+        return p + base;
+    }
+};
+
+template <>
+struct Mth<double> {
+    static __device__ __forceinline__ void sincos(double x, double &s, double &c) { ::sincos(x, &s, &c); }
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double two_pi() { return kTwoPi; }
+    static __device__ __forceinline__ double pi_minus_eps() { return 3.14159265358979323846 - kEpsTheta; }
+    // Python float % (native_kernels.py:133-134): exact fmod, shifted when negative.
+    static __device__ __forceinline__ double wrap(double phi, double base) {
+        double m = fmod(phi - base, kTwoPi);
+        if (m < 0.0) m += kTwoPi;
+        return m + base;
+    }
+};
+
+// Python max(a, b): b if b > a else a (NaN in a survives).
+template <typename R>
+__device__ __forceinline__ R pymax(R a, R b) { return (b > a) ? b : a; }
+
+// np.clip to [eps, pi - eps], NaN-propagating (kerr.py:35-36).
+template <typename R>
+__device__ __forceinline__ R clamp_theta(R th) {
+    const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
+    return th < lo ? lo : (th > hi ? hi : th);
+}
+
+template <typename R>
+struct RayK {
+    R a, a2, E, L, L2, a2E2, twoaEL, aE, aL, fourE;
+};
+
+// d/dlambda (r, theta, phi, p_r, p_theta) of the Kerr null Hamiltonian.
+template <typename R>
+__device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
+                                         R &dpth) {
+    const R th = clamp_theta(th_in);
+    R s, c;
+    Mth<R>::sincos(th, s, c);
+    R s2 = s * s;
+    s2 = (s2 < R(kEpsTheta * kEpsTheta)) ? R(kEpsTheta * kEpsTheta) : s2;   // kerr.py:93 floor
+    const R inv_s2 = Mth<R>::rcp(s2);
+    const R r2 = r * r;
+    const R r2a2 = r2 + k.a2;
+    const R sig = r2 + k.a2 * (c * c);
+    const R dlt = r2a2 - R(2) * r;
+    const R P = r2a2 * k.E - k.aL;
+    const R inv_sig = Mth<R>::rcp(sig);
+    const R inv_dlt = Mth<R>::rcp(dlt);
+    const R Wd = (P * P) * inv_dlt;
+    const R T = k.L2 * inv_s2 - k.twoaEL + k.a2E2 * s2;
+    const R pr2 = pr * pr;
+    const R N = dlt * pr2 + pth * pth + T - Wd;   // 2 Sigma H
+    dr = dlt * pr * inv_sig;
+    dth = pth * inv_sig;
+    dph = (k.L * inv_s2 - k.aE + k.a * P * inv_dlt) * inv_sig;
+    const R dlt_p = R(2) * r - R(2);
+    const R dNr = dlt_p * pr2 - k.fourE * r * P * inv_dlt + Wd * dlt_p * inv_dlt;
+    dpr = R(-0.5) * inv_sig * (dNr - R(2) * r * N * inv_sig);
+    const R sc = s * c;
+    const R dNth = R(2) * sc * (k.a2E2 - k.L2 * inv_s2 * inv_s2);
+    dpth = R(-0.5) * inv_sig * (dNth + R(2) * k.a2 * sc * N * inv_sig);
+}
+
+// ---- brick sampler -----------------------------------------------------------
+
+template <int STORE>
+__device__ __forceinline__ float load_coeff(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff) {
+    if constexpr (STORE == BH_STORE_F32) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, int(voff), int(soff), 0));
+    } else if constexpr (STORE == BH_STORE_BF16) {
+        unsigned h = __builtin_amdgcn_raw_buffer_load_b16(rs, int(voff), int(soff), 0);
+        return __builtin_bit_cast(float, h << 16);
+    } else {
+        unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(rs, int(voff), int(soff), 0);
+        return float(__builtin_bit_cast(_Float16, h));
+    }
+}
+
+__device__ __forceinline__ int guess_index(float x, GridHint h, int nmax) {
+    float u = h.log ? __builtin_amdgcn_logf(x) * h.scale + h.offset : x * h.scale + h.offset;
+    u = fminf(fmaxf(u, 0.f), float(nmax));   // NaN -> 0
+    return int(u);
+}
+
+// native_kernels.py:137-146 via guess + correction.
+template <typename R>
+__device__ __forceinline__ bool bracket_lin(const R *g, int n, R x, GridHint h, int &i0, R &w) {
+    if (x < g[0] || x > g[n - 1]) return false;
+    int i = guess_index(float(x), h, n - 2);
+    while (i > 0 && x < g[i]) --i;
+    while (i < n - 2 && x >= g[i + 1]) ++i;
+    i0 = i;
+    w = (x == g[n - 1]) ? R(1) : Mth<R>::div(x - g[i], g[i + 1] - g[i]);
+    return true;
+}
+
+// native_kernels.py:149-161 via guess + correction.
+template <typename R>
+__device__ __forceinline__ void bracket_phi(const R *g, int n, R phi, GridHint h, int &i0, int &i1, R &w) {
+    const R p = Mth<R>::wrap(phi, g[0]);
+    int i = guess_index(float(p), h, n - 1);
+    while (i > 0 && p < g[i]) --i;
+    while (i < n - 1 && p >= g[i + 1]) ++i;
+    if (p < g[0]) i = n - 1;
+    const int j = (i + 1 == n) ? 0 : i + 1;
+    const R hi = (j > i) ? g[j] : g[0] + Mth<R>::two_pi();
+    const R pp = (p >= g[i]) ? p : p + Mth<R>::two_pi();
+    i0 = i;
+    i1 = j;
+    w = Mth<R>::div(pp - g[i], hi - g[i]);
+}
+
+// Trilinear sample of all 11 channels at (r, theta, phi); false when outside r/theta.
+// FP64 keeps the reference's nested (1-w)a + wb lerps (native_kernels.py:180-186);
+// FP32 uses the equivalent eight-weight sum.
+template <typename R, int STORE>
+__device__ __forceinline__ bool sample_soa(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
+                                           const R *gp, R x, R y, R z, R out[kCoeffs]) {
+    int r0, t0, p0, p1;
+    R wr, wt, wp;
+    if (!bracket_lin(gr, A.nr, x, A.hr, r0, wr)) return false;
+    if (!bracket_lin(gt, A.nt, y, A.ht, t0, wt)) return false;
+    bracket_phi(gp, A.np, z, A.hp, p0, p1, wp);
+    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+    const unsigned row00 = (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
+    const unsigned row01 = row00 + unsigned(A.np);
+    const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
+    const unsigned row11 = row10 + unsigned(A.np);
+    const unsigned o[8] = {(row00 + p0) * esz, (row00 + p1) * esz, (row01 + p0) * esz, (row01 + p1) * esz,
+                           (row10 + p0) * esz, (row10 + p1) * esz, (row11 + p0) * esz, (row11 + p1) * esz};
+    if constexpr (sizeof(R) == 8) {
+        const R ap = R(1) - wp, at = R(1) - wt, ar = R(1) - wr;
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) {
+            const unsigned so = unsigned(k) * A.plane_bytes;
+            R v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = R(load_coeff<STORE>(rs, o[q], so));
+            const R c00 = ap * v[0] + wp * v[1];
+            const R c01 = ap * v[2] + wp * v[3];
+            const R c10 = ap * v[4] + wp * v[5];
+            const R c11 = ap * v[6] + wp * v[7];
+            const R c0 = at * c00 + wt * c01;
+            const R c1 = at * c10 + wt * c11;
+            out[k] = ar * c0 + wr * c1;
+        }
+    } else {
+        const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
+        const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
+        const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) {
+            const unsigned so = unsigned(k) * A.plane_bytes;
+            R acc = w8[0] * load_coeff<STORE>(rs, o[0], so);
+#pragma unroll
+            for (int q = 1; q < 8; ++q) acc += w8[q] * load_coeff<STORE>(rs, o[q], so);
+            out[k] = acc;
+        }
+    }
+    return true;
+}
+
+// lib.rs:37-59 and 210-225.
+template <typename R>
+__device__ __forceinline__ void stokes_rhs(const R s[4], const R c[kCoeffs], R o[4]) {
+    o[0] = c[0] - (c[4] * s[0] + c[5] * s[1] + c[6] * s[2] + c[7] * s[3]);
+    o[1] = c[1] - (c[5] * s[0] + c[4] * s[1] + c[8] * s[2] - c[10] * s[3]);
+    o[2] = c[2] - (c[6] * s[0] - c[8] * s[1] + c[4] * s[2] + c[9] * s[3]);
+    o[3] = c[3] - (c[7] * s[0] + c[10] * s[1] - c[9] * s[2] + c[4] * s[3]);
+}
+
+template <typename R>
+__device__ __forceinline__ void stokes_rk2(R s[4], const R c[kCoeffs], R ds) {
+    R k1[4], mid[4], k2[4];
+    stokes_rhs(s, c, k1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mid[i] = s[i] + R(0.5) * ds * k1[i];
+    stokes_rhs(mid, c, k2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = s[i] + ds * k2[i];
+}
+
+// ---- camera (kerr.py:188-233), FP64 per ray --------------------------------
+
+__device__ __forceinline__ void camera_ray(const RenderArgs &A, int px, int py, double &pr, double &pth, double &pt,
+                                           double &pph) {
+    const double ndc_x = 2.0 * (double(px) + 0.5) / double(A.W) - 1.0;
+    const double ndc_y = 1.0 - 2.0 * (double(py) + 0.5) / double(A.H);
+    const CamConst &C = A.cam;
+    double n_r = -1.0, n_th = -ndc_y * C.tan_y, n_ph = ndc_x * C.tan_x;
+    if (C.roll) {
+        const double t0 = C.cos_roll * n_th - C.sin_roll * n_ph;
+        const double t1 = C.sin_roll * n_th + C.cos_roll * n_ph;
+        n_th = t0;
+        n_ph = t1;
+    }
+    const double nrm = ::sqrt(n_r * n_r + n_th * n_th + n_ph * n_ph);
+    n_r /= nrm;
+    n_th /= nrm;
+    n_ph /= nrm;
+    const double pc_t = C.e_t0;
+    const double pc_r = n_r * C.e_r1;
+    const double pc_th = n_th * C.e_th2;
+    const double pc_ph = C.e_t3 + n_ph * C.e_ph3;
+    pt = C.g00 * pc_t + C.g03 * pc_ph;
+    pr = C.g11 * pc_r;
+    pth = C.g22 * pc_th;
+    pph = C.g03 * pc_t + C.g33 * pc_ph;
+}
+
+template <typename R>
+struct RayState {
+    R r, th, ph, pr, pth;
+    R S[4];
+    R tau;
+    RayK<R> k;
+    int steps;
+    int pix;
+    bool clamped;
+};
+
+template <typename R>
+__device__ __forceinline__ void ray_init(const RenderArgs &A, int px, int py, RayState<R> &s) {
+    double pr, pth, pt, pph;
+    camera_ray(A, px, py, pr, pth, pt, pph);
+    s.r = R(A.cam.r);
+    s.th = R(A.cam.th);
+    s.ph = R(A.cam.ph);
+    s.pr = R(pr);
+    s.pth = R(pth);
+    const double E = -pt, L = pph, a = A.a;
+    s.k.a = R(a);
+    s.k.a2 = R(a * a);
+    s.k.E = R(E);
+    s.k.L = R(L);
+    s.k.L2 = R(L * L);
+    s.k.a2E2 = R(a * a * E * E);
+    s.k.twoaEL = R(2.0 * a * E * L);
+    s.k.aE = R(a * E);
+    s.k.aL = R(a * L);
+    s.k.fourE = R(4.0 * E);
+    s.S[0] = s.S[1] = s.S[2] = s.S[3] = R(0);
+    s.tau = R(0);
+    s.steps = 0;
+    s.pix = py * A.W + px;
+    s.clamped = false;
+}
+
+__device__ __forceinline__ bool finite_(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool finite_(double x) { return __builtin_isfinite(x); }
+
+template <typename R>
+__device__ __forceinline__ R nan_to_zero(R x) { return __builtin_isfinite(x) ? x : R(0); }
+
+// One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
+template <typename R, int STORE>
+__device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
+                                        const R *gp, RayState<R> &s, unsigned &n_samples) {
+    const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
+    const R h = A.adaptive ? R(A.step) * (s.r - R(A.r_plus)) : R(A.step);
+    R k1r, k1t, k1p, k1pr, k1pt;
+    kerr_rhs(s.k, s.r, s.th, s.pr, s.pth, k1r, k1t, k1p, k1pr, k1pt);
+    const R hh = R(0.5) * h;
+    const R mt = s.th + hh * k1t;
+    R k2r, k2t, k2p, k2pr, k2pt;
+    kerr_rhs(s.k, s.r + hh * k1r, mt, s.pr + hh * k1pr, s.pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    const R nr = s.r + h * k2r;
+    const R nt_raw = s.th + h * k2t;
+    const R nt = clamp_theta(nt_raw);
+    const R np = s.ph + h * k2p;
+    const R npr = s.pr + h * k2pr;
+    const R npt = s.pth + h * k2pt;
+    s.clamped = s.clamped || (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi);
+    const bool fin = finite_(nr) && finite_(nt) && finite_(np) && finite_(npr) && finite_(npt);
+
+    // transfer segment (accelerated_renderer.py:113-125)
+    R c[kCoeffs];
+    if (sample_soa<R, STORE>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt), R(0.5) * (s.ph + np), c)) {
+        const R dx = nr - s.r, dy = nt - s.th, dz = np - s.ph;
+        const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
+        const R ds = pymax(R(A.L) * pymax(dlam, R(1.0e-14)), R(0));
+        stokes_rk2(s.S, c, ds);
+        s.tau = s.tau + pymax(c[4], R(0)) * ds;
+        ++n_samples;
+    }
+    const R r_prev = s.r;
+    s.r = nr;
+    s.th = nt;
+    s.ph = np;
+    s.pr = npr;
+    s.pth = npt;
+    s.steps += 1;
+    if (!fin) return 3;
+    if (nr <= R(A.r_cap)) return 1;
+    if (nr > R(A.r_esc) && nr > r_prev) return 2;
+    if (s.tau >= R(A.tau_max)) return 4;
+    if (s.steps >= A.max_steps) return 0;
+    return -1;
+}
+
+template <typename R>
+__device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R> &s, int code) {
+    R *o = static_cast<R *>(A.iquv) + size_t(s.pix) * 4;
+    o[0] = nan_to_zero(s.S[0]);
+    o[1] = nan_to_zero(s.S[1]);
+    o[2] = nan_to_zero(s.S[2]);
+    o[3] = nan_to_zero(s.S[3]);
+    if (A.steps) A.steps[s.pix] = s.steps;
+    if (A.status) A.status[s.pix] = uint8_t(code | (s.clamped ? 8 : 0));
+}
+
+__device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi(unsigned(mask >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(mask), 0u));
+}
+
+template <typename R, int STORE>
+__global__ __launch_bounds__(256) void render_kernel(RenderArgs A) {
+    __shared__ R s_grid[kMaxGrid];
+    const int ng = A.nr + A.nt + A.np;
+    for (int i = threadIdx.x; i < ng; i += blockDim.x) s_grid[i] = R(A.grids[i]);
+    __syncthreads();
+    const R *gr = s_grid;
+    const R *gt = s_grid + A.nr;
+    const R *gp = gt + A.nt;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(A.coeffs), (short)0, int(A.total_bytes), int(kRsrcWord3));
+
+    const int lane = threadIdx.x & 63;
+    const int q_own = blockIdx.x % A.queues;
+    const unsigned all_q = (1u << A.queues) - 1u;
+    unsigned exhausted = 0;   // wave-uniform
+    bool active = false;
+    RayState<R> s;
+    unsigned n_steps = 0, n_samples = 0;   // this lane's work over all its rays
+
+    for (;;) {
+        unsigned long long idle = __ballot(!active);
+        const int n_idle = __popcll(idle);
+        if (exhausted != all_q && (n_idle >= A.refill_min || n_idle == 64)) {
+            for (int t = 0; t < A.queues && idle; ++t) {
+                const int q = (q_own + t) % A.queues;
+                if (exhausted & (1u << q)) continue;
+                const int need = __popcll(idle);
+                const int leader = __builtin_ctzll(idle);
+                unsigned got = 0;
+                if (lane == leader) got = atomicAdd(&A.counters[q], unsigned(need));
+                got = __builtin_amdgcn_readlane(got, leader);
+                const int q_begin = q * A.q_len;
+                const int q_end = min(q_begin + A.q_len, A.n_ids);
+                const long long avail = (long long)q_end - ((long long)q_begin + (long long)got);
+                if (avail < need) exhausted |= (1u << q);
+                if (avail <= 0) continue;
+                const unsigned rank = lane_rank(idle);
+                if (!active && (long long)rank < avail) {
+                    const int id = q_begin + int(got) + int(rank);
+                    const int tile = id >> 6, w = id & 63;
+                    const int px = (tile % A.tiles_x) * 8 + (w & 7);
+                    const int py = (tile / A.tiles_x) * 8 + (w >> 3);
+                    if (px < A.W && py < A.H) {
+                        ray_init(A, px, py, s);
+                        active = true;
+                    }
+                }
+                idle = __ballot(!active);
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted == all_q) break;
+            continue;
+        }
+        for (int k = 0; k < A.steps_per_sync; ++k) {
+            if (active) {
+                const int code = ray_step<R, STORE>(A, rs, gr, gt, gp, s, n_samples);
+                ++n_steps;
+                if (code >= 0) {
+                    ray_store(A, s, code);
+                    active = false;
+                }
+            }
+        }
+    }
+    // wave-reduce the work counters, one atomic per wave
+    unsigned long long ws = n_steps, wsm = n_samples;
+    for (int off = 32; off > 0; off >>= 1) {
+        ws += __shfl_xor(ws, off);
+        wsm += __shfl_xor(wsm, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&A.work[0], ws);
+        atomicAdd(&A.work[1], wsm);
+    }
+}
+
+// ---- host side ------------------------------------------------------------------
+
+static int camera_constants(const bh_scene_desc &d, CamConst &C) {
+    const double a = d.spin;
+    const double th = d.cam_theta < kEpsTheta ? kEpsTheta
+                      : (d.cam_theta > 3.14159265358979323846 - kEpsTheta ? 3.14159265358979323846 - kEpsTheta
+                                                                           : d.cam_theta);
+    const double r = d.cam_r;
+    const double s = std::sin(th), c = std::cos(th);
+    const double sig = r * r + a * a * c * c;
+    const double dlt = r * r - 2.0 * r + a * a;
+    const double A = (r * r + a * a) * (r * r + a * a) - a * a * dlt * s * s;
+    if (!(dlt > 0.0)) return set_error(BH_ERR_VALUE, "ZAMO tetrad must be outside the outer horizon");
+    if (std::fabs(s) < kEpsTheta) return set_error(BH_ERR_VALUE, "ZAMO tetrad is ill-conditioned at the coordinate pole");
+    const double s2 = s * s;
+    C.g00 = -(1.0 - 2.0 * r / sig);
+    C.g03 = -2.0 * a * r * s2 / sig;
+    C.g11 = sig / dlt;
+    C.g22 = sig;
+    C.g33 = A * s2 / sig;
+    const double lapse = std::sqrt(sig * dlt / A);
+    const double omega = 2.0 * a * r / A;
+    C.e_t0 = 1.0 / lapse;
+    C.e_t3 = omega / lapse;
+    C.e_r1 = std::sqrt(dlt / sig);
+    C.e_th2 = 1.0 / std::sqrt(sig);
+    C.e_ph3 = 1.0 / std::sqrt(C.g33);
+    const double deg = 3.14159265358979323846 / 180.0;   // math.radians multiplies by pi/180
+    C.tan_y = std::tan(0.5 * (d.fov_y_degrees * deg));
+    C.tan_x = (double(d.width) / double(d.height)) * C.tan_y;
+    C.roll = d.roll_degrees != 0.0;
+    const double roll = d.roll_degrees * deg;
+    C.cos_roll = std::cos(roll);
+    C.sin_roll = std::sin(roll);
+    C.r = r;
+    C.th = th;
+    C.ph = d.cam_phi;
+    return BH_OK;
+}
+
+static int validate_scene(const bh_scene_desc &d) {
+    if (!std::isfinite(d.spin) || std::fabs(d.spin) >= 1.0)
+        return set_error(BH_ERR_VALUE, "Kerr spin must be finite and satisfy |a| < 1 for this integrator");
+    if (d.width <= 0 || d.height <= 0) return set_error(BH_ERR_VALUE, "width and height must be positive");
+    if (int64_t(d.width) * d.height > (int64_t(1) << 30)) return set_error(BH_ERR_VALUE, "image is too large");
+    if (d.max_steps < 1) return set_error(BH_ERR_VALUE, "max_steps must be at least 1");
+    if (!std::isfinite(d.step) || d.step <= 0.0) return set_error(BH_ERR_VALUE, "step must be finite and positive");
+    if (d.step_mode != 0 && d.step_mode != 1) return set_error(BH_ERR_VALUE, "step_mode must be 0 (fixed) or 1 (adaptive)");
+    if (!std::isfinite(d.cam_r) || !std::isfinite(d.cam_theta) || !std::isfinite(d.cam_phi) ||
+        !std::isfinite(d.fov_y_degrees) || !std::isfinite(d.roll_degrees))
+        return set_error(BH_ERR_VALUE, "camera parameters must be finite");
+    if (!(d.fov_y_degrees > 0.0 && d.fov_y_degrees < 180.0))
+        return set_error(BH_ERR_VALUE, "fov_y_degrees must lie in (0, 180)");
+    if (std::isnan(d.escape_radius) || std::isnan(d.affine_to_length_cm) || std::isnan(d.max_optical_depth))
+        return set_error(BH_ERR_VALUE, "escape_radius, affine_to_length_cm and max_optical_depth must not be NaN");
+    return BH_OK;
+}
+
+static GridHint make_hint(const float *g, int n, bool periodic) {
+    GridHint h{};
+    const double g0 = g[0], gl = g[n - 1];
+    bool geometric = g0 > 0.0;
+    if (geometric && n > 2) {
+        const double q = std::log2(double(g[1]) / g0);
+        for (int i = 1; i < n && geometric; ++i) {
+            double qi = std::log2(double(g[i]) / double(g[i - 1]));
+            if (std::fabs(qi - q) > 1e-3 * std::fabs(q)) geometric = false;
+        }
+        if (std::fabs(q) < 1e-12) geometric = false;
+    }
+    if (periodic) geometric = false;
+    if (geometric) {
+        const double l0 = std::log2(g0), l1 = std::log2(gl);
+        h.log = 1;
+        h.scale = float((n - 1) / (l1 - l0));
+        h.offset = float(-l0 * (n - 1) / (l1 - l0));
+    } else {
+        h.log = 0;
+        const double span = periodic ? (gl - g0) * double(n) / double(n - 1) : (gl - g0);
+        const double cells = periodic ? double(n) : double(n - 1);
+        h.scale = float(cells / span);
+        h.offset = float(-g0 * cells / span);
+    }
+    return h;
+}
+
+template <typename T>
+static void transpose_soa(const float *aos, int nr, int nt, int np, T *soa, T (*cvt)(float)) {
+    const size_t plane = size_t(nr) * nt * np;
+    for (size_t e = 0; e < plane; ++e)
+        for (int k = 0; k < kCoeffs; ++k) soa[size_t(k) * plane + e] = cvt(aos[e * kCoeffs + k]);
+}
+
+static float cvt_f32(float x) { return x; }
+static uint16_t cvt_bf16(float x) {   // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return uint16_t(u >> 16);
+}
+static uint16_t cvt_f16(float x) {
+    _Float16 h = (_Float16)x;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+typedef void (*RenderFn)(RenderArgs);
+
+template <typename R, int STORE>
+static RenderFn kernel_ptr() { return render_kernel<R, STORE>; }
+
+static RenderFn pick_kernel(int precision, int storage) {
+    if (precision == BH_PREC_FP64) return storage == BH_STORE_F32 ? kernel_ptr<double, BH_STORE_F32>() : nullptr;
+    switch (storage) {
+        case BH_STORE_F32: return kernel_ptr<float, BH_STORE_F32>();
+        case BH_STORE_BF16: return kernel_ptr<float, BH_STORE_BF16>();
+        case BH_STORE_F16: return kernel_ptr<float, BH_STORE_F16>();
+    }
+    return nullptr;
+}
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, int32_t nr, const float *theta_grid,
+                    int32_t nt, const float *phi_grid, int32_t np, int32_t storage, int32_t *brick_id) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!coeffs_aos || !r_grid || !theta_grid || !phi_grid || !brick_id)
+        return set_error(BH_ERR_VALUE, "brick arrays must not be NULL");
+    if (nr < 2 || nt < 2 || np < 2) return set_error(BH_ERR_VALUE, "brick grids must contain at least 2 values");
+    if (nr + nt + np > kMaxGrid) return set_error(BH_ERR_VALUE, "brick grids hold more than %d values", kMaxGrid);
+    if (storage != BH_STORE_F32 && storage != BH_STORE_BF16 && storage != BH_STORE_F16)
+        return set_error(BH_ERR_VALUE, "storage must be BH_STORE_F32, BH_STORE_BF16 or BH_STORE_F16");
+    const float *gs[3] = {r_grid, theta_grid, phi_grid};
+    const int ns[3] = {nr, nt, np};
+    const char *names[3] = {"r", "theta", "phi"};
+    for (int k = 0; k < 3; ++k) {
+        for (int i = 0; i < ns[k]; ++i) {
+            if (!std::isfinite(gs[k][i])) return set_error(BH_ERR_VALUE, "%s grid must contain only finite values", names[k]);
+            if (i && !(gs[k][i] > gs[k][i - 1])) return set_error(BH_ERR_VALUE, "%s grid must be strictly increasing", names[k]);
+        }
+    }
+    const size_t plane = size_t(nr) * nt * np;
+    const size_t esz = storage == BH_STORE_F32 ? 4 : 2;
+    if (plane * kCoeffs * esz >= (size_t(1) << 31)) return set_error(BH_ERR_VALUE, "coefficient brick is too large");
+    for (size_t i = 0; i < plane * kCoeffs; ++i)
+        if (!std::isfinite(coeffs_aos[i])) return set_error(BH_ERR_VALUE, "coefficient brick contains non-finite values");
+    std::vector<unsigned char> host(plane * kCoeffs * esz);
+    if (storage == BH_STORE_F32) transpose_soa<float>(coeffs_aos, nr, nt, np, reinterpret_cast<float *>(host.data()), cvt_f32);
+    else if (storage == BH_STORE_BF16) transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16);
+    else transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16);
+    BrickDev b;
+    b.storage = storage;
+    b.nr = nr;
+    b.nt = nt;
+    b.np = np;
+    b.plane = plane;
+    if (hipMalloc(&b.coeffs, host.size()) != hipSuccess) return set_error(BH_ERR_MEMORY, "brick allocation failed");
+    if (hipMalloc(&b.grids, sizeof(float) * (nr + nt + np)) != hipSuccess) {
+        (void)hipFree(b.coeffs);
+        return set_error(BH_ERR_MEMORY, "brick grid allocation failed");
+    }
+    std::vector<float> g(nr + nt + np);
+    std::memcpy(g.data(), r_grid, sizeof(float) * nr);
+    std::memcpy(g.data() + nr, theta_grid, sizeof(float) * nt);
+    std::memcpy(g.data() + nr + nt, phi_grid, sizeof(float) * np);
+    BH_HIP(hipMemcpy(b.coeffs, host.data(), host.size(), hipMemcpyHostToDevice));
+    BH_HIP(hipMemcpy(b.grids, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice));
+    b.hr = make_hint(r_grid, nr, false);
+    b.ht = make_hint(theta_grid, nt, false);
+    b.hp = make_hint(phi_grid, np, true);
+    b.live = true;
+    ctx->bricks.push_back(b);
+    *brick_id = int32_t(ctx->bricks.size() - 1);
+    return BH_OK;
+}
+
+int bh_brick_release(bh_ctx *ctx, int32_t brick_id) {
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
+        return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    BrickDev &b = ctx->bricks[brick_id];
+    (void)hipFree(b.coeffs);
+    (void)hipFree(b.grids);
+    b = BrickDev();
+    return BH_OK;
+}
+
+int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples) {
+    int st = ctx_check(ctx);
+    if (st) return st;
+    unsigned long long w[2] = {0, 0};
+    BH_HIP(hipMemcpyAsync(w, ctx->d_counters + kWorkOffset, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    if (steps) *steps = w[0];
+    if (samples) *samples = w[1];
+    return BH_OK;
+}
+
+int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t) {
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (precision < 0 || precision > 2 || !t) return set_error(BH_ERR_VALUE, "bad precision or tuning");
+    Tuning u = ctx->tuning[precision];
+    if (t->block_threads) u.block_threads = t->block_threads;
+    if (t->blocks_per_cu >= 0) u.blocks_per_cu = t->blocks_per_cu;
+    if (t->steps_per_sync) u.steps_per_sync = t->steps_per_sync;
+    if (t->refill_min) u.refill_min = t->refill_min;
+    if (t->queues) u.queues = t->queues;
+    if (u.block_threads < 64 || u.block_threads > 256 || u.block_threads % 64)
+        return set_error(BH_ERR_VALUE, "block_threads must be 64, 128, 192 or 256");
+    if (u.steps_per_sync < 1 || u.steps_per_sync > 4096) return set_error(BH_ERR_VALUE, "steps_per_sync out of range");
+    if (u.refill_min < 1 || u.refill_min > 64) return set_error(BH_ERR_VALUE, "refill_min must lie in 1..64");
+    if (u.queues < 1 || u.queues > kMaxQueues) return set_error(BH_ERR_VALUE, "queues must lie in 1..%d", kMaxQueues);
+    if (u.blocks_per_cu > 64) return set_error(BH_ERR_VALUE, "blocks_per_cu out of range");
+    ctx->tuning[precision] = u;
+    return BH_OK;
+}
+
+int bh_ctx_get_tuning(bh_ctx *ctx, int32_t precision, bh_tuning *t) {
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (precision < 0 || precision > 2 || !t) return set_error(BH_ERR_VALUE, "bad precision or tuning");
+    const Tuning &u = ctx->tuning[precision];
+    t->block_threads = u.block_threads;
+    t->blocks_per_cu = u.blocks_per_cu;
+    t->steps_per_sync = u.steps_per_sync;
+    t->refill_min = u.refill_min;
+    t->queues = u.queues;
+    return BH_OK;
+}
+
+int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision, void *d_iquv,
+                     int32_t *d_steps, uint8_t *d_status) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!scene || !d_iquv) return set_error(BH_ERR_VALUE, "scene and iquv must not be NULL");
+    if ((st = validate_scene(*scene))) return st;
+    if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
+        return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
+    if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32)
+        return set_error(BH_ERR_VALUE, "precision %d is not available in this entry point", precision);
+    const BrickDev &b = ctx->bricks[brick_id];
+    RenderFn fn = pick_kernel(precision, b.storage);
+    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick");
+    RenderArgs A{};
+    if ((st = camera_constants(*scene, A.cam))) return st;
+    const double a = scene->spin;
+    A.a = a;
+    A.r_plus = 1.0 + std::sqrt(1.0 - a * a);
+    A.r_cap = A.r_plus * (1.0 + 2.0e-4);
+    A.step = scene->step;
+    A.adaptive = scene->step_mode;
+    A.r_esc = scene->escape_radius;
+    A.L = scene->affine_to_length_cm;
+    A.tau_max = scene->max_optical_depth;
+    A.W = scene->width;
+    A.H = scene->height;
+    A.max_steps = scene->max_steps;
+    A.coeffs = b.coeffs;
+    A.nr = b.nr;
+    A.nt = b.nt;
+    A.np = b.np;
+    const unsigned esz = b.storage == BH_STORE_F32 ? 4u : 2u;
+    A.plane_bytes = unsigned(b.plane * esz);
+    A.total_bytes = unsigned(b.plane * esz * kCoeffs);
+    A.grids = b.grids;
+    A.hr = b.hr;
+    A.ht = b.ht;
+    A.hp = b.hp;
+    A.iquv = d_iquv;
+    A.steps = d_steps;
+    A.status = d_status;
+    A.counters = ctx->d_counters;
+    A.work = reinterpret_cast<unsigned long long *>(ctx->d_counters + kWorkOffset);
+    const Tuning &u = ctx->tuning[precision];
+    A.tiles_x = (A.W + 7) / 8;
+    const int tiles_y = (A.H + 7) / 8;
+    const int n_tiles = A.tiles_x * tiles_y;
+    A.n_ids = n_tiles * 64;
+    A.queues = u.queues < n_tiles ? u.queues : n_tiles;
+    A.q_len = ((n_tiles + A.queues - 1) / A.queues) * 64;
+    A.steps_per_sync = u.steps_per_sync;
+    A.refill_min = u.refill_min;
+    int per_cu = u.blocks_per_cu;
+    if (per_cu <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, u.block_threads, 0) != hipSuccess || nb <= 0) nb = 1;
+        per_cu = nb;
+    }
+    int grid = ctx->cus * per_cu;
+    const int max_useful = (A.n_ids + u.block_threads - 1) / u.block_threads;
+    if (grid > max_useful) grid = max_useful;
+    if (grid < 1) grid = 1;
+    if ((st = fill_u32(ctx, ctx->d_counters, 0u, kCounterWords))) return st;
+    begin_timing(ctx);
+    void *args[] = {&A};
+    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(fn), dim3(grid), dim3(u.block_threads), args, 0, ctx->stream));
+    end_timing(ctx, 1);
+    return BH_OK;
+}
+
+int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision, void *iquv, int32_t *steps,
+              uint8_t *status) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!scene || !iquv) return set_error(BH_ERR_VALUE, "scene and iquv must not be NULL");
+    if ((st = validate_scene(*scene))) return st;
+    const size_t n = size_t(scene->width) * scene->height;
+    const size_t esz = precision == BH_PREC_FP64 ? 8 : 4;
+    void *d = nullptr;
+    if (hipMalloc(&d, n * 4 * esz + n * 4 + n + 1024) != hipSuccess)
+        return set_error(BH_ERR_MEMORY, "render output allocation failed");
+    char *base = static_cast<char *>(d);
+    void *d_iquv = base;
+    int32_t *d_steps = reinterpret_cast<int32_t *>(base + ((n * 4 * esz + 255) & ~size_t(255)));
+    uint8_t *d_status = reinterpret_cast<uint8_t *>(reinterpret_cast<char *>(d_steps) + ((n * 4 + 255) & ~size_t(255)));
+    st = bh_render_device(ctx, scene, brick_id, precision, d_iquv, d_steps, d_status);
+    if (st == BH_OK) {
+        hipError_t e = hipMemcpyAsync(iquv, d_iquv, n * 4 * esz, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess && steps) e = hipMemcpyAsync(steps, d_steps, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess && status) e = hipMemcpyAsync(status, d_status, n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = set_error(BH_ERR_DEVICE, "render readback failed: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+/*
+ * bhgrrt.h -- C ABI of the MI355X (gfx950) Kerr Stokes hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (BH_OK == 0, negative on error); bh_last_error() returns the calling
+ * thread's message for the last failure.  Messages of the drop-in
+ * micro-kernels are the reference's ValueError texts.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   blackhole_native (PyO3 module, native/core/src/lib.rs:368-376), called from
+ *   python/blackhole_sim/native_kernels.py:252-276, 302-322, 359-407 and
+ *   python/blackhole_sim/benchmark.py:135-151, 238-247.
+ */
+#ifndef BHGRRT_H
+#define BHGRRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BH_OK            0
+#define BH_ERR_VALUE    -1   /* invalid argument: the reference raises ValueError */
+#define BH_ERR_DEVICE   -2   /* HIP runtime failure or no usable GPU */
+#define BH_ERR_MEMORY   -3   /* device allocation failed */
+#define BH_ERR_STATE    -4   /* bad handle / object state */
+
+/* invalid_policy of native_kernels.py:16, 127-130, 392-396 */
+#define BH_POLICY_NAN      0
+#define BH_POLICY_INITIAL  1
+#define BH_POLICY_ZERO     2
+
+/* ---- module identity ----------------------------------------------------
+ * bh_core_version replaces core_version() (lib.rs:19-22).
+ * bh_detect_arch replaces detect_arch() (lib.rs:24-35): writes
+ * "arch=<a>;os=<o>;family=<f>;pointer_width=<n>" into buf.
+ */
+const char *bh_core_version(void);
+int bh_detect_arch(char *buf, size_t buflen);
+const char *bh_last_error(void);
+int bh_device_count(int *count);
+int bh_device_name(int device, char *buf, size_t buflen, int *compute_units);
+
+/* ---- drop-in FP64 micro-kernels (host buffers, synchronous) ------------- */
+
+/* stokes_rk2_brick(coeffs, ds_cm, initial) -- lib.rs:227-279.
+ * coeffs: n_coeffs = 11 * cells AoS values; initial: 4 or 4 * cells values;
+ * out: 4 * cells values. */
+int bh_stokes_rk2_brick(const double *coeffs, size_t n_coeffs, double ds_cm,
+                        const double *initial, size_t n_initial, double *out);
+
+/* sample_brick_trilinear(coeffs, r_grid, theta_grid, phi_grid, points) -- lib.rs:281-306.
+ * coeffs: AoS (nr, nt, np, 11); points: n_points_flat = 3 * n values;
+ * out: 11 * n values, NaN rows outside the r/theta domain. */
+int bh_sample_brick_trilinear(const double *coeffs, size_t n_coeffs,
+                              const double *r_grid, size_t nr,
+                              const double *theta_grid, size_t nt,
+                              const double *phi_grid, size_t np,
+                              const double *points, size_t n_points_flat,
+                              double *out);
+
+/* sample_and_step_stokes(coeffs, grids, points, ds_cm, initial) -- lib.rs:308-366,
+ * with the wrapper's invalid_policy applied on the device
+ * (native_kernels.py:373-397).  out: 4 * n values. */
+int bh_sample_and_step_stokes(const double *coeffs, size_t n_coeffs,
+                              const double *r_grid, size_t nr,
+                              const double *theta_grid, size_t nt,
+                              const double *phi_grid, size_t np,
+                              const double *points, size_t n_points_flat,
+                              double ds_cm, const double *initial, size_t n_initial,
+                              int invalid_policy, double *out);
+
+/* sample_brick_valid_mask(r_grid, theta_grid, points) -- native_kernels.py:189-210.
+ * out: n bytes, 1 where r and theta lie inside the closed grid ranges. */
+int bh_sample_brick_valid_mask(const double *r_grid, size_t nr,
+                               const double *theta_grid, size_t nt,
+                               const double *points, size_t n_points_flat,
+                               uint8_t *out);
+
+/* ---- device contexts ----------------------------------------------------
+ * A context owns one HIP stream on one device, uploaded bricks, a small
+ * ray-queue counter block and timing events.  Calls on one context are not
+ * thread-safe; use one context per host thread.
+ */
+typedef struct bh_ctx bh_ctx;
+
+int bh_ctx_create(int device, bh_ctx **out);
+int bh_ctx_destroy(bh_ctx *ctx);
+int bh_ctx_synchronize(bh_ctx *ctx);
+/* average duration (ms) of the kernels launched by the last render/sampler call */
+int bh_ctx_last_kernel_ms(bh_ctx *ctx, double *ms);
+
+int bh_device_alloc(bh_ctx *ctx, size_t bytes, void **ptr);
+int bh_device_free(bh_ctx *ctx, void *ptr);
+int bh_copy_to_device(bh_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bh_copy_to_host(bh_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* Device-resident FP64 sampler over validated inputs (no host copies, no
+ * allocation): same arithmetic as bh_sample_brick_trilinear.  Used for the
+ * large-N sampler measurement. */
+int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs,
+                                     const double *d_r, int32_t nr,
+                                     const double *d_theta, int32_t nt,
+                                     const double *d_phi, int32_t np,
+                                     const double *d_points, int64_t n,
+                                     double *d_out);
+
+/* ---- coefficient bricks (CoefficientBrickGrid, coefficient_bricks.py:28-75) */
+#define BH_STORE_F32   0
+#define BH_STORE_BF16  1
+#define BH_STORE_F16   2
+
+/* Upload an AoS float32 brick (nr, nt, np, 11) with float32 grids; it is
+ * stored on the device as SoA [11][nr][nt][np] in the requested type. */
+int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos,
+                    const float *r_grid, int32_t nr,
+                    const float *theta_grid, int32_t nt,
+                    const float *phi_grid, int32_t np,
+                    int32_t storage, int32_t *brick_id);
+int bh_brick_release(bh_ctx *ctx, int32_t brick_id);
+
+/* ---- fused render (render_stokes_image_bricks, accelerated_renderer.py:100-158,
+ *      and the GPU asset's kerr_stokes_render_kernel, native/rocm/kerr_stokes_kernel.hip:20) */
+typedef struct bh_scene_desc {
+    double spin;              /* a, |a| < 1 */
+    double cam_r;             /* LocalCamera (kerr.py:155-185) */
+    double cam_theta;
+    double cam_phi;
+    double fov_y_degrees;
+    double roll_degrees;
+    int32_t width;
+    int32_t height;
+    int32_t step_mode;        /* 0: fixed affine step; 1: h = step * (r - r_+) */
+    int32_t max_steps;
+    double step;
+    double escape_radius;     /* AcceleratedRenderConfig.escape_radius */
+    double affine_to_length_cm;
+    double max_optical_depth; /* PolarizedTransferConfig.max_optical_depth */
+} bh_scene_desc;
+
+#define BH_PREC_FP64   0   /* full FP64 render (reference-grade) */
+#define BH_PREC_FP32   1   /* plain FP32 geodesic + transfer */
+#define BH_PREC_MIXED  2   /* FP32 with FP64 islands; BF16/FP16 bricks */
+
+/* Per-pixel status byte: low 3 bits 0 max_steps, 1 captured, 2 escaped,
+ * 3 invalid, 4 optically thick; bit 3 set when the theta clamp fired. */
+
+/* Launch shape of the persistent ray-queue kernels (0 fields = defaults). */
+typedef struct bh_tuning {
+    int32_t block_threads;    /* multiple of 64 */
+    int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu (0: occupancy) */
+    int32_t steps_per_sync;   /* steps between wave refill checks */
+    int32_t refill_min;       /* refill when at least this many lanes are idle (1..64) */
+    int32_t queues;           /* 1, or 8 for one ray queue per XCD */
+} bh_tuning;
+
+int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t);
+
+/* Work done by the last render launch on this context: total RK2 steps and
+ * total transfer segments that sampled inside the brick (waits for it). */
+int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples);
+int bh_ctx_get_tuning(bh_ctx *ctx, int32_t precision, bh_tuning *t);
+
+/* Host outputs: iquv is H*W*4 doubles for BH_PREC_FP64, floats otherwise;
+ * steps H*W int32; status H*W bytes (either may be NULL). */
+int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id,
+              int32_t precision, void *iquv, int32_t *steps, uint8_t *status);
+/* Device outputs, enqueued on the context stream (bh_ctx_synchronize to wait). */
+int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id,
+                     int32_t precision, void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BHGRRT_H */
