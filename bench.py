@@ -38,6 +38,9 @@ F_GEO = 155.0
 F_SAMPLE = 298.0
 PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed); scalar issue caps at half of it
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the plain FP32 kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled x2.0 / x1.0 by a 256 MiB copy); profiles/render_kerr_stokes_1024.md
+TRAFFIC_PLAIN_FP32 = 1.83e9 + 27.6e6
 
 
 def log(msg: str) -> None:
@@ -111,7 +114,7 @@ def main(argv=None) -> int:
         return 2
 
     from bhgrrt import scenes
-    from bhgrrt.render import PREC_FP32, PREC_FP64, Renderer
+    from bhgrrt.render import PREC_FP32, PREC_FP64, PREC_MIXED, Renderer
 
     scene = scenes.scale_plain_scene(args.size, args.size)
     brick = scenes.scale_brick()
@@ -126,12 +129,14 @@ def main(argv=None) -> int:
 
     log("[bench] FP64 reference render")
     ref = rr.render(scene, db32, PREC_FP64)
-    ok = (ref["status"] & 8) == 0
     I64 = ref["iquv"][..., 0]
 
-    log("[bench] plain FP32 path")
-    plain = time_path(rr, scene, db32, PREC_FP32, bufs, args.steps, args.warmup)
-    out32 = rr.render(scene, db32, PREC_FP32)
+    log("[bench] plain FP32 path (corner-packed f32 brick)")
+    dbc = rr.upload(brick, "f32", "corner")
+    plain = time_path(rr, scene, dbc, PREC_FP32, bufs, args.steps, args.warmup)
+    out32 = rr.render(scene, dbc, PREC_FP32)
+    # rays flagged (status bit 3) in either render left the geodesic at a pole: excluded, counted
+    ok = ((ref["status"] | out32["status"]) & 8) == 0
     err32 = rel_l1(out32["iquv"][..., 0][ok].astype(np.float64), I64[ok])
     quv32 = float(np.max(np.abs(out32["iquv"][..., 1:][ok].astype(np.float64) - ref["iquv"][..., 1:][ok])))
     steps_total, samples_total = plain["work"]
@@ -140,6 +145,28 @@ def main(argv=None) -> int:
     value = n / (plain["ms_per_step"] * 1e-3) / 1e6
     log(f"[bench] plain fp32: {plain['ms_per_step']:.3f} ms/render, kernel {plain['kernel_ms']:.3f} ms, "
         f"{value:.1f} Mrays/s, rel L1 I {err32:.2e}")
+
+    paths = {"plain_fp32": {"mrays_s": value, "ms_per_render": plain["ms_per_step"], "kernel_ms": plain["kernel_ms"],
+                            "rel_l1_I": err32, "max_abs_QUV": quv32, "tflops": achieved, "layout": "corner"}}
+    log("[bench] plain FP32 path (SoA f32 brick)")
+    soa = time_path(rr, scene, db32, PREC_FP32, bufs, args.steps, args.warmup)
+    paths["plain_fp32_soa"] = {"mrays_s": n / (soa["ms_per_step"] * 1e-3) / 1e6, "ms_per_render": soa["ms_per_step"],
+                               "kernel_ms": soa["kernel_ms"], "layout": "soa"}
+    for storage in ("bf16", "f16"):
+        log(f"[bench] mixed precision path, {storage} brick")
+        dbx = rr.upload(brick, storage, "corner")
+        mx = time_path(rr, scene, dbx, PREC_MIXED, bufs, args.steps, args.warmup)
+        outx = rr.render(scene, dbx, PREC_MIXED)
+        okx = ((ref["status"] | outx["status"]) & 8) == 0
+        st, sm = mx["work"]
+        paths[f"mixed_{storage}"] = {
+            "mrays_s": n / (mx["ms_per_step"] * 1e-3) / 1e6, "ms_per_render": mx["ms_per_step"],
+            "kernel_ms": mx["kernel_ms"],
+            "rel_l1_I": rel_l1(outx["iquv"][..., 0][okx].astype(np.float64), I64[okx]),
+            "max_abs_QUV": float(np.max(np.abs(outx["iquv"][..., 1:][okx].astype(np.float64) - ref["iquv"][..., 1:][okx]))),
+            "pole_flagged_frac": float(1 - okx.mean()), "rk2_steps": st, "brick_samples": sm}
+        log(f"[bench] mixed {storage}: {paths[f'mixed_{storage}']}")
+        rr.release(dbx)
 
     cpu = None
     if not args.no_cpu:
@@ -160,15 +187,18 @@ def main(argv=None) -> int:
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"plain FP32 RK2 Kerr Stokes render, a=0.94, i=17deg, {scene.width}x{scene.height} "
-                               f"over +-20M, h=0.05(r-r+), <=4096 steps, brick 128x64x128x11 f32 (thin disk)",
+                               f"over +-20M, h=0.05(r-r+), <=4096 steps, brick 128x64x128x11 f32 (thin disk), "
+                               f"corner-packed layout",
                    "path": "plain_fp32", "rays": n},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": TRAFFIC_PLAIN_FP32 if args.size == 1024 else None,
                      "work": {"rk2_steps": steps_total, "brick_samples": samples_total, "F_geo": F_GEO,
                               "F_sample": F_SAMPLE, "kernel_ms": plain["kernel_ms"]}},
         "cpu_baseline": cpu,
         "error": {"plain_fp32": {"rel_l1_I": err32, "max_abs_QUV": quv32,
                                  "pole_flagged_frac": float(1 - ok.mean())}},
+        "paths": paths,
         "host": platform.node(),
     }
     print(json.dumps(result), flush=True)

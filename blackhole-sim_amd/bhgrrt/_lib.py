@@ -19,6 +19,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libbhgrrt.so"
 BH_OK, BH_ERR_VALUE, BH_ERR_DEVICE, BH_ERR_MEMORY, BH_ERR_STATE = 0, -1, -2, -3, -4
 POLICY = {"nan": 0, "initial": 1, "zero": 2}
 STORE = {"f32": 0, "bf16": 1, "f16": 2}
+LAYOUT = {"soa": 0, "corner": 1}
 PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
 
 _dp = C.POINTER(C.c_double)
@@ -41,7 +42,7 @@ class SceneDesc(C.Structure):
 class Tuning(C.Structure):
     """bh_tuning."""
     _fields_ = [("block_threads", C.c_int32), ("blocks_per_cu", C.c_int32), ("steps_per_sync", C.c_int32),
-                ("refill_min", C.c_int32), ("queues", C.c_int32)]
+                ("refill_min", C.c_int32), ("queues", C.c_int32), ("order", C.c_int32)]
 
 
 # (name, restype, argtypes) for every symbol include/bhgrrt.h declares
@@ -66,11 +67,12 @@ SIGNATURES = [
     ("bh_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("bh_copy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("bh_copy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("bh_copy_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("bh_sample_brick_trilinear_device", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32,
       C.c_void_p, C.c_int64, C.c_void_p]),
     ("bh_brick_upload", C.c_int,
-     [C.c_void_p, _fp, _fp, C.c_int32, _fp, C.c_int32, _fp, C.c_int32, C.c_int32, _i32p]),
+     [C.c_void_p, _fp, _fp, C.c_int32, _fp, C.c_int32, _fp, C.c_int32, C.c_int32, C.c_int32, _i32p]),
     ("bh_brick_release", C.c_int, [C.c_void_p, C.c_int32]),
     ("bh_ctx_set_tuning", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),
     ("bh_ctx_get_tuning", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),

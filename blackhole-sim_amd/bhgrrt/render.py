@@ -35,6 +35,7 @@ class DeviceBrick:
     brick_id: int
     storage: str
     shape: tuple
+    layout: str = "soa"
 
 
 class Renderer:
@@ -65,13 +66,14 @@ class Renderer:
             pass
 
     # --- bricks -------------------------------------------------------------
-    def upload(self, brick: Brick, storage: str = "f32") -> DeviceBrick:
+    def upload(self, brick: Brick, storage: str = "f32", layout: str = "soa") -> DeviceBrick:
+        """Upload an AoS brick; storage f32/bf16/f16, layout soa (channel planes) or corner (per-cell records)."""
         c = np.ascontiguousarray(brick.coeffs, dtype=np.float32)
         bid = C.c_int32(-1)
         _lib.check(self._lib.bh_brick_upload(self.ctx, _lib.fptr(c), _lib.fptr(brick.r), brick.r.size,
                                              _lib.fptr(brick.theta), brick.theta.size, _lib.fptr(brick.phi),
-                                             brick.phi.size, _lib.STORE[storage], C.byref(bid)))
-        return DeviceBrick(int(bid.value), storage, tuple(c.shape))
+                                             brick.phi.size, _lib.STORE[storage], _lib.LAYOUT[layout], C.byref(bid)))
+        return DeviceBrick(int(bid.value), storage, tuple(c.shape), layout)
 
     def release(self, dbrick: DeviceBrick) -> None:
         _lib.check(self._lib.bh_brick_release(self.ctx, dbrick.brick_id))
@@ -80,7 +82,7 @@ class Renderer:
     def set_tuning(self, precision: int, **kw) -> None:
         t = _lib.Tuning(block_threads=kw.get("block_threads", 0), blocks_per_cu=kw.get("blocks_per_cu", -1),
                         steps_per_sync=kw.get("steps_per_sync", 0), refill_min=kw.get("refill_min", 0),
-                        queues=kw.get("queues", 0))
+                        queues=kw.get("queues", 0), order=kw.get("order", -1))
         _lib.check(self._lib.bh_ctx_set_tuning(self.ctx, int(precision), C.byref(t)))
 
     def get_tuning(self, precision: int) -> dict:

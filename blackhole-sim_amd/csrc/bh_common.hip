@@ -168,6 +168,7 @@ int bh_ctx_destroy(bh_ctx *ctx) {
         if (b.grids) (void)hipFree(b.grids);
     }
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_order) (void)hipFree(ctx->d_order);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -221,6 +222,14 @@ int bh_copy_to_host(bh_ctx *ctx, void *dst, const void *src, size_t bytes) {
     int st = ctx_check(ctx);
     if (st) return st;
     BH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    return BH_OK;
+}
+
+int bh_copy_device(bh_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    int st = ctx_check(ctx);
+    if (st) return st;
+    BH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     return BH_OK;
 }

@@ -39,9 +39,11 @@ struct GridHint {      // index guess u = scale * f(x) + offset, f = log2 or ide
 struct BrickDev {
     bool live = false;
     int storage = BH_STORE_F32;
+    int layout = BH_LAYOUT_SOA;
     int nr = 0, nt = 0, np = 0;
     size_t plane = 0;          // nr * nt * np elements
-    void *coeffs = nullptr;    // SoA [11][nr][nt][np]
+    size_t bytes = 0;          // device bytes of the coefficient array
+    void *coeffs = nullptr;    // SoA [11][nr][nt][np], or corner cells [nr-1][nt-1][np][8][11]
     float *grids = nullptr;    // r | theta | phi, float32
     GridHint hr{}, ht{}, hp{};
 };
@@ -54,8 +56,9 @@ struct Tuning {
     int block_threads = 256;
     int blocks_per_cu = 0;
     int steps_per_sync = 8;
-    int refill_min = 16;
+    int refill_min = 64;
     int queues = 8;
+    int order = 1;        // 1: tiles nearest the critical curve (longest rays) first
 };
 
 }  // namespace bh
@@ -69,6 +72,9 @@ struct bh_ctx {
     unsigned int *d_counters = nullptr;   // ray-queue counters, kCounterWords words
     std::vector<bh::BrickDev> bricks;
     bh::Tuning tuning[3];
+    int *d_order = nullptr;             // tile schedule of the last scene (device)
+    size_t order_cap = 0;
+    std::vector<double> order_key;      // scene parameters the schedule was built for
 };
 
 namespace bh {

@@ -30,12 +30,15 @@
 #include "bh_internal.h"
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 
 namespace bh {
 
 constexpr int kMaxGrid = 2048;   // r + theta + phi grid values staged in LDS
 constexpr unsigned kRsrcWord3 = 0x00020000u;
+constexpr double kIslandTheta = 0.1;    // rad from a pole: FP64 geodesic steps in the mixed path
+constexpr double kIslandPtheta = 0.25;  // |p_theta| below this: near a theta turning point
 
 struct CamConst {
     double tan_x, tan_y, cos_roll, sin_roll;
@@ -45,9 +48,17 @@ struct CamConst {
     double r, th, ph;
 };
 
+struct RenderArgs;
+template <typename R>
+struct HotParams {
+    R step, r_plus, r_cap, r_esc, L, tau_max;
+};
+
 struct RenderArgs {
     CamConst cam;
     double a, r_plus, r_cap, step, r_esc, L, tau_max;
+    double r_island, th_island, pth_island;   // FP64 islands of the mixed-precision path
+    float step_f, r_plus_f, r_cap_f, r_esc_f, L_f, tau_max_f;   // FP32 copies for the hot loop
     int W, H, max_steps, adaptive;
     const void *coeffs;
     int nr, nt, np;
@@ -59,9 +70,16 @@ struct RenderArgs {
     uint8_t *status;
     unsigned *counters;
     unsigned long long *work;   // [0] RK2 steps, [1] in-brick samples of this launch
+    const int *tile_order;      // queue slot -> tile, or null for row-major
     int tiles_x, n_ids, queues, q_len;
     int steps_per_sync, refill_min;
 };
+
+template <typename R>
+__device__ __forceinline__ HotParams<R> hot_params(const RenderArgs &A) {
+    if constexpr (sizeof(R) == 4) return {A.step_f, A.r_plus_f, A.r_cap_f, A.r_esc_f, A.L_f, A.tau_max_f};
+    else return {A.step, A.r_plus, A.r_cap, A.r_esc, A.L, A.tau_max};
+}
 
 // ---- precision traits ------------------------------------------------------
 
@@ -129,14 +147,24 @@ __device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R
     Mth<R>::sincos(th, s, c);
     R s2 = s * s;
     s2 = (s2 < R(kEpsTheta * kEpsTheta)) ? R(kEpsTheta * kEpsTheta) : s2;   // kerr.py:93 floor
-    const R inv_s2 = Mth<R>::rcp(s2);
     const R r2 = r * r;
     const R r2a2 = r2 + k.a2;
     const R sig = r2 + k.a2 * (c * c);
     const R dlt = r2a2 - R(2) * r;
     const R P = r2a2 * k.E - k.aL;
-    const R inv_sig = Mth<R>::rcp(sig);
-    const R inv_dlt = Mth<R>::rcp(dlt);
+    R inv_s2, inv_sig, inv_dlt;
+    if constexpr (sizeof(R) == 4) {
+        // one reciprocal of Sigma * Delta * sin^2 serves all three (|product| stays well inside FP32 range)
+        const R sd = sig * dlt;
+        const R inv = Mth<R>::rcp(sd * s2);
+        inv_s2 = sd * inv;
+        inv_sig = (dlt * s2) * inv;
+        inv_dlt = (sig * s2) * inv;
+    } else {
+        inv_s2 = Mth<R>::rcp(s2);
+        inv_sig = Mth<R>::rcp(sig);
+        inv_dlt = Mth<R>::rcp(dlt);
+    }
     const R Wd = (P * P) * inv_dlt;
     const R T = k.L2 * inv_s2 - k.twoaEL + k.a2E2 * s2;
     const R pr2 = pr * pr;
@@ -178,10 +206,23 @@ template <typename R>
 __device__ __forceinline__ bool bracket_lin(const R *g, int n, R x, GridHint h, int &i0, R &w) {
     if (x < g[0] || x > g[n - 1]) return false;
     int i = guess_index(float(x), h, n - 2);
-    while (i > 0 && x < g[i]) --i;
-    while (i < n - 2 && x >= g[i + 1]) ++i;
+    R lo = g[i], hi = g[i + 1];
+    // the hint is exact or off by one cell for the grids it was built for: one branch-free correction
+    const bool down = (i > 0) && (x < lo);
+    const bool up = (i < n - 2) && (x >= hi);
+    if (down | up) {
+        i += up ? 1 : -1;
+        lo = g[i];
+        hi = g[i + 1];
+        if (((i > 0) && (x < lo)) || ((i < n - 2) && (x >= hi))) {   // any other grid: exact search
+            while (i > 0 && x < g[i]) --i;
+            while (i < n - 2 && x >= g[i + 1]) ++i;
+            lo = g[i];
+            hi = g[i + 1];
+        }
+    }
     i0 = i;
-    w = (x == g[n - 1]) ? R(1) : Mth<R>::div(x - g[i], g[i + 1] - g[i]);
+    w = (x == g[n - 1]) ? R(1) : Mth<R>::div(x - lo, hi - lo);
     return true;
 }
 
@@ -190,8 +231,16 @@ template <typename R>
 __device__ __forceinline__ void bracket_phi(const R *g, int n, R phi, GridHint h, int &i0, int &i1, R &w) {
     const R p = Mth<R>::wrap(phi, g[0]);
     int i = guess_index(float(p), h, n - 1);
-    while (i > 0 && p < g[i]) --i;
-    while (i < n - 1 && p >= g[i + 1]) ++i;
+    // g[n] is a +inf sentinel in LDS, so g[i + 1] is always readable
+    const bool down = (i > 0) && (p < g[i]);
+    const bool up = (i < n - 1) && (p >= g[i + 1]);
+    if (down | up) {
+        i += up ? 1 : -1;
+        if (((i > 0) && (p < g[i])) || ((i < n - 1) && (p >= g[i + 1]))) {
+            while (i > 0 && p < g[i]) --i;
+            while (i < n - 1 && p >= g[i + 1]) ++i;
+        }
+    }
     if (p < g[0]) i = n - 1;
     const int j = (i + 1 == n) ? 0 : i + 1;
     const R hi = (j > i) ? g[j] : g[0] + Mth<R>::two_pi();
@@ -204,15 +253,59 @@ __device__ __forceinline__ void bracket_phi(const R *g, int n, R phi, GridHint h
 // Trilinear sample of all 11 channels at (r, theta, phi); false when outside r/theta.
 // FP64 keeps the reference's nested (1-w)a + wb lerps (native_kernels.py:180-186);
 // FP32 uses the equivalent eight-weight sum.
-template <typename R, int STORE>
-__device__ __forceinline__ bool sample_soa(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
-                                           const R *gp, R x, R y, R z, R out[kCoeffs]) {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float half_lo(unsigned w, int store) {
+    return store == BH_STORE_BF16 ? __builtin_bit_cast(float, w << 16)
+                                  : float(__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu)));
+}
+__device__ __forceinline__ float half_hi(unsigned w, int store) {
+    return store == BH_STORE_BF16 ? __builtin_bit_cast(float, w & 0xffff0000u)
+                                  : float(__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)));
+}
+
+template <typename R, int STORE, int LAYOUT>
+__device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
+                                             const R *gp, R x, R y, R z, R out[kCoeffs]) {
     int r0, t0, p0, p1;
     R wr, wt, wp;
     if (!bracket_lin(gr, A.nr, x, A.hr, r0, wr)) return false;
     if (!bracket_lin(gt, A.nt, y, A.ht, t0, wt)) return false;
     bracket_phi(gp, A.np, z, A.hp, p0, p1, wp);
     constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+    if constexpr (LAYOUT == BH_LAYOUT_CORNER) {
+        // one cell record: corners (dr, dt, dp) in order dr*4 + dt*2 + dp, 11 channels each
+        const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
+        constexpr int kWords = 8 * kCoeffs * int(esz) / 4;   // 88 (f32) or 44 (16-bit)
+        constexpr int kVec = kWords / 4;                      // 22 or 11 loads of 16 bytes
+        const unsigned base = cell * unsigned(8 * kCoeffs * esz);
+        unsigned w[kWords];
+#pragma unroll
+        for (int j = 0; j < kVec; ++j) {
+            const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, int(base + 16u * unsigned(j)), 0, 0);
+            w[4 * j] = q.x;
+            w[4 * j + 1] = q.y;
+            w[4 * j + 2] = q.z;
+            w[4 * j + 3] = q.w;
+        }
+        const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
+        const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
+        const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) {
+            R acc = R(0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = q * kCoeffs + k;
+                float v;
+                if constexpr (STORE == BH_STORE_F32) v = __builtin_bit_cast(float, w[e]);
+                else v = (e & 1) ? half_hi(w[e >> 1], STORE) : half_lo(w[e >> 1], STORE);
+                acc = (q == 0) ? w8[0] * R(v) : acc + w8[q] * R(v);
+            }
+            out[k] = acc;
+        }
+        return true;
+    }
     const unsigned row00 = (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
     const unsigned row01 = row00 + unsigned(A.np);
     const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
@@ -236,15 +329,22 @@ __device__ __forceinline__ bool sample_soa(const RenderArgs &A, __amdgpu_buffer_
             out[k] = ar * c0 + wr * c1;
         }
     } else {
+        // issue all 88 loads before any arithmetic so they are in flight together
+        float v[kCoeffs][8];
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) {
+            const unsigned so = unsigned(k) * A.plane_bytes;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[k][q] = load_coeff<STORE>(rs, o[q], so);
+        }
         const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
 #pragma unroll
         for (int k = 0; k < kCoeffs; ++k) {
-            const unsigned so = unsigned(k) * A.plane_bytes;
-            R acc = w8[0] * load_coeff<STORE>(rs, o[0], so);
+            R acc = w8[0] * v[k][0];
 #pragma unroll
-            for (int q = 1; q < 8; ++q) acc += w8[q] * load_coeff<STORE>(rs, o[q], so);
+            for (int q = 1; q < 8; ++q) acc += w8[q] * v[k][q];
             out[k] = acc;
         }
     }
@@ -305,6 +405,7 @@ struct RayState {
     R S[4];
     R tau;
     RayK<R> k;
+    double Ed, Ld;   // E = -p_t and L = p_phi in FP64 (mixed-precision islands)
     int steps;
     int pix;
     bool clamped;
@@ -330,6 +431,8 @@ __device__ __forceinline__ void ray_init(const RenderArgs &A, int px, int py, Ra
     s.k.aE = R(a * E);
     s.k.aL = R(a * L);
     s.k.fourE = R(4.0 * E);
+    s.Ed = E;
+    s.Ld = L;
     s.S[0] = s.S[1] = s.S[2] = s.S[3] = R(0);
     s.tau = R(0);
     s.steps = 0;
@@ -343,33 +446,82 @@ __device__ __forceinline__ bool finite_(double x) { return __builtin_isfinite(x)
 template <typename R>
 __device__ __forceinline__ R nan_to_zero(R x) { return __builtin_isfinite(x) ? x : R(0); }
 
+template <typename G>
+__device__ __forceinline__ RayK<G> make_k(double a, double E, double L) {
+    RayK<G> k;
+    k.a = G(a);
+    k.a2 = G(a * a);
+    k.E = G(E);
+    k.L = G(L);
+    k.L2 = G(L * L);
+    k.a2E2 = G(a * a * E * E);
+    k.twoaEL = G(2.0 * a * E * L);
+    k.aE = G(a * E);
+    k.aL = G(a * L);
+    k.fourE = G(4.0 * E);
+    return k;
+}
+
+// RK2 midpoint step of the geodesic in precision G: k1 = f(y), k2 = f(y + h/2 k1), y + h k2.
+// Returns the raw (unclamped) end theta in nt_raw and the midpoint theta in mt.
+template <typename G>
+__device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
+                                         G &npr, G &npt, G &mt) {
+    G k1r, k1t, k1p, k1pr, k1pt;
+    kerr_rhs(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    const G hh = G(0.5) * h;
+    mt = th + hh * k1t;
+    G k2r, k2t, k2p, k2pr, k2pt;
+    kerr_rhs(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    nr = r + h * k2r;
+    nt_raw = th + h * k2t;
+    np = ph + h * k2p;
+    npr = pr + h * k2pr;
+    npt = pth + h * k2pt;
+}
+
 // One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
-template <typename R, int STORE>
+// MIXED: the geodesic step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
+// |p_theta| < pth_island near a theta turning point) and the horizon/escape tests compare in FP64.
+template <typename R, int STORE, int LAYOUT, bool MIXED>
 __device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
                                         const R *gp, RayState<R> &s, unsigned &n_samples) {
     const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
-    const R h = A.adaptive ? R(A.step) * (s.r - R(A.r_plus)) : R(A.step);
-    R k1r, k1t, k1p, k1pr, k1pt;
-    kerr_rhs(s.k, s.r, s.th, s.pr, s.pth, k1r, k1t, k1p, k1pr, k1pt);
-    const R hh = R(0.5) * h;
-    const R mt = s.th + hh * k1t;
-    R k2r, k2t, k2p, k2pr, k2pt;
-    kerr_rhs(s.k, s.r + hh * k1r, mt, s.pr + hh * k1pr, s.pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
-    const R nr = s.r + h * k2r;
-    const R nt_raw = s.th + h * k2t;
+    R nr, nt_raw, np, npr, npt, mt;
+    bool island = false;
+    if constexpr (MIXED) {
+        island = (double(s.r) < A.r_island) || (double(s.th) < A.th_island) ||
+                 (double(s.th) > 3.14159265358979323846 - A.th_island) || (fabs(double(s.pth)) < A.pth_island);
+    }
+    if (MIXED && island) {
+        const double hd = A.adaptive ? A.step * (double(s.r) - A.r_plus) : A.step;
+        const RayK<double> kd = make_k<double>(A.a, s.Ed, s.Ld);
+        double dr_, dt_, dp_, dpr_, dpt_, dm_;
+        geo_step<double>(kd, double(s.r), double(s.th), double(s.ph), double(s.pr), double(s.pth), hd, dr_, dt_, dp_,
+                         dpr_, dpt_, dm_);
+        nr = R(dr_);
+        nt_raw = R(dt_);
+        np = R(dp_);
+        npr = R(dpr_);
+        npt = R(dpt_);
+        mt = R(dm_);
+    } else {
+        const HotParams<R> P = hot_params<R>(A);
+        const R h = A.adaptive ? P.step * (s.r - P.r_plus) : P.step;
+        geo_step<R>(s.k, s.r, s.th, s.ph, s.pr, s.pth, h, nr, nt_raw, np, npr, npt, mt);
+    }
     const R nt = clamp_theta(nt_raw);
-    const R np = s.ph + h * k2p;
-    const R npr = s.pr + h * k2pr;
-    const R npt = s.pth + h * k2pt;
-    s.clamped = s.clamped || (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi);
+    // pole pathology: the theta clamp fired, or one step swung phi by more than 1 rad
+    s.clamped = s.clamped || (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi) || (np - s.ph > R(1)) ||
+                (s.ph - np > R(1));
     const bool fin = finite_(nr) && finite_(nt) && finite_(np) && finite_(npr) && finite_(npt);
 
     // transfer segment (accelerated_renderer.py:113-125)
     R c[kCoeffs];
-    if (sample_soa<R, STORE>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt), R(0.5) * (s.ph + np), c)) {
+    if (sample_brick<R, STORE, LAYOUT>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt), R(0.5) * (s.ph + np), c)) {
         const R dx = nr - s.r, dy = nt - s.th, dz = np - s.ph;
         const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
-        const R ds = pymax(R(A.L) * pymax(dlam, R(1.0e-14)), R(0));
+        const R ds = pymax(hot_params<R>(A).L * pymax(dlam, R(1.0e-14)), R(0));
         stokes_rk2(s.S, c, ds);
         s.tau = s.tau + pymax(c[4], R(0)) * ds;
         ++n_samples;
@@ -382,9 +534,14 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsr
     s.pth = npt;
     s.steps += 1;
     if (!fin) return 3;
-    if (nr <= R(A.r_cap)) return 1;
-    if (nr > R(A.r_esc) && nr > r_prev) return 2;
-    if (s.tau >= R(A.tau_max)) return 4;
+    if constexpr (MIXED) {
+        if (double(nr) <= A.r_cap) return 1;
+        if (double(nr) > A.r_esc && nr > r_prev) return 2;
+    } else {
+        if (nr <= hot_params<R>(A).r_cap) return 1;
+        if (nr > hot_params<R>(A).r_esc && nr > r_prev) return 2;
+    }
+    if (s.tau >= hot_params<R>(A).tau_max) return 4;
     if (s.steps >= A.max_steps) return 0;
     return -1;
 }
@@ -404,11 +561,17 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi(unsigned(mask >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(mask), 0u));
 }
 
-template <typename R, int STORE>
-__global__ __launch_bounds__(256) void render_kernel(RenderArgs A) {
-    __shared__ R s_grid[kMaxGrid];
+#ifndef BH_RENDER_WAVES_PER_EU
+#define BH_RENDER_WAVES_PER_EU 2
+#endif
+
+template <typename R, int STORE, int LAYOUT, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void render_kernel(
+    RenderArgs A) {
+    __shared__ R s_grid[kMaxGrid + 1];
     const int ng = A.nr + A.nt + A.np;
     for (int i = threadIdx.x; i < ng; i += blockDim.x) s_grid[i] = R(A.grids[i]);
+    if (threadIdx.x == 0) s_grid[ng] = R(__builtin_huge_valf());   // sentinel g[np] of the phi grid
     __syncthreads();
     const R *gr = s_grid;
     const R *gt = s_grid + A.nr;
@@ -444,7 +607,7 @@ __global__ __launch_bounds__(256) void render_kernel(RenderArgs A) {
                 const unsigned rank = lane_rank(idle);
                 if (!active && (long long)rank < avail) {
                     const int id = q_begin + int(got) + int(rank);
-                    const int tile = id >> 6, w = id & 63;
+                    const int tile = A.tile_order ? A.tile_order[id >> 6] : (id >> 6), w = id & 63;
                     const int px = (tile % A.tiles_x) * 8 + (w & 7);
                     const int py = (tile / A.tiles_x) * 8 + (w >> 3);
                     if (px < A.W && py < A.H) {
@@ -455,13 +618,16 @@ __global__ __launch_bounds__(256) void render_kernel(RenderArgs A) {
                 idle = __ballot(!active);
             }
         }
-        if (!__any(active)) {
+        const int live = __popcll(__ballot(active));
+        if (live == 0) {
             if (exhausted == all_q) break;
             continue;
         }
+        if (live <= 8) __builtin_amdgcn_s_setprio(2);   // a few long rays hold this wave: let them issue first
+        else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < A.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE>(A, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
@@ -522,6 +688,88 @@ static int camera_constants(const bh_scene_desc &d, CamConst &C) {
     return BH_OK;
 }
 
+// Bardeen's critical curve of a distant observer at inclination inc: spherical photon orbits
+// r_p in [r_ph+, r_ph-] give xi = L/E and eta = Q/E^2; alpha = -xi / sin i,
+// beta = +-sqrt(eta + a^2 cos^2 i - xi^2 cot^2 i).
+static void critical_curve(double a, double inc, std::vector<double> &al, std::vector<double> &be) {
+    const double aa = std::fabs(a) > 1e-9 ? a : 1e-9;
+    const double r1 = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(-std::fabs(aa))));
+    const double r2 = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::fabs(aa))));
+    const double si = std::sin(inc), ci = std::cos(inc);
+    const int n = 1024;
+    for (int k = 0; k < n; ++k) {
+        const double r = r2 + (r1 - r2) * (k + 0.5) / n;
+        const double xi = (r * r * (3.0 - r) - aa * aa * (r + 1.0)) / (aa * (r - 1.0));
+        const double eta = r * r * r * (4.0 * aa * aa - r * (r - 3.0) * (r - 3.0)) / (aa * aa * (r - 1.0) * (r - 1.0));
+        const double b2 = eta + aa * aa * ci * ci - xi * xi * (ci / si) * (ci / si);
+        if (b2 < 0.0) continue;
+        al.push_back(-xi / si);
+        be.push_back(std::sqrt(b2));
+    }
+}
+
+// Tile schedule: 8x8 tiles sorted by the distance of their central ray from the critical curve
+// (rays there circle the hole and need the most steps), dealt round-robin over the queues so every
+// queue starts with its longest rays.
+static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y,
+                          int queues, int q_tiles) {
+    std::vector<double> key = {d.spin, d.cam_r, d.cam_theta, d.cam_phi, d.fov_y_degrees, d.roll_degrees,
+                               double(d.width), double(d.height), double(queues)};
+    const int n_tiles = tiles_x * tiles_y;
+    if (ctx->d_order && ctx->order_key == key) return BH_OK;
+    std::vector<double> al, be;
+    const double inc = C.th;
+    if (std::sin(inc) > 1e-3) critical_curve(d.spin, inc, al, be);
+    std::vector<float> dist(n_tiles, 0.f);
+    const double a = d.spin, s0 = std::sin(C.th), c0 = std::cos(C.th);
+    for (int t = 0; t < n_tiles; ++t) {
+        const double px = (t % tiles_x) * 8 + 3.5, py = (t / tiles_x) * 8 + 3.5;
+        const double ndc_x = 2.0 * (px + 0.5) / d.width - 1.0, ndc_y = 1.0 - 2.0 * (py + 0.5) / d.height;
+        double n_r = -1.0, n_th = -ndc_y * C.tan_y, n_ph = ndc_x * C.tan_x;
+        if (C.roll) {
+            const double t0 = C.cos_roll * n_th - C.sin_roll * n_ph, t1 = C.sin_roll * n_th + C.cos_roll * n_ph;
+            n_th = t0;
+            n_ph = t1;
+        }
+        const double nrm = std::sqrt(n_r * n_r + n_th * n_th + n_ph * n_ph);
+        const double pc_t = C.e_t0, pc_th = n_th / nrm * C.e_th2, pc_ph = C.e_t3 + n_ph / nrm * C.e_ph3;
+        const double E = -(C.g00 * pc_t + C.g03 * pc_ph), L = C.g03 * pc_t + C.g33 * pc_ph, pth = C.g22 * pc_th;
+        const double Q = pth * pth + c0 * c0 * (-a * a * E * E + L * L / (s0 * s0));
+        const double xi = L / E, bb = std::sqrt(std::fmax(Q / (E * E) + a * a * c0 * c0 - xi * xi * (c0 / s0) * (c0 / s0), 0.0));
+        const double alpha = -xi / std::sin(inc), beta = bb;
+        double best = 1e300;
+        for (size_t k = 0; k < al.size(); ++k) {
+            const double dx = alpha - al[k], dy = beta - be[k];
+            best = std::fmin(best, dx * dx + dy * dy);
+        }
+        dist[t] = al.empty() ? float(t) : float(best);
+    }
+    std::vector<int> idx(n_tiles);
+    for (int t = 0; t < n_tiles; ++t) idx[t] = t;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return dist[x] < dist[y]; });
+    std::vector<int> order(size_t(queues) * q_tiles, 0);
+    std::vector<int> fill(queues, 0);
+    for (int k = 0; k < n_tiles; ++k) {
+        const int q = k % queues;
+        order[size_t(q) * q_tiles + fill[q]++] = idx[k];
+    }
+    // queue slots past the dealt tiles repeat a tile index that lies outside the image
+    for (int q = 0; q < queues; ++q)
+        for (int j = fill[q]; j < q_tiles; ++j) order[size_t(q) * q_tiles + j] = n_tiles;
+    const size_t bytes = order.size() * sizeof(int);
+    if (bytes > ctx->order_cap) {
+        if (ctx->d_order) (void)hipFree(ctx->d_order);
+        ctx->d_order = nullptr;
+        ctx->order_cap = 0;
+        if (hipMalloc(&ctx->d_order, bytes) != hipSuccess) return set_error(BH_ERR_MEMORY, "schedule allocation failed");
+        ctx->order_cap = bytes;
+    }
+    BH_HIP(hipMemcpyAsync(ctx->d_order, order.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->order_key = key;
+    return BH_OK;
+}
+
 static int validate_scene(const bh_scene_desc &d) {
     if (!std::isfinite(d.spin) || std::fabs(d.spin) >= 1.0)
         return set_error(BH_ERR_VALUE, "Kerr spin must be finite and satisfy |a| < 1 for this integrator");
@@ -575,6 +823,20 @@ static void transpose_soa(const float *aos, int nr, int nt, int np, T *soa, T (*
         for (int k = 0; k < kCoeffs; ++k) soa[size_t(k) * plane + e] = cvt(aos[e * kCoeffs + k]);
 }
 
+// Corner records: cell (ir, it, ip) holds corners (ir+dr, it+dt, (ip+dp) mod np) in order dr*4 + dt*2 + dp.
+template <typename T>
+static void pack_corners(const float *aos, int nr, int nt, int np, T *out, T (*cvt)(float)) {
+    size_t o = 0;
+    for (int ir = 0; ir < nr - 1; ++ir)
+        for (int it = 0; it < nt - 1; ++it)
+            for (int ip = 0; ip < np; ++ip)
+                for (int q = 0; q < 8; ++q) {
+                    const int jr = ir + (q >> 2), jt = it + ((q >> 1) & 1), jp = (ip + (q & 1)) % np;
+                    const float *src = aos + ((size_t(jr) * nt + jt) * np + jp) * kCoeffs;
+                    for (int k = 0; k < kCoeffs; ++k) out[o++] = cvt(src[k]);
+                }
+}
+
 static float cvt_f32(float x) { return x; }
 static uint16_t cvt_bf16(float x) {   // round to nearest even
     uint32_t u;
@@ -592,17 +854,25 @@ static uint16_t cvt_f16(float x) {
 
 typedef void (*RenderFn)(RenderArgs);
 
-template <typename R, int STORE>
-static RenderFn kernel_ptr() { return render_kernel<R, STORE>; }
+template <typename R, int STORE, int LAYOUT, bool MIXED>
+static RenderFn kernel_ptr() { return render_kernel<R, STORE, LAYOUT, MIXED>; }
 
-static RenderFn pick_kernel(int precision, int storage) {
-    if (precision == BH_PREC_FP64) return storage == BH_STORE_F32 ? kernel_ptr<double, BH_STORE_F32>() : nullptr;
+template <int LAYOUT>
+static RenderFn pick_fp32(int storage, bool mixed) {
     switch (storage) {
-        case BH_STORE_F32: return kernel_ptr<float, BH_STORE_F32>();
-        case BH_STORE_BF16: return kernel_ptr<float, BH_STORE_BF16>();
-        case BH_STORE_F16: return kernel_ptr<float, BH_STORE_F16>();
+        case BH_STORE_F32: return mixed ? kernel_ptr<float, BH_STORE_F32, LAYOUT, true>() : kernel_ptr<float, BH_STORE_F32, LAYOUT, false>();
+        case BH_STORE_BF16: return mixed ? kernel_ptr<float, BH_STORE_BF16, LAYOUT, true>() : kernel_ptr<float, BH_STORE_BF16, LAYOUT, false>();
+        case BH_STORE_F16: return mixed ? kernel_ptr<float, BH_STORE_F16, LAYOUT, true>() : kernel_ptr<float, BH_STORE_F16, LAYOUT, false>();
     }
     return nullptr;
+}
+
+static RenderFn pick_kernel(int precision, int storage, int layout) {
+    if (precision == BH_PREC_FP64)
+        return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA) ? kernel_ptr<double, BH_STORE_F32, BH_LAYOUT_SOA, false>()
+                                                                    : nullptr;
+    const bool mixed = precision == BH_PREC_MIXED;
+    return layout == BH_LAYOUT_CORNER ? pick_fp32<BH_LAYOUT_CORNER>(storage, mixed) : pick_fp32<BH_LAYOUT_SOA>(storage, mixed);
 }
 
 }  // namespace bh
@@ -612,7 +882,7 @@ using namespace bh;
 extern "C" {
 
 int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, int32_t nr, const float *theta_grid,
-                    int32_t nt, const float *phi_grid, int32_t np, int32_t storage, int32_t *brick_id) {
+                    int32_t nt, const float *phi_grid, int32_t np, int32_t storage, int32_t layout, int32_t *brick_id) {
     clear_error();
     int st = ctx_check(ctx);
     if (st) return st;
@@ -622,6 +892,8 @@ int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, i
     if (nr + nt + np > kMaxGrid) return set_error(BH_ERR_VALUE, "brick grids hold more than %d values", kMaxGrid);
     if (storage != BH_STORE_F32 && storage != BH_STORE_BF16 && storage != BH_STORE_F16)
         return set_error(BH_ERR_VALUE, "storage must be BH_STORE_F32, BH_STORE_BF16 or BH_STORE_F16");
+    if (layout != BH_LAYOUT_SOA && layout != BH_LAYOUT_CORNER)
+        return set_error(BH_ERR_VALUE, "layout must be BH_LAYOUT_SOA or BH_LAYOUT_CORNER");
     const float *gs[3] = {r_grid, theta_grid, phi_grid};
     const int ns[3] = {nr, nt, np};
     const char *names[3] = {"r", "theta", "phi"};
@@ -633,15 +905,25 @@ int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, i
     }
     const size_t plane = size_t(nr) * nt * np;
     const size_t esz = storage == BH_STORE_F32 ? 4 : 2;
-    if (plane * kCoeffs * esz >= (size_t(1) << 31)) return set_error(BH_ERR_VALUE, "coefficient brick is too large");
+    const size_t cells = size_t(nr - 1) * (nt - 1) * np;
+    const size_t nbytes = layout == BH_LAYOUT_SOA ? plane * kCoeffs * esz : cells * 8 * kCoeffs * esz;
+    if (nbytes >= (size_t(1) << 31)) return set_error(BH_ERR_VALUE, "coefficient brick is too large");
     for (size_t i = 0; i < plane * kCoeffs; ++i)
         if (!std::isfinite(coeffs_aos[i])) return set_error(BH_ERR_VALUE, "coefficient brick contains non-finite values");
-    std::vector<unsigned char> host(plane * kCoeffs * esz);
-    if (storage == BH_STORE_F32) transpose_soa<float>(coeffs_aos, nr, nt, np, reinterpret_cast<float *>(host.data()), cvt_f32);
-    else if (storage == BH_STORE_BF16) transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16);
-    else transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16);
+    std::vector<unsigned char> host(nbytes);
+    if (layout == BH_LAYOUT_SOA) {
+        if (storage == BH_STORE_F32) transpose_soa<float>(coeffs_aos, nr, nt, np, reinterpret_cast<float *>(host.data()), cvt_f32);
+        else if (storage == BH_STORE_BF16) transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16);
+        else transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16);
+    } else {
+        if (storage == BH_STORE_F32) pack_corners<float>(coeffs_aos, nr, nt, np, reinterpret_cast<float *>(host.data()), cvt_f32);
+        else if (storage == BH_STORE_BF16) pack_corners<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16);
+        else pack_corners<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16);
+    }
     BrickDev b;
     b.storage = storage;
+    b.layout = layout;
+    b.bytes = nbytes;
     b.nr = nr;
     b.nt = nt;
     b.np = np;
@@ -700,12 +982,14 @@ int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t) {
     if (t->steps_per_sync) u.steps_per_sync = t->steps_per_sync;
     if (t->refill_min) u.refill_min = t->refill_min;
     if (t->queues) u.queues = t->queues;
+    if (t->order >= 0) u.order = t->order;
     if (u.block_threads < 64 || u.block_threads > 256 || u.block_threads % 64)
         return set_error(BH_ERR_VALUE, "block_threads must be 64, 128, 192 or 256");
     if (u.steps_per_sync < 1 || u.steps_per_sync > 4096) return set_error(BH_ERR_VALUE, "steps_per_sync out of range");
     if (u.refill_min < 1 || u.refill_min > 64) return set_error(BH_ERR_VALUE, "refill_min must lie in 1..64");
     if (u.queues < 1 || u.queues > kMaxQueues) return set_error(BH_ERR_VALUE, "queues must lie in 1..%d", kMaxQueues);
     if (u.blocks_per_cu > 64) return set_error(BH_ERR_VALUE, "blocks_per_cu out of range");
+    if (u.order != 0 && u.order != 1) return set_error(BH_ERR_VALUE, "order must be 0 or 1");
     ctx->tuning[precision] = u;
     return BH_OK;
 }
@@ -720,6 +1004,7 @@ int bh_ctx_get_tuning(bh_ctx *ctx, int32_t precision, bh_tuning *t) {
     t->steps_per_sync = u.steps_per_sync;
     t->refill_min = u.refill_min;
     t->queues = u.queues;
+    t->order = u.order;
     return BH_OK;
 }
 
@@ -732,17 +1017,20 @@ int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, 
     if ((st = validate_scene(*scene))) return st;
     if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
         return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
-    if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32)
-        return set_error(BH_ERR_VALUE, "precision %d is not available in this entry point", precision);
+    if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32 && precision != BH_PREC_MIXED)
+        return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
     const BrickDev &b = ctx->bricks[brick_id];
-    RenderFn fn = pick_kernel(precision, b.storage);
-    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick");
+    RenderFn fn = pick_kernel(precision, b.storage, b.layout);
+    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick in the SoA layout");
     RenderArgs A{};
     if ((st = camera_constants(*scene, A.cam))) return st;
     const double a = scene->spin;
     A.a = a;
     A.r_plus = 1.0 + std::sqrt(1.0 - a * a);
     A.r_cap = A.r_plus * (1.0 + 2.0e-4);
+    A.r_island = 1.5 * A.r_plus;
+    A.th_island = kIslandTheta;
+    A.pth_island = kIslandPtheta;
     A.step = scene->step;
     A.adaptive = scene->step_mode;
     A.r_esc = scene->escape_radius;
@@ -751,13 +1039,19 @@ int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, 
     A.W = scene->width;
     A.H = scene->height;
     A.max_steps = scene->max_steps;
+    A.step_f = float(A.step);
+    A.r_plus_f = float(A.r_plus);
+    A.r_cap_f = float(A.r_cap);
+    A.r_esc_f = float(A.r_esc);
+    A.L_f = float(A.L);
+    A.tau_max_f = float(A.tau_max);
     A.coeffs = b.coeffs;
     A.nr = b.nr;
     A.nt = b.nt;
     A.np = b.np;
     const unsigned esz = b.storage == BH_STORE_F32 ? 4u : 2u;
     A.plane_bytes = unsigned(b.plane * esz);
-    A.total_bytes = unsigned(b.plane * esz * kCoeffs);
+    A.total_bytes = unsigned(b.bytes);
     A.grids = b.grids;
     A.hr = b.hr;
     A.ht = b.ht;
@@ -773,7 +1067,14 @@ int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, 
     const int n_tiles = A.tiles_x * tiles_y;
     A.n_ids = n_tiles * 64;
     A.queues = u.queues < n_tiles ? u.queues : n_tiles;
-    A.q_len = ((n_tiles + A.queues - 1) / A.queues) * 64;
+    const int q_tiles = (n_tiles + A.queues - 1) / A.queues;
+    A.q_len = q_tiles * 64;
+    A.n_ids = A.queues * A.q_len;   // padded id space: every queue slot is issued; padding maps outside the image
+    A.tile_order = nullptr;
+    if (u.order == 1) {
+        if ((st = build_schedule(ctx, *scene, A.cam, A.tiles_x, tiles_y, A.queues, q_tiles))) return st;
+        A.tile_order = ctx->d_order;
+    }
     A.steps_per_sync = u.steps_per_sync;
     A.refill_min = u.refill_min;
     int per_cu = u.blocks_per_cu;

@@ -97,6 +97,8 @@ int bh_device_alloc(bh_ctx *ctx, size_t bytes, void **ptr);
 int bh_device_free(bh_ctx *ctx, void *ptr);
 int bh_copy_to_device(bh_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bh_copy_to_host(bh_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* device-to-device copy on the context stream (synchronous) */
+int bh_copy_device(bh_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Device-resident FP64 sampler over validated inputs (no host copies, no
  * allocation): same arithmetic as bh_sample_brick_trilinear.  Used for the
@@ -113,13 +115,22 @@ int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs,
 #define BH_STORE_BF16  1
 #define BH_STORE_F16   2
 
-/* Upload an AoS float32 brick (nr, nt, np, 11) with float32 grids; it is
- * stored on the device as SoA [11][nr][nt][np] in the requested type. */
+/* Device layouts of a brick:
+ *  BH_LAYOUT_SOA     channel planes [11][nr][nt][np];
+ *  BH_LAYOUT_CORNER  one record per trilinear cell [nr-1][nt-1][np] holding the
+ *                    cell's 8 corners x 11 channels contiguously (phi seam cell
+ *                    included): 8x the bytes, one contiguous read per sample. */
+#define BH_LAYOUT_SOA     0
+#define BH_LAYOUT_CORNER  1
+
+/* Upload an AoS float32 brick (nr, nt, np, 11) with float32 grids, stored on
+ * the device in the requested type and layout.  The FP64 render needs
+ * BH_STORE_F32 + BH_LAYOUT_SOA. */
 int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos,
                     const float *r_grid, int32_t nr,
                     const float *theta_grid, int32_t nt,
                     const float *phi_grid, int32_t np,
-                    int32_t storage, int32_t *brick_id);
+                    int32_t storage, int32_t layout, int32_t *brick_id);
 int bh_brick_release(bh_ctx *ctx, int32_t brick_id);
 
 /* ---- fused render (render_stokes_image_bricks, accelerated_renderer.py:100-158,
@@ -146,7 +157,8 @@ typedef struct bh_scene_desc {
 #define BH_PREC_MIXED  2   /* FP32 with FP64 islands; BF16/FP16 bricks */
 
 /* Per-pixel status byte: low 3 bits 0 max_steps, 1 captured, 2 escaped,
- * 3 invalid, 4 optically thick; bit 3 set when the theta clamp fired. */
+ * 3 invalid, 4 optically thick; bit 3 set when the steps could not follow
+ * the geodesic through a pole (theta clamp fired, or |d phi| > 1 rad in one step). */
 
 /* Launch shape of the persistent ray-queue kernels (0 fields = defaults). */
 typedef struct bh_tuning {
@@ -155,6 +167,7 @@ typedef struct bh_tuning {
     int32_t steps_per_sync;   /* steps between wave refill checks */
     int32_t refill_min;       /* refill when at least this many lanes are idle (1..64) */
     int32_t queues;           /* 1, or 8 for one ray queue per XCD */
+    int32_t order;            /* 0: row-major tiles; 1: tiles nearest the critical curve first */
 } bh_tuning;
 
 int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t);

@@ -26,11 +26,13 @@ sampler works in FP64 on the float32 grid and coefficient values, like
 native_kernels.py's FP64 sampler.
 
 Status codes (low 3 bits): 0 max_steps, 1 captured, 2 escaped, 3 invalid,
-4 optically thick.  Bit 3 (value 8) is set when the theta clamp of
-kerr.py:35-36 fired at a midpoint or end state: such a ray stepped over a pole,
-and the clamp turns d(phi)/dlambda = L / (Sigma sin^2 theta) into ~1e14 L,
-so its phi and ds -- and its Stokes vector -- are not meaningful.  Error
-statistics exclude these rays and report how many there were.
+4 optically thick.  Bit 3 (value 8) marks a ray whose RK2 steps could not
+follow the geodesic through a coordinate pole: the theta clamp of kerr.py:35-36
+fired at a midpoint or end state, or one step changed phi by more than 1 rad
+(d(phi)/dlambda = L / (Sigma sin^2 theta) blows up within a step of the axis;
+regular steps move phi by well under 1 rad).  Such rays pick up huge ds and
+their Stokes vectors are not meaningful in any precision, so error statistics
+exclude them and report how many there were.
 """
 
 from __future__ import annotations
@@ -44,6 +46,7 @@ from . import kerr, micro
 
 ST_MAX_STEPS, ST_CAPTURED, ST_ESCAPED, ST_INVALID, ST_OPAQUE = 0, 1, 2, 3, 4
 ST_POLE_CLAMPED = 8
+POLE_DPHI = 1.0   # rad per step
 
 
 def _pymax(a, b):
@@ -110,6 +113,7 @@ def render_rays(scene, brick, pix: np.ndarray | None = None):
         nth = kerr.safe_theta(nth_raw)
         clamped |= (mth_ < lo) | (mth_ > hi) | (nth_raw < lo) | (nth_raw > hi)
         nph = ph + h * k2[2]
+        clamped |= np.abs(nph - ph) > POLE_DPHI
         npr = pr + h * k2[3]
         npth = pth + h * k2[4]
         finite = np.isfinite(nr_) & np.isfinite(nth) & np.isfinite(nph) & np.isfinite(npr) & np.isfinite(npth)
