@@ -14,7 +14,9 @@ Prints ONE JSON line.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
+import math
 import os
 import platform
 import sys
@@ -107,6 +109,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--frames", type=int, default=8, help="1024^2 frames per batched launch")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -167,6 +170,77 @@ def main(argv=None) -> int:
             "pole_flagged_frac": float(1 - okx.mean()), "rk2_steps": st, "brick_samples": sm}
         log(f"[bench] mixed {storage}: {paths[f'mixed_{storage}']}")
         rr.release(dbx)
+
+    # ---- batched paths -----------------------------------------------------------------
+    from bhgrrt.autotune import autotune
+    from bhgrrt.render import Batch
+
+    frames = args.frames
+    log(f"[bench] batched: {frames} frames of the {scene.width}^2 scene (camera azimuth steps), one launch")
+    frame_scenes = [dataclasses.replace(scene, camera=dataclasses.replace(scene.camera, phi=2.0 * math.pi * k / frames))
+                    for k in range(frames)]
+    for name, storage, prec in (("batched_fp32", "f32", PREC_FP32), ("batched_mixed_bf16", "bf16", PREC_MIXED),
+                                ("batched_mixed_f16", "f16", PREC_MIXED)):
+        dbx = rr.upload(brick, storage, "corner")
+        bt = Batch(rr, [(fs, dbx) for fs in frame_scenes], prec)
+        tuned = autotune(rr, prec, lambda: bt.launch(graph=False), space={"steps_per_sync": (8, 16, 32),
+                                                                          "block_threads": (64, 256)}, reps=2)
+        for _ in range(args.warmup):
+            bt.launch(graph=True)
+        rr.synchronize()
+        t0 = time.perf_counter()
+        kms = []
+        for _ in range(args.steps):
+            bt.launch(graph=True)
+            kms.append(rr.last_kernel_ms())
+        rr.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        util = rr.last_lane_utilisation()
+        outs = bt.results()
+        bt.close()
+        rr.release(dbx)
+        paths[name] = {"mrays_s": frames * n / dt / 1e6, "ms_per_launch": dt * 1e3, "kernel_ms": float(np.mean(kms)),
+                       "frames": frames, "lane_utilisation": util, "tuning": tuned["best"]}
+        if name == "batched_fp32":
+            batch32 = outs
+        log(f"[bench] {name}: {paths[name]}")
+    # accuracy of the batched frames against the FP64 render of the same frames
+    bt64 = Batch(rr, [(fs, db32) for fs in frame_scenes], PREC_FP64)
+    bt64.launch(graph=False)
+    outs64 = bt64.results()
+    bt64.close()
+    ok = np.concatenate([((a["status"] | b["status"]) & 8).ravel() == 0 for a, b in zip(outs64, batch32)])
+    I64b = np.concatenate([o["iquv"][..., 0].ravel() for o in outs64])
+    I32b = np.concatenate([o["iquv"][..., 0].ravel() for o in batch32]).astype(np.float64)
+    paths["batched_fp32"]["rel_l1_I"] = rel_l1(I32b[ok], I64b[ok])
+
+    log("[bench] configs[3]-style batch: 64 renders, 128^2..1024^2, random spin and inclination, 4 bf16 bricks")
+    bricks = [rr.upload(b, "bf16", "corner") for b in scenes.batch_bricks()]
+    jobs = [(sc, bricks[k]) for sc, k in scenes.batch_scenes(64)]
+    bt = Batch(rr, jobs, PREC_MIXED)
+    tuned = autotune(rr, PREC_MIXED, lambda: bt.launch(graph=False), space={"steps_per_sync": (8, 16, 32),
+                                                                             "block_threads": (64, 256)}, reps=1)
+    bt.launch(graph=True)
+    rr.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        bt.launch(graph=True)
+    rr.synchronize()
+    dt = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    kms = rr.last_kernel_ms()
+    util = rr.last_lane_utilisation()
+    outs = bt.results()
+    steps_all = np.concatenate([o["steps"].ravel() for o in outs])
+    edges = [1, 64, 128, 256, 512, 1024, 2048, 4096, 4097]
+    hist = np.histogram(steps_all, bins=edges)[0].tolist()
+    rays_b = int(steps_all.size)
+    paths["batch64_mixed_bf16"] = {"mrays_s": rays_b / dt / 1e6, "ms_per_launch": dt * 1e3, "kernel_ms": kms,
+                                   "rays": rays_b, "lane_utilisation": util, "tuning": tuned["best"],
+                                   "step_histogram": {"edges": edges, "counts": hist}}
+    log(f"[bench] batch64: {paths['batch64_mixed_bf16']}")
+    bt.close()
+    for b in bricks:
+        rr.release(b)
 
     cpu = None
     if not args.no_cpu:

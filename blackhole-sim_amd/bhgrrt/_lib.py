@@ -45,6 +45,11 @@ class Tuning(C.Structure):
                 ("refill_min", C.c_int32), ("queues", C.c_int32), ("order", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    """bh_batch_item."""
+    _fields_ = [("scene", SceneDesc), ("brick_id", C.c_int32), ("reserved", C.c_int32)]
+
+
 # (name, restype, argtypes) for every symbol include/bhgrrt.h declares
 SIGNATURES = [
     ("bh_core_version", C.c_char_p, []),
@@ -76,7 +81,12 @@ SIGNATURES = [
     ("bh_brick_release", C.c_int, [C.c_void_p, C.c_int32]),
     ("bh_ctx_set_tuning", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),
     ("bh_ctx_get_tuning", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),
-    ("bh_ctx_last_work", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("bh_ctx_last_work", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("bh_batch_create", C.c_int,
+     [C.c_void_p, C.POINTER(BatchItem), C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_void_p)]),
+    ("bh_batch_launch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_batch_launch_graph", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_batch_destroy", C.c_int, [C.c_void_p]),
     ("bh_render", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, _i32p, _u8p]),
     ("bh_render_device", C.c_int,
      [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -121,8 +121,14 @@ class Renderer:
     def last_work(self) -> tuple[int, int]:
         """(RK2 steps, in-brick samples) of the last render launch."""
         a, b = C.c_uint64(0), C.c_uint64(0)
-        _lib.check(self._lib.bh_ctx_last_work(self.ctx, C.byref(a), C.byref(b)))
+        _lib.check(self._lib.bh_ctx_last_work(self.ctx, C.byref(a), C.byref(b), None))
         return int(a.value), int(b.value)
+
+    def last_lane_utilisation(self) -> float:
+        """RK2 steps / lane slots of the last launch: SIMD lane occupancy after termination and compaction."""
+        a, c = C.c_uint64(0), C.c_uint64(0)
+        _lib.check(self._lib.bh_ctx_last_work(self.ctx, C.byref(a), None, C.byref(c)))
+        return float(a.value) / max(float(c.value), 1.0)
 
     # --- device memory ----------------------------------------------------------
     def alloc(self, nbytes: int) -> "DeviceBuffer":
@@ -153,6 +159,70 @@ class DeviceBuffer:
         if self.ptr:
             _lib.check(self.r._lib.bh_device_free(self.r.ctx, self.ptr))
             self.ptr = None
+
+
+class Batch:
+    """Many renders traced by one persistent launch (bh_batch_*); outputs are concatenated per pixel."""
+
+    def __init__(self, renderer: Renderer, jobs, precision: int = PREC_FP32):
+        """jobs: sequence of (Scene, DeviceBrick); all bricks must share storage and layout."""
+        self.r = renderer
+        self.scenes = [sc for sc, _ in jobs]
+        self.precision = int(precision)
+        n = len(self.scenes)
+        items = (_lib.BatchItem * n)()
+        for i, (sc, db) in enumerate(jobs):
+            items[i].scene = scene_desc(sc)
+            items[i].brick_id = db.brick_id
+        off = (C.c_int64 * (n + 1))()
+        h = C.c_void_p()
+        _lib.check(renderer._lib.bh_batch_create(renderer.ctx, items, n, self.precision, off, C.byref(h)))
+        self.handle = h
+        self.offsets = [int(off[i]) for i in range(n + 1)]
+        self.pixels = self.offsets[-1]
+        self._bufs = None
+
+    def buffers(self):
+        if self._bufs is None:
+            esz = 8 if self.precision == PREC_FP64 else 4
+            self._bufs = (self.r.alloc(self.pixels * 4 * esz), self.r.alloc(self.pixels * 4), self.r.alloc(self.pixels))
+        return self._bufs
+
+    def launch(self, graph: bool = True) -> None:
+        """Enqueue one batched launch into the batch's own device outputs."""
+        d_iquv, d_steps, d_status = self.buffers()
+        fn = self.r._lib.bh_batch_launch_graph if graph else self.r._lib.bh_batch_launch
+        _lib.check(fn(self.handle, d_iquv.ptr, d_steps.ptr, d_status.ptr))
+
+    def results(self) -> list[dict]:
+        """Synchronise and return per-render dicts of iquv (H, W, 4), steps and status."""
+        d_iquv, d_steps, d_status = self.buffers()
+        esz = np.float64 if self.precision == PREC_FP64 else np.float32
+        iquv = d_iquv.to_host(np.empty((self.pixels, 4), dtype=esz))
+        steps = d_steps.to_host(np.empty(self.pixels, dtype=np.int32))
+        status = d_status.to_host(np.empty(self.pixels, dtype=np.uint8))
+        out = []
+        for i, sc in enumerate(self.scenes):
+            a, b = self.offsets[i], self.offsets[i + 1]
+            H, W = sc.height, sc.width
+            out.append({"iquv": iquv[a:b].reshape(H, W, 4), "steps": steps[a:b].reshape(H, W),
+                        "status": status[a:b].reshape(H, W)})
+        return out
+
+    def close(self) -> None:
+        if self._bufs is not None:
+            for b in self._bufs:
+                b.free()
+            self._bufs = None
+        if self.handle:
+            self.r._lib.bh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _ptr(x):

@@ -224,3 +224,24 @@ def pixel_ndc(scene: Scene, px: np.ndarray, py: np.ndarray) -> tuple[np.ndarray,
     ndc_x = 2.0 * (np.asarray(px, dtype=np.float64) + 0.5) / scene.width - 1.0
     ndc_y = 1.0 - 2.0 * (np.asarray(py, dtype=np.float64) + 0.5) / scene.height
     return ndc_x, ndc_y
+
+
+def batch_bricks(seed: int = 20240603) -> list[Brick]:
+    """Four seeded disk bricks from 64x32x64 to 128x64x128 (grid from r_+ of a=0.998, so every spin fits)."""
+    shapes = [(64, 32, 64), (80, 40, 80), (96, 48, 96), (128, 64, 128)]
+    return [make_brick(nr, nt, np_, spin=0.998, seed=seed + k, disk=True) for k, (nr, nt, np_) in enumerate(shapes)]
+
+
+def batch_scenes(n: int = 64, seed: int = 20240604, min_px: int = 128, max_px: int = 1024) -> list[tuple[Scene, int]]:
+    """Batch workload: n renders, square sizes log-uniform in [min_px, max_px], a ~ U[0, 0.998],
+    inclination ~ U[5, 85] deg, +-20 M field, h = 0.05 (r - r_+), <= 4096 steps; brick index i % 4."""
+    rng = np.random.default_rng(int(seed))
+    out = []
+    for i in range(int(n)):
+        npx = int(round(math.exp(rng.uniform(math.log(min_px), math.log(max_px)))))
+        spin = float(rng.uniform(0.0, 0.998))
+        inc = float(rng.uniform(5.0, 85.0))
+        cam = Camera.spanning(r=1000.0, inclination_degrees=inc, half_width_m=20.0, width=npx, height=npx)
+        out.append((Scene(spin=spin, camera=cam, width=npx, height=npx, step_mode=STEP_ADAPTIVE, step=0.05,
+                          max_steps=4096, escape_radius=220.0), i % 4))
+    return out

@@ -45,6 +45,7 @@ struct BrickDev {
     size_t bytes = 0;          // device bytes of the coefficient array
     void *coeffs = nullptr;    // SoA [11][nr][nt][np], or corner cells [nr-1][nt-1][np][8][11]
     float *grids = nullptr;    // r | theta | phi, float32
+    std::vector<float> hgrids; // host copy of the grids
     GridHint hr{}, ht{}, hp{};
 };
 
@@ -84,6 +85,7 @@ void begin_timing(bh_ctx *ctx);
 void end_timing(bh_ctx *ctx, int launches);
 // Fill n words of device memory with v on the context stream (graph-safe, no memset node).
 int fill_u32(bh_ctx *ctx, unsigned int *dst, unsigned int v, int n);
+__global__ void fill_u32_kernel(unsigned int *dst, unsigned int v, int n);
 // Process-wide context on the current device used by the host-buffer entry points.
 bh_ctx *default_ctx(int *status);
 std::mutex &default_ctx_mutex();

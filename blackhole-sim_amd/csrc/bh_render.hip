@@ -63,6 +63,9 @@ struct RenderArgs {
     const void *coeffs;
     int nr, nt, np;
     unsigned plane_bytes, total_bytes;
+    unsigned brick_base;        // byte offset of this brick inside the coefficient buffer (batches merge bricks)
+    int grid_off;               // offset of this brick's r grid in the LDS grid table
+    long long out_off;          // first pixel of this render in the batch output arrays
     const float *grids;
     GridHint hr, ht, hp;
     void *iquv;
@@ -278,7 +281,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
         constexpr int kWords = 8 * kCoeffs * int(esz) / 4;   // 88 (f32) or 44 (16-bit)
         constexpr int kVec = kWords / 4;                      // 22 or 11 loads of 16 bytes
-        const unsigned base = cell * unsigned(8 * kCoeffs * esz);
+        const unsigned base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
         unsigned w[kWords];
 #pragma unroll
         for (int j = 0; j < kVec; ++j) {
@@ -306,7 +309,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         }
         return true;
     }
-    const unsigned row00 = (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
+    const unsigned row00 = A.brick_base / esz + (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
     const unsigned row01 = row00 + unsigned(A.np);
     const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
     const unsigned row11 = row10 + unsigned(A.np);
@@ -586,6 +589,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     bool active = false;
     RayState<R> s;
     unsigned n_steps = 0, n_samples = 0;   // this lane's work over all its rays
+    unsigned long long iters = 0;          // wave loop iterations
 
     for (;;) {
         unsigned long long idle = __ballot(!active);
@@ -635,6 +639,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
                 }
             }
         }
+        iters += unsigned(A.steps_per_sync);
     }
     // wave-reduce the work counters, one atomic per wave
     unsigned long long ws = n_steps, wsm = n_samples;
@@ -645,6 +650,109 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     if (lane == 0) {
         atomicAdd(&A.work[0], ws);
         atomicAdd(&A.work[1], wsm);
+        atomicAdd(&A.work[2], iters * 64ull);
+    }
+}
+
+
+// ---- batched persistent launch ---------------------------------------------------
+// Many renders (own size, spin, camera and brick) in one launch.  Waves take whole 8x8
+// tiles from per-XCD queues of a global longest-first schedule; a tile's render descriptor is
+// loaded with scalar loads, so every render parameter is wave-uniform.  All bricks of a batch
+// live in one coefficient buffer (one buffer resource) and one LDS grid table.
+
+struct BatchArgs {
+    const RenderArgs *descs;     // per render: scene, brick view, output offset
+    const int *slots;            // queue slot -> (render << 20 | tile) or -1 for padding
+    const float *grids;          // per brick: r | theta | phi | +inf
+    int ng;
+    const void *coeffs;
+    unsigned total_bytes;
+    void *iquv;
+    int *steps;
+    uint8_t *status;
+    unsigned *counters;          // per-queue tile counters
+    unsigned long long *work;    // [0] RK2 steps, [1] brick samples, [2] wave iterations x 64
+    int queues, q_tiles, steps_per_sync;
+};
+
+template <typename R, int STORE, int LAYOUT, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void batch_kernel(
+    BatchArgs B) {
+    __shared__ R s_grid[kMaxGrid + 8];
+    for (int i = threadIdx.x; i < B.ng; i += blockDim.x) s_grid[i] = R(B.grids[i]);
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(B.coeffs), (short)0, int(B.total_bytes), int(kRsrcWord3));
+    const int lane = threadIdx.x & 63;
+    const int q_own = blockIdx.x % B.queues;
+    const unsigned all_q = (1u << B.queues) - 1u;
+    unsigned exhausted = 0;
+    bool active = false;
+    RayState<R> s;
+    RenderArgs A;
+    unsigned n_steps = 0, n_samples = 0;
+    unsigned long long iters = 0;
+    for (;;) {
+        if (!__any(active)) {
+            bool took = false;
+            while (!took && exhausted != all_q) {
+                for (int t = 0; t < B.queues && !took; ++t) {
+                    const int q = (q_own + t) % B.queues;
+                    if (exhausted & (1u << q)) continue;
+                    unsigned slot = 0;
+                    if (lane == 0) slot = atomicAdd(&B.counters[q], 1u);
+                    slot = __builtin_amdgcn_readfirstlane(slot);
+                    if (slot >= unsigned(B.q_tiles)) {
+                        exhausted |= (1u << q);
+                        continue;
+                    }
+                    const int e = B.slots[q * B.q_tiles + int(slot)];
+                    if (e < 0) continue;
+                    const int rix = e >> 20, tile = e & 0xfffff;
+                    A = B.descs[rix];
+                    const size_t esz = sizeof(R);
+                    A.iquv = static_cast<char *>(B.iquv) + size_t(A.out_off) * 4 * esz;
+                    A.steps = B.steps ? B.steps + A.out_off : nullptr;
+                    A.status = B.status ? B.status + A.out_off : nullptr;
+                    const int px = (tile % A.tiles_x) * 8 + (lane & 7);
+                    const int py = (tile / A.tiles_x) * 8 + (lane >> 3);
+                    if (px < A.W && py < A.H) {
+                        ray_init(A, px, py, s);
+                        active = true;
+                    }
+                    took = true;
+                }
+            }
+            if (!took) break;
+        }
+        const int live = __popcll(__ballot(active));
+        if (live <= 8) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
+        const R *gr = s_grid + A.grid_off;
+        const R *gt = gr + A.nr;
+        const R *gp = gt + A.nt;
+        for (int k = 0; k < B.steps_per_sync; ++k) {
+            if (active) {
+                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, rs, gr, gt, gp, s, n_samples);
+                ++n_steps;
+                if (code >= 0) {
+                    ray_store(A, s, code);
+                    active = false;
+                }
+            }
+        }
+        iters += unsigned(B.steps_per_sync);
+    }
+    unsigned long long ws = n_steps, wsm = n_samples;
+    for (int off = 32; off > 0; off >>= 1) {
+        ws += __shfl_xor(ws, off);
+        wsm += __shfl_xor(wsm, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&B.work[0], ws);
+        atomicAdd(&B.work[1], wsm);
+        atomicAdd(&B.work[2], iters * 64ull);
     }
 }
 
@@ -711,12 +819,9 @@ static void critical_curve(double a, double inc, std::vector<double> &al, std::v
 // Tile schedule: 8x8 tiles sorted by the distance of their central ray from the critical curve
 // (rays there circle the hole and need the most steps), dealt round-robin over the queues so every
 // queue starts with its longest rays.
-static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y,
-                          int queues, int q_tiles) {
-    std::vector<double> key = {d.spin, d.cam_r, d.cam_theta, d.cam_phi, d.fov_y_degrees, d.roll_degrees,
-                               double(d.width), double(d.height), double(queues)};
+// Squared distance of each 8x8 tile's central ray from the critical curve in Bardeen's image plane.
+static std::vector<float> tile_distances(const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y) {
     const int n_tiles = tiles_x * tiles_y;
-    if (ctx->d_order && ctx->order_key == key) return BH_OK;
     std::vector<double> al, be;
     const double inc = C.th;
     if (std::sin(inc) > 1e-3) critical_curve(d.spin, inc, al, be);
@@ -744,6 +849,16 @@ static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C
         }
         dist[t] = al.empty() ? float(t) : float(best);
     }
+    return dist;
+}
+
+static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y,
+                          int queues, int q_tiles) {
+    std::vector<double> key = {d.spin, d.cam_r, d.cam_theta, d.cam_phi, d.fov_y_degrees, d.roll_degrees,
+                               double(d.width), double(d.height), double(queues)};
+    const int n_tiles = tiles_x * tiles_y;
+    if (ctx->d_order && ctx->order_key == key) return BH_OK;
+    const std::vector<float> dist = tile_distances(d, C, tiles_x, tiles_y);
     std::vector<int> idx(n_tiles);
     for (int t = 0; t < n_tiles; ++t) idx[t] = t;
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return dist[x] < dist[y]; });
@@ -875,6 +990,69 @@ static RenderFn pick_kernel(int precision, int storage, int layout) {
     return layout == BH_LAYOUT_CORNER ? pick_fp32<BH_LAYOUT_CORNER>(storage, mixed) : pick_fp32<BH_LAYOUT_SOA>(storage, mixed);
 }
 
+typedef void (*BatchFn)(BatchArgs);
+
+template <typename R, int STORE, int LAYOUT, bool MIXED>
+static BatchFn batch_ptr() { return batch_kernel<R, STORE, LAYOUT, MIXED>; }
+
+template <int LAYOUT>
+static BatchFn pick_batch_fp32(int storage, bool mixed) {
+    switch (storage) {
+        case BH_STORE_F32: return mixed ? batch_ptr<float, BH_STORE_F32, LAYOUT, true>() : batch_ptr<float, BH_STORE_F32, LAYOUT, false>();
+        case BH_STORE_BF16: return mixed ? batch_ptr<float, BH_STORE_BF16, LAYOUT, true>() : batch_ptr<float, BH_STORE_BF16, LAYOUT, false>();
+        case BH_STORE_F16: return mixed ? batch_ptr<float, BH_STORE_F16, LAYOUT, true>() : batch_ptr<float, BH_STORE_F16, LAYOUT, false>();
+    }
+    return nullptr;
+}
+
+static BatchFn pick_batch_kernel(int precision, int storage, int layout) {
+    if (precision == BH_PREC_FP64)
+        return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA) ? batch_ptr<double, BH_STORE_F32, BH_LAYOUT_SOA, false>()
+                                                                    : nullptr;
+    const bool mixed = precision == BH_PREC_MIXED;
+    return layout == BH_LAYOUT_CORNER ? pick_batch_fp32<BH_LAYOUT_CORNER>(storage, mixed)
+                                      : pick_batch_fp32<BH_LAYOUT_SOA>(storage, mixed);
+}
+
+static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A) {
+    int st;
+    if ((st = camera_constants(sc, A.cam))) return st;
+    const double a = sc.spin;
+    A.a = a;
+    A.r_plus = 1.0 + std::sqrt(1.0 - a * a);
+    A.r_cap = A.r_plus * (1.0 + 2.0e-4);
+    A.r_island = 1.5 * A.r_plus;
+    A.th_island = kIslandTheta;
+    A.pth_island = kIslandPtheta;
+    A.step = sc.step;
+    A.adaptive = sc.step_mode;
+    A.r_esc = sc.escape_radius;
+    A.L = sc.affine_to_length_cm;
+    A.tau_max = sc.max_optical_depth;
+    A.W = sc.width;
+    A.H = sc.height;
+    A.max_steps = sc.max_steps;
+    A.step_f = float(A.step);
+    A.r_plus_f = float(A.r_plus);
+    A.r_cap_f = float(A.r_cap);
+    A.r_esc_f = float(A.r_esc);
+    A.L_f = float(A.L);
+    A.tau_max_f = float(A.tau_max);
+    A.coeffs = b.coeffs;
+    A.nr = b.nr;
+    A.nt = b.nt;
+    A.np = b.np;
+    const unsigned esz = b.storage == BH_STORE_F32 ? 4u : 2u;
+    A.plane_bytes = unsigned(b.plane * esz);
+    A.total_bytes = unsigned(b.bytes);
+    A.grids = b.grids;
+    A.hr = b.hr;
+    A.ht = b.ht;
+    A.hp = b.hp;
+    A.tiles_x = (A.W + 7) / 8;
+    return BH_OK;
+}
+
 }  // namespace bh
 
 using namespace bh;
@@ -939,6 +1117,7 @@ int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, i
     std::memcpy(g.data() + nr + nt, phi_grid, sizeof(float) * np);
     BH_HIP(hipMemcpy(b.coeffs, host.data(), host.size(), hipMemcpyHostToDevice));
     BH_HIP(hipMemcpy(b.grids, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice));
+    b.hgrids = g;
     b.hr = make_hint(r_grid, nr, false);
     b.ht = make_hint(theta_grid, nt, false);
     b.hp = make_hint(phi_grid, np, true);
@@ -961,14 +1140,15 @@ int bh_brick_release(bh_ctx *ctx, int32_t brick_id) {
     return BH_OK;
 }
 
-int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples) {
+int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples, uint64_t *lane_slots) {
     int st = ctx_check(ctx);
     if (st) return st;
-    unsigned long long w[2] = {0, 0};
+    unsigned long long w[3] = {0, 0, 0};
     BH_HIP(hipMemcpyAsync(w, ctx->d_counters + kWorkOffset, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     if (steps) *steps = w[0];
     if (samples) *samples = w[1];
+    if (lane_slots) *lane_slots = w[2];
     return BH_OK;
 }
 
@@ -1023,39 +1203,7 @@ int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, 
     RenderFn fn = pick_kernel(precision, b.storage, b.layout);
     if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick in the SoA layout");
     RenderArgs A{};
-    if ((st = camera_constants(*scene, A.cam))) return st;
-    const double a = scene->spin;
-    A.a = a;
-    A.r_plus = 1.0 + std::sqrt(1.0 - a * a);
-    A.r_cap = A.r_plus * (1.0 + 2.0e-4);
-    A.r_island = 1.5 * A.r_plus;
-    A.th_island = kIslandTheta;
-    A.pth_island = kIslandPtheta;
-    A.step = scene->step;
-    A.adaptive = scene->step_mode;
-    A.r_esc = scene->escape_radius;
-    A.L = scene->affine_to_length_cm;
-    A.tau_max = scene->max_optical_depth;
-    A.W = scene->width;
-    A.H = scene->height;
-    A.max_steps = scene->max_steps;
-    A.step_f = float(A.step);
-    A.r_plus_f = float(A.r_plus);
-    A.r_cap_f = float(A.r_cap);
-    A.r_esc_f = float(A.r_esc);
-    A.L_f = float(A.L);
-    A.tau_max_f = float(A.tau_max);
-    A.coeffs = b.coeffs;
-    A.nr = b.nr;
-    A.nt = b.nt;
-    A.np = b.np;
-    const unsigned esz = b.storage == BH_STORE_F32 ? 4u : 2u;
-    A.plane_bytes = unsigned(b.plane * esz);
-    A.total_bytes = unsigned(b.bytes);
-    A.grids = b.grids;
-    A.hr = b.hr;
-    A.ht = b.ht;
-    A.hp = b.hp;
+    if ((st = setup_args(*scene, b, A))) return st;
     A.iquv = d_iquv;
     A.steps = d_steps;
     A.status = d_status;
@@ -1122,6 +1270,259 @@ int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     return st;
+}
+
+}  // extern "C"
+
+struct bh_batch {
+    bh_ctx *ctx = nullptr;
+    int precision = 0, storage = 0, layout = 0, n = 0;
+    bh::RenderArgs *d_descs = nullptr;
+    int *d_slots = nullptr;
+    float *d_grids = nullptr;
+    void *d_coeffs = nullptr;
+    size_t coeff_bytes = 0;
+    int ng = 0, queues = 1, q_tiles = 0;
+    long long total_pixels = 0;
+    bh::BatchFn fn = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *g_iquv = nullptr, *g_steps = nullptr, *g_status = nullptr;
+    int g_grid = 0, g_block = 0, g_spc = 0;
+};
+
+namespace bh {
+
+static int batch_args(bh_batch *bt, void *d_iquv, int32_t *d_steps, uint8_t *d_status, BatchArgs &B, int &grid,
+                      int &block) {
+    bh_ctx *ctx = bt->ctx;
+    const Tuning &u = ctx->tuning[bt->precision];
+    B.descs = bt->d_descs;
+    B.slots = bt->d_slots;
+    B.grids = bt->d_grids;
+    B.ng = bt->ng;
+    B.coeffs = bt->d_coeffs;
+    B.total_bytes = unsigned(bt->coeff_bytes);
+    B.iquv = d_iquv;
+    B.steps = d_steps;
+    B.status = d_status;
+    B.counters = ctx->d_counters;
+    B.work = reinterpret_cast<unsigned long long *>(ctx->d_counters + kWorkOffset);
+    B.queues = bt->queues;
+    B.q_tiles = bt->q_tiles;
+    B.steps_per_sync = u.steps_per_sync;
+    int per_cu = u.blocks_per_cu;
+    if (per_cu <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bt->fn, u.block_threads, 0) != hipSuccess || nb <= 0) nb = 1;
+        per_cu = nb;
+    }
+    block = u.block_threads;
+    grid = ctx->cus * per_cu;
+    const int waves_per_block = block / 64;
+    const int max_useful = (bt->queues * bt->q_tiles + waves_per_block - 1) / waves_per_block;
+    if (grid > max_useful) grid = max_useful;
+    if (grid < 1) grid = 1;
+    return BH_OK;
+}
+
+}  // namespace bh
+
+extern "C" {
+
+int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t precision, int64_t *offsets,
+                    bh_batch **out) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!items || !out || n < 1) return set_error(BH_ERR_VALUE, "batch needs at least one item");
+    if (n >= (1 << 11)) return set_error(BH_ERR_VALUE, "batch holds at most 2047 renders");
+    if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32 && precision != BH_PREC_MIXED)
+        return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
+    *out = nullptr;
+    std::vector<int> uniq;
+    std::vector<int> kmap(n);
+    int storage = -1, layout = -1;
+    for (int i = 0; i < n; ++i) {
+        if ((st = validate_scene(items[i].scene))) return st;
+        const int id = items[i].brick_id;
+        if (id < 0 || size_t(id) >= ctx->bricks.size() || !ctx->bricks[id].live)
+            return set_error(BH_ERR_STATE, "unknown brick id %d", id);
+        const BrickDev &b = ctx->bricks[id];
+        if (storage < 0) {
+            storage = b.storage;
+            layout = b.layout;
+        } else if (storage != b.storage || layout != b.layout) {
+            return set_error(BH_ERR_VALUE, "all bricks of a batch must share storage and layout");
+        }
+        int k = -1;
+        for (size_t j = 0; j < uniq.size(); ++j)
+            if (uniq[j] == id) k = int(j);
+        if (k < 0) {
+            k = int(uniq.size());
+            uniq.push_back(id);
+        }
+        kmap[i] = k;
+        const long long tiles = (long long)((items[i].scene.width + 7) / 8) * ((items[i].scene.height + 7) / 8);
+        if (tiles >= (1 << 20)) return set_error(BH_ERR_VALUE, "render %d has too many 8x8 tiles", i);
+    }
+    BatchFn fn = pick_batch_kernel(precision, storage, layout);
+    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 batch needs float32 bricks in the SoA layout");
+    // merged coefficient buffer and grid table
+    std::vector<size_t> coff(uniq.size());
+    std::vector<int> goff(uniq.size());
+    size_t total = 0;
+    int ng = 0;
+    for (size_t k = 0; k < uniq.size(); ++k) {
+        const BrickDev &b = ctx->bricks[uniq[k]];
+        coff[k] = total;
+        total += (b.bytes + 255) & ~size_t(255);
+        goff[k] = ng;
+        ng += b.nr + b.nt + b.np + 1;
+    }
+    if (total >= (size_t(1) << 31)) return set_error(BH_ERR_VALUE, "the bricks of a batch exceed 2 GiB");
+    if (ng > kMaxGrid) return set_error(BH_ERR_VALUE, "the bricks of a batch hold more than %d grid values", kMaxGrid);
+    bh_batch *bt = new bh_batch();
+    bt->ctx = ctx;
+    bt->precision = precision;
+    bt->storage = storage;
+    bt->layout = layout;
+    bt->n = n;
+    bt->fn = fn;
+    bt->ng = ng;
+    bt->coeff_bytes = total;
+    auto fail = [&](int code) {
+        bh_batch_destroy(bt);
+        return code;
+    };
+    if (hipMalloc(&bt->d_coeffs, total) != hipSuccess) return fail(set_error(BH_ERR_MEMORY, "batch brick buffer allocation failed"));
+    std::vector<float> grids(ng);
+    for (size_t k = 0; k < uniq.size(); ++k) {
+        const BrickDev &b = ctx->bricks[uniq[k]];
+        if (hipMemcpyAsync(static_cast<char *>(bt->d_coeffs) + coff[k], b.coeffs, b.bytes, hipMemcpyDeviceToDevice,
+                           ctx->stream) != hipSuccess)
+            return fail(set_error(BH_ERR_DEVICE, "batch brick copy failed"));
+        std::copy(b.hgrids.begin(), b.hgrids.end(), grids.begin() + goff[k]);
+        grids[goff[k] + b.nr + b.nt + b.np] = __builtin_huge_valf();
+    }
+    if (hipMalloc(&bt->d_grids, sizeof(float) * ng) != hipSuccess) return fail(set_error(BH_ERR_MEMORY, "batch grid allocation failed"));
+    if (hipMemcpyAsync(bt->d_grids, grids.data(), sizeof(float) * ng, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return fail(set_error(BH_ERR_DEVICE, "batch grid copy failed"));
+    // render descriptors
+    std::vector<RenderArgs> descs(n);
+    std::vector<std::pair<float, int>> order;   // (distance, render << 20 | tile)
+    long long pix = 0;
+    const Tuning &u = ctx->tuning[precision];
+    for (int i = 0; i < n; ++i) {
+        const BrickDev &b = ctx->bricks[uniq[kmap[i]]];
+        RenderArgs &A = descs[i];
+        A = RenderArgs{};
+        if ((st = setup_args(items[i].scene, b, A))) return fail(st);
+        A.coeffs = bt->d_coeffs;
+        A.total_bytes = unsigned(total);
+        A.brick_base = unsigned(coff[kmap[i]]);
+        A.grid_off = goff[kmap[i]];
+        A.out_off = pix;
+        if (offsets) offsets[i] = pix;
+        pix += (long long)items[i].scene.width * items[i].scene.height;
+        const int tx = A.tiles_x, ty = (A.H + 7) / 8;
+        std::vector<float> dist;
+        if (u.order == 1) dist = tile_distances(items[i].scene, A.cam, tx, ty);
+        for (int t = 0; t < tx * ty; ++t)
+            order.push_back({u.order == 1 ? dist[t] : float(order.size()), (i << 20) | t});
+    }
+    if (offsets) offsets[n] = pix;
+    bt->total_pixels = pix;
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<float, int> &x, const std::pair<float, int> &y) { return x.first < y.first; });
+    const int n_tiles = int(order.size());
+    bt->queues = u.queues < n_tiles ? u.queues : n_tiles;
+    bt->q_tiles = (n_tiles + bt->queues - 1) / bt->queues;
+    std::vector<int> slots(size_t(bt->queues) * bt->q_tiles, -1);
+    for (int k = 0; k < n_tiles; ++k) slots[size_t(k % bt->queues) * bt->q_tiles + k / bt->queues] = order[k].second;
+    if (hipMalloc(&bt->d_descs, sizeof(RenderArgs) * n) != hipSuccess ||
+        hipMalloc(&bt->d_slots, sizeof(int) * slots.size()) != hipSuccess)
+        return fail(set_error(BH_ERR_MEMORY, "batch table allocation failed"));
+    if (hipMemcpyAsync(bt->d_descs, descs.data(), sizeof(RenderArgs) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(bt->d_slots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(set_error(BH_ERR_DEVICE, "batch table upload failed"));
+    *out = bt;
+    return BH_OK;
+}
+
+int bh_batch_launch(bh_batch *bt, void *d_iquv, int32_t *d_steps, uint8_t *d_status) {
+    clear_error();
+    if (!bt) return set_error(BH_ERR_STATE, "batch is NULL");
+    int st = ctx_check(bt->ctx);
+    if (st) return st;
+    if (!d_iquv) return set_error(BH_ERR_VALUE, "iquv must not be NULL");
+    BatchArgs B{};
+    int grid = 0, block = 0;
+    if ((st = batch_args(bt, d_iquv, d_steps, d_status, B, grid, block))) return st;
+    if ((st = fill_u32(bt->ctx, bt->ctx->d_counters, 0u, kCounterWords))) return st;
+    begin_timing(bt->ctx);
+    void *args[] = {&B};
+    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(bt->fn), dim3(grid), dim3(block), args, 0, bt->ctx->stream));
+    end_timing(bt->ctx, 1);
+    return BH_OK;
+}
+
+int bh_batch_launch_graph(bh_batch *bt, void *d_iquv, int32_t *d_steps, uint8_t *d_status) {
+    clear_error();
+    if (!bt) return set_error(BH_ERR_STATE, "batch is NULL");
+    int st = ctx_check(bt->ctx);
+    if (st) return st;
+    if (!d_iquv) return set_error(BH_ERR_VALUE, "iquv must not be NULL");
+    bh_ctx *ctx = bt->ctx;
+    BatchArgs B{};
+    int grid = 0, block = 0;
+    if ((st = batch_args(bt, d_iquv, d_steps, d_status, B, grid, block))) return st;
+    const bool same = bt->exec && bt->g_iquv == d_iquv && bt->g_steps == (void *)d_steps && bt->g_status == (void *)d_status &&
+                      bt->g_grid == grid && bt->g_block == block && bt->g_spc == B.steps_per_sync;
+    if (!same) {
+        if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+        if (bt->graph) (void)hipGraphDestroy(bt->graph);
+        bt->exec = nullptr;
+        bt->graph = nullptr;
+        BH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        // counter reset by a fill kernel (no memset node in the graph), then the persistent kernel
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kCounterWords), 0, ctx->stream, ctx->d_counters, 0u, kCounterWords);
+        void *args[] = {&B};
+        hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(bt->fn), dim3(grid), dim3(block), args, 0, ctx->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess || !g)
+            return set_error(BH_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        bt->graph = g;
+        BH_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+        bt->g_iquv = d_iquv;
+        bt->g_steps = d_steps;
+        bt->g_status = d_status;
+        bt->g_grid = grid;
+        bt->g_block = block;
+        bt->g_spc = B.steps_per_sync;
+    }
+    begin_timing(ctx);
+    BH_HIP(hipGraphLaunch(bt->exec, ctx->stream));
+    end_timing(ctx, 1);
+    return BH_OK;
+}
+
+int bh_batch_destroy(bh_batch *bt) {
+    if (!bt) return BH_OK;
+    if (bt->ctx) {
+        (void)hipSetDevice(bt->ctx->device);
+        (void)hipStreamSynchronize(bt->ctx->stream);
+    }
+    if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+    if (bt->graph) (void)hipGraphDestroy(bt->graph);
+    if (bt->d_descs) (void)hipFree(bt->d_descs);
+    if (bt->d_slots) (void)hipFree(bt->d_slots);
+    if (bt->d_grids) (void)hipFree(bt->d_grids);
+    if (bt->d_coeffs) (void)hipFree(bt->d_coeffs);
+    delete bt;
+    return BH_OK;
 }
 
 }  // extern "C"

@@ -172,9 +172,11 @@ typedef struct bh_tuning {
 
 int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t);
 
-/* Work done by the last render launch on this context: total RK2 steps and
- * total transfer segments that sampled inside the brick (waits for it). */
-int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples);
+/* Work done by the last render launch on this context (waits for it): total RK2
+ * steps, transfer segments that sampled inside the brick, and lane slots
+ * (wave loop iterations x 64), so steps / lane_slots is the SIMD lane utilisation
+ * after ray termination and compaction.  Any pointer may be NULL. */
+int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples, uint64_t *lane_slots);
 int bh_ctx_get_tuning(bh_ctx *ctx, int32_t precision, bh_tuning *t);
 
 /* Host outputs: iquv is H*W*4 doubles for BH_PREC_FP64, floats otherwise;
@@ -184,6 +186,25 @@ int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id,
 /* Device outputs, enqueued on the context stream (bh_ctx_synchronize to wait). */
 int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id,
                      int32_t precision, void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+
+/* ---- batched persistent launch --------------------------------------------
+ * Many renders (own image size, spin, camera and brick) traced by one persistent
+ * kernel launch.  All bricks of a batch must share storage and layout.  Render i
+ * writes pixels [offsets[i], offsets[i+1]) of the batch outputs (offsets has n+1
+ * entries, filled by bh_batch_create).  bh_batch_launch_graph captures the launch
+ * (counter reset + kernel) in a hipGraph on first use and replays it afterwards. */
+typedef struct bh_batch bh_batch;
+typedef struct bh_batch_item {
+    bh_scene_desc scene;
+    int32_t brick_id;
+    int32_t reserved;
+} bh_batch_item;
+
+int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t precision,
+                    int64_t *offsets, bh_batch **out);
+int bh_batch_launch(bh_batch *batch, void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+int bh_batch_launch_graph(bh_batch *batch, void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+int bh_batch_destroy(bh_batch *batch);
 
 #ifdef __cplusplus
 }
