@@ -50,6 +50,20 @@ class BatchItem(C.Structure):
     _fields_ = [("scene", SceneDesc), ("brick_id", C.c_int32), ("reserved", C.c_int32)]
 
 
+class KnotDesc(C.Structure):
+    """bh_knot_desc."""
+    _fields_ = [("cam_r", C.c_double), ("step", C.c_double), ("escape_radius", C.c_double), ("r_lo", C.c_double),
+                ("r_hi", C.c_double), ("step_mode", C.c_int32), ("max_steps", C.c_int32), ("n_knots", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class SurrogateDesc(C.Structure):
+    """bh_surrogate_desc."""
+    _fields_ = [("in_mul", C.c_float * 4), ("in_add", C.c_float * 4), ("out_mul", _fp), ("out_add", _fp),
+                ("n_knots", C.c_int32), ("fallback_precision", C.c_int32), ("crit_distance", C.c_float),
+                ("flag_max", C.c_float), ("cap_lo", C.c_float), ("cap_hi", C.c_float)]
+
+
 # (name, restype, argtypes) for every symbol include/bhgrrt.h declares
 SIGNATURES = [
     ("bh_core_version", C.c_char_p, []),
@@ -90,6 +104,27 @@ SIGNATURES = [
     ("bh_render", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, _i32p, _u8p]),
     ("bh_render_device", C.c_int,
      [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_mlp_create", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("bh_mlp_destroy", C.c_int, [C.c_void_p]),
+    ("bh_mlp_shape", C.c_int, [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_int64)]),
+    ("bh_mlp_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("bh_mlp_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("bh_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
+    ("bh_mlp_gradient", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _fp, _fp]),
+    ("bh_mlp_train_step", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_float, _fp]),
+    ("bh_columns_affine", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, _fp, _fp]),
+    ("bh_geodesic_knots", C.c_int,
+     [C.c_void_p, C.POINTER(KnotDesc), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_surrogate_render_device", C.c_int,
+     [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.POINTER(SurrogateDesc), C.c_void_p, C.c_size_t,
+      C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_knot_render_device", C.c_int,
+     [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_void_p, C.c_int64, C.POINTER(SurrogateDesc), C.c_void_p,
+      C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_surrogate_last_fallback", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("bh_surrogate_work_bytes", C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]),
 ]
 
 _lock = threading.Lock()

@@ -8,6 +8,7 @@ one Kerr ray per pixel, RK2 geodesic, brick transfer, image (H, W, 4).
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -47,9 +48,16 @@ class Renderer:
         _lib.check(self._lib.bh_ctx_create(int(device), C.byref(h)))
         self.ctx = h
         self.device = int(device)
+        self._dependents = weakref.WeakSet()   # objects holding library handles on this context
+
+    def adopt(self, obj) -> None:
+        """Register an object whose ``close()`` must run before the context is destroyed."""
+        self._dependents.add(obj)
 
     def close(self) -> None:
         if self.ctx:
+            for obj in list(self._dependents):
+                obj.close()
             self._lib.bh_ctx_destroy(self.ctx)
             self.ctx = None
 
@@ -156,9 +164,9 @@ class DeviceBuffer:
         _lib.check(self.r._lib.bh_copy_to_device(self.r.ctx, self.ptr, src.ctypes.data_as(C.c_void_p), src.nbytes))
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.r.ctx:
             _lib.check(self.r._lib.bh_device_free(self.r.ctx, self.ptr))
-            self.ptr = None
+        self.ptr = None
 
 
 class Batch:
@@ -181,6 +189,7 @@ class Batch:
         self.offsets = [int(off[i]) for i in range(n + 1)]
         self.pixels = self.offsets[-1]
         self._bufs = None
+        renderer.adopt(self)
 
     def buffers(self):
         if self._bufs is None:
@@ -214,9 +223,9 @@ class Batch:
             for b in self._bufs:
                 b.free()
             self._bufs = None
-        if self.handle:
+        if self.handle and self.r.ctx:
             self.r._lib.bh_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

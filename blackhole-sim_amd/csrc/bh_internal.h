@@ -76,7 +76,10 @@ struct bh_ctx {
     int *d_order = nullptr;             // tile schedule of the last scene (device)
     size_t order_cap = 0;
     std::vector<double> order_key;      // scene parameters the schedule was built for
+    unsigned *sur_count = nullptr;      // fallback counter of the last surrogate render (in its work buffer)
 };
+
+struct bh_mlp;
 
 namespace bh {
 int ctx_check(bh_ctx *ctx);
@@ -91,4 +94,10 @@ bh_ctx *default_ctx(int *status);
 std::mutex &default_ctx_mutex();
 // Grow-only device staging buffer owned by the default context.
 int staging(size_t bytes, void **ptr);
+// MLP inference enqueued on the MLP's context stream, without timing events:
+// d_x [rows][n_in] -> d_y [rows][n_out] or [n_out][rows] (transposed).
+int mlp_forward_enqueue(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int transposed);
+int mlp_in_cols(const bh_mlp *m);
+bh_ctx *mlp_context(const bh_mlp *m);
+int mlp_out_cols(const bh_mlp *m);
 }  // namespace bh

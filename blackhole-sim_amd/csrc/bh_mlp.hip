@@ -1,0 +1,601 @@
+// MLP surrogate: BF16 MFMA GEMMs with FP32 accumulation, forward / backward / Adam on gfx950.
+//
+// Network: x (n_in) -> [Linear width, SiLU] x depth -> Linear n_out.  Parameters are FP32
+// masters updated by Adam, with two BF16 copies feeding v_mfma_f32_16x16x32_bf16: W as
+// stored ([fan_in][fan_out], used by the input-gradient GEMM) and W^T ([fan_out][fan_in],
+// used by the forward GEMM), so both GEMMs read operands with k contiguous.  Shapes are
+// padded to multiples of 8 (n_in, n_out), so every global operand load is one 16-byte
+// vector; padded rows and columns hold zeros and receive zero gradients, so they stay zero.
+//
+// Parameter vector layout (FP32, padded): for each layer l, W_l[fi_pad][fo_pad] row-major,
+// then b_l[fo_pad].
+
+#include "bh_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace bh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 64;     // output tile edge per workgroup
+constexpr int kKStep = 32;    // K per MFMA pass
+constexpr int kLdsK = kKStep + 8;
+
+enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3, EPI_ATOMIC = 4 };
+
+struct GemmArgs {
+    const __bf16 *A;   // op(A): M x K; stored [M][K] (A_KC) or [K][M]
+    const __bf16 *B;   // op(B): K x N; stored [N][K] (B_KC) or [K][N]
+    int M, N, K;
+    int lda, ldb;
+    int n_store;       // columns n < n_store are written
+    const float *bias; // EPI_BIAS*: N values
+    float *Z;          // EPI_BIAS_SILU: pre-activation out (optional), EPI_DSILU: in; [M][ldz]
+    int ldz;
+    float *Cf;         // FP32 result (optional)
+    __bf16 *Cb;        // BF16 result (optional)
+    int ldc;
+    int c_trans;       // Cf stored [n][m] (ld = ldc) instead of [m][n]
+    int k_split;       // K range of one grid.z slice (multiple of kKStep)
+};
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// Stage a 64 x 32 (rows x k) operand tile into LDS as [row][k].
+template <bool KC>
+__device__ __forceinline__ void stage(const __bf16 *src, int ld, int rows, int K, int r0, int k0,
+                                      __bf16 (*dst)[kLdsK], int tid) {
+    if constexpr (KC) {   // stored [row][k]
+        const int r = tid >> 2, kk = (tid & 3) * 8;
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (r0 + r < rows && k0 + kk < K) v = *reinterpret_cast<const uint4 *>(src + size_t(r0 + r) * ld + k0 + kk);
+        *reinterpret_cast<uint4 *>(&dst[r][kk]) = v;
+    } else {              // stored [k][row]
+        const int k = tid >> 3, rr = (tid & 7) * 8;
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (k0 + k < K && r0 + rr < rows) v = *reinterpret_cast<const uint4 *>(src + size_t(k0 + k) * ld + r0 + rr);
+        const unsigned short *e = reinterpret_cast<const unsigned short *>(&v);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) *reinterpret_cast<unsigned short *>(&dst[rr + i][k]) = e[i];
+    }
+}
+
+// C tile (64 x 64) = op(A) op(B) over this slice's K range.  Four waves, each a 32 x 32 quadrant of
+// 2 x 2 MFMA 16x16x32 tiles.  Lane l of an MFMA holds A[l%16][(l/16)*8 .. +7] and B[(l/16)*8 .. +7][l%16]
+// and receives D[(l/16)*4 + q][l%16], q = 0..3.
+template <bool A_KC, bool B_KC, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+    __shared__ __align__(16) __bf16 sA[kTile][kLdsK];
+    __shared__ __align__(16) __bf16 sB[kTile][kLdsK];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int kbeg = blockIdx.z * g.k_split;
+    const int kend = min(g.K, kbeg + g.k_split);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int kb = (lane >> 4) * 8, rl = lane & 15;
+    for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
+        stage<A_KC>(g.A, g.lda, g.M, kend, m0, k0, sA, tid);
+        stage<B_KC>(g.B, g.ldb, g.N, kend, n0, k0, sB, tid);
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + rl][kb]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8 *>(&sB[wn + j * 16 + rl][kb]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
+                const int n = n0 + wn + j * 16 + rl;
+                if (m >= g.M || n >= g.n_store) continue;
+                float v = acc[i][j][q];
+                if constexpr (EPI == EPI_BIAS_SILU) {
+                    v += g.bias[n];
+                    if (g.Z) g.Z[size_t(m) * g.ldz + n] = v;
+                    v = silu(v);
+                } else if constexpr (EPI == EPI_BIAS) {
+                    v += g.bias[n];
+                } else if constexpr (EPI == EPI_DSILU) {
+                    v *= dsilu(g.Z[size_t(m) * g.ldz + n]);
+                }
+                const size_t ci = g.c_trans ? size_t(n) * g.ldc + m : size_t(m) * g.ldc + n;
+                if constexpr (EPI == EPI_ATOMIC) {
+                    atomicAdd(&g.Cf[ci], v);
+                } else {
+                    if (g.Cf) g.Cf[ci] = v;
+                    if (g.Cb) g.Cb[ci] = __bf16(v);
+                }
+            }
+}
+
+// FP32 [rows][n] -> BF16 [rows][n_pad], zero padded
+__global__ void pack_input_kernel(const float *x, int rows, int n, int n_pad, __bf16 *y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(rows) * n_pad) return;
+    const int c = int(i % n_pad);
+    const size_t r = i / n_pad;
+    y[i] = __bf16(c < n ? x[r * n + c] : 0.0f);
+}
+
+__global__ void cast_bf16_kernel(const float *x, __bf16 *y, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = __bf16(x[i]);
+}
+
+// BF16 W^T of one layer: wt[n][k] = w[k][n]
+__global__ void transpose_bf16_kernel(const float *w, int K, int N, __bf16 *wt) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(K) * N) return;
+    const int k = int(i / N), n = int(i % N);
+    wt[size_t(n) * K + k] = __bf16(w[i]);
+}
+
+// Column sums of an [M][N] FP32 matrix into out (zeroed beforehand), 256 rows per block row.
+__global__ void colsum_kernel(const float *x, int M, int N, float *out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int r0 = blockIdx.y * 256, r1 = min(M, r0 + 256);
+    float s = 0.f;
+    for (int m = r0; m < r1; ++m) s += x[size_t(m) * N + n];
+    atomicAdd(&out[n], s);
+}
+
+// Weighted squared error: L = sum_{i,o} w_io (y_io - t_io)^2 / rows,  dY = 2 w_io (y - t) / rows, with
+// w_io = w_o * rw_i for o < masked (rw null: 1) and w_o otherwise.  y is [rows][n_pad]; t is [rows][n];
+// padded columns get zero gradient.
+__global__ void mse_grad_kernel(const float *y, const float *t, const float *w, const float *rw, int masked, int rows,
+                                int n, int n_pad, float *dy, __bf16 *dyb, float *loss) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < size_t(rows) * n_pad) {
+        const int o = int(i % n_pad);
+        const size_t r = i / n_pad;
+        float gr = 0.f;
+        if (o < n) {
+            const float wo = (rw && o < masked) ? w[o] * rw[r] : w[o];
+            const float d = y[i] - t[r * n + o];
+            gr = wo == 0.f ? 0.f : 2.0f * wo * d / float(rows);
+            l = wo == 0.f ? 0.f : wo * d * d / float(rows);
+        }
+        dy[i] = gr;
+        dyb[i] = __bf16(gr);
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+}
+
+__global__ void adam_kernel(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2, __bf16 *wb) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    wb[i] = __bf16(wi);
+}
+
+}  // namespace bh
+
+using namespace bh;
+
+struct bh_mlp {
+    bh_ctx *ctx = nullptr;
+    int n_in = 0, n_in_pad = 0, width = 0, depth = 0, n_out = 0, n_out_pad = 0;
+    std::vector<int> fi, fo;                   // padded fan-in / fan-out per linear layer
+    std::vector<size_t> w_off, b_off;          // offsets into the parameter vector
+    size_t n_params = 0;
+    float *w32 = nullptr;                      // FP32 master parameters
+    __bf16 *wb = nullptr;                      // BF16 copy, same layout
+    __bf16 *wtb = nullptr;                     // BF16 W^T per layer (at w_off)
+    float *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    long long adam_t = 0;
+    int cap = 0;                               // rows the activation buffers hold
+    bool cap_train = false;                    // training buffers present
+    __bf16 *xb = nullptr;                      // [cap][n_in_pad]
+    std::vector<__bf16 *> act;                 // per hidden layer [cap][width]
+    std::vector<float *> zpre;                 // per hidden layer pre-activation [cap][width]
+    float *y = nullptr, *dy = nullptr;         // [cap][n_out_pad]
+    __bf16 *dyb = nullptr, *dab = nullptr, *dab2 = nullptr;
+    float *dz = nullptr;
+    float *loss = nullptr;
+};
+
+namespace bh {
+
+template <bool A_KC, bool B_KC, int EPI>
+static int run_gemm(bh_ctx *ctx, const GemmArgs &g0, int splits = 1) {
+    GemmArgs g = g0;
+    if (g.n_store == 0) g.n_store = g.N;
+    g.k_split = ((g.K + splits - 1) / splits + kKStep - 1) / kKStep * kKStep;
+    const int z = (g.K + g.k_split - 1) / g.k_split;
+    dim3 grid((g.N + kTile - 1) / kTile, (g.M + kTile - 1) / kTile, z);
+    hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI>), grid, dim3(256), 0, ctx->stream, g);
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
+static unsigned blocks_for(size_t n) { return unsigned((n + 255) / 256); }
+
+static void free_acts(bh_mlp *m) {
+    if (m->xb) (void)hipFree(m->xb);
+    for (auto p : m->act) (void)hipFree(p);
+    for (auto p : m->zpre)
+        if (p) (void)hipFree(p);
+    for (void *p : {(void *)m->y, (void *)m->dy, (void *)m->dyb, (void *)m->dab, (void *)m->dab2, (void *)m->dz})
+        if (p) (void)hipFree(p);
+    m->xb = nullptr;
+    m->act.clear();
+    m->zpre.clear();
+    m->y = m->dy = m->dz = nullptr;
+    m->dyb = m->dab = m->dab2 = nullptr;
+    m->cap = 0;
+    m->cap_train = false;
+}
+
+static int ensure_cap(bh_mlp *m, int rows, bool train) {
+    if (rows <= m->cap && (!train || m->cap_train)) return BH_OK;
+    if (rows < m->cap) rows = m->cap;
+    train = train || m->cap_train;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    free_acts(m);
+    const size_t R = size_t(rows);
+    const int widest = m->width > m->n_out_pad ? m->width : m->n_out_pad;
+    bool ok = hipMalloc(&m->xb, R * m->n_in_pad * 2) == hipSuccess;
+    for (int l = 0; l < m->depth && ok; ++l) {
+        __bf16 *a = nullptr;
+        float *z = nullptr;
+        ok = hipMalloc(&a, R * m->width * 2) == hipSuccess;
+        if (ok && train) ok = hipMalloc(&z, R * m->width * 4) == hipSuccess;
+        m->act.push_back(a);
+        m->zpre.push_back(z);
+    }
+    ok = ok && hipMalloc(&m->y, R * m->n_out_pad * 4) == hipSuccess;
+    if (ok && train)
+        ok = hipMalloc(&m->dy, R * m->n_out_pad * 4) == hipSuccess && hipMalloc(&m->dyb, R * m->n_out_pad * 2) == hipSuccess &&
+             hipMalloc(&m->dab, R * widest * 2) == hipSuccess && hipMalloc(&m->dab2, R * widest * 2) == hipSuccess &&
+             hipMalloc(&m->dz, R * widest * 4) == hipSuccess;
+    if (!ok) {
+        free_acts(m);
+        return set_error(BH_ERR_MEMORY, "MLP activation allocation for %d rows failed", rows);
+    }
+    m->cap = rows;
+    m->cap_train = train;
+    return BH_OK;
+}
+
+static void refresh_transposed(bh_mlp *m) {
+    for (int l = 0; l <= m->depth; ++l)
+        hipLaunchKernelGGL(transpose_bf16_kernel, dim3(blocks_for(size_t(m->fi[l]) * m->fo[l])), dim3(256), 0,
+                           m->ctx->stream, m->w32 + m->w_off[l], m->fi[l], m->fo[l], m->wtb + m->w_off[l]);
+}
+
+// BF16 copies (W and W^T) of the FP32 masters.
+static int refresh_bf16(bh_mlp *m) {
+    hipLaunchKernelGGL(cast_bf16_kernel, dim3(blocks_for(m->n_params)), dim3(256), 0, m->ctx->stream, m->w32, m->wb,
+                       m->n_params);
+    refresh_transposed(m);
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
+// Forward of `rows` rows of x (FP32 [rows][n_in]).  Output: FP32 into m->y (ld n_out_pad) when
+// y_out is null, else the n_out real columns into y_out ([rows][ld_out], or [n_out][ld_out] when
+// trans).  train: keep the pre-activations for backward.
+static int forward(bh_mlp *m, const float *d_x, int rows, bool train, float *y_out, int trans, int ld_out) {
+    bh_ctx *ctx = m->ctx;
+    hipLaunchKernelGGL(pack_input_kernel, dim3(blocks_for(size_t(rows) * m->n_in_pad)), dim3(256), 0, ctx->stream, d_x,
+                       rows, m->n_in, m->n_in_pad, m->xb);
+    const __bf16 *in = m->xb;
+    int st;
+    for (int l = 0; l <= m->depth; ++l) {
+        GemmArgs g{};
+        g.A = in;
+        g.lda = m->fi[l];
+        g.B = m->wtb + m->w_off[l];
+        g.ldb = m->fi[l];
+        g.M = rows;
+        g.K = m->fi[l];
+        g.N = m->fo[l];
+        g.bias = m->w32 + m->b_off[l];
+        if (l < m->depth) {
+            g.Z = train ? m->zpre[l] : nullptr;
+            g.ldz = g.N;
+            g.Cb = m->act[l];
+            g.ldc = g.N;
+            st = run_gemm<true, true, EPI_BIAS_SILU>(ctx, g);
+            in = m->act[l];
+        } else if (y_out) {
+            g.Cf = y_out;
+            g.n_store = m->n_out;
+            g.c_trans = trans;
+            g.ldc = ld_out;
+            st = run_gemm<true, true, EPI_BIAS>(ctx, g);
+        } else {
+            g.Cf = m->y;
+            g.ldc = g.N;
+            st = run_gemm<true, true, EPI_BIAS>(ctx, g);
+        }
+        if (st) return st;
+    }
+    return BH_OK;
+}
+
+// Backward of the weighted MSE loss at the activations of the last forward; gradients into m->grad.
+static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, const float *d_rw, int masked) {
+    bh_ctx *ctx = m->ctx;
+    BH_HIP(hipMemsetAsync(m->grad, 0, m->n_params * 4, ctx->stream));
+    BH_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), ctx->stream));
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(size_t(rows) * m->n_out_pad)), dim3(256), 0, ctx->stream, m->y,
+                       d_t, d_w, d_rw, masked, rows, m->n_out, m->n_out_pad, m->dy, m->dyb, m->loss);
+    int st;
+    const __bf16 *dcur = m->dyb;   // gradient w.r.t. this layer's pre-activation, BF16
+    const float *dcurf = m->dy;    // the same in FP32 (bias gradient)
+    const int splits = rows >= 2048 ? rows / 1024 : 1;
+    for (int l = m->depth; l >= 0; --l) {
+        const int K = m->fi[l], N = m->fo[l];
+        const __bf16 *xin = l == 0 ? m->xb : m->act[l - 1];
+        // dW [K][N] = x^T dZ, split over rows
+        GemmArgs gw{};
+        gw.A = xin;
+        gw.lda = K;
+        gw.B = dcur;
+        gw.ldb = N;
+        gw.M = K;
+        gw.N = N;
+        gw.K = rows;
+        gw.Cf = m->grad + m->w_off[l];
+        gw.ldc = N;
+        if ((st = run_gemm<false, false, EPI_ATOMIC>(ctx, gw, splits))) return st;
+        hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256, (rows + 255) / 256), dim3(256), 0, ctx->stream, dcurf,
+                           rows, N, m->grad + m->b_off[l]);
+        if (l == 0) break;
+        // dZ_prev = (dZ W^T) * silu'(z_prev)
+        __bf16 *dnext = (dcur == m->dab) ? m->dab2 : m->dab;
+        GemmArgs gd{};
+        gd.A = dcur;
+        gd.lda = N;
+        gd.B = m->wb + m->w_off[l];   // W [K][N]: op(B)[n_out][k_in] with n_out contiguous
+        gd.ldb = N;
+        gd.M = rows;
+        gd.N = K;
+        gd.K = N;
+        gd.Z = m->zpre[l - 1];
+        gd.ldz = K;
+        gd.Cf = m->dz;
+        gd.Cb = dnext;
+        gd.ldc = K;
+        if ((st = run_gemm<true, true, EPI_DSILU>(ctx, gd))) return st;
+        dcur = dnext;
+        dcurf = m->dz;
+    }
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
+__global__ void columns_affine_kernel(float *d, size_t n, int cols, const float *mul, const float *add) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = int(i % cols);
+    d[i] = d[i] * mul[c] + add[c];
+}
+
+int mlp_in_cols(const bh_mlp *m) { return m->n_in; }
+bh_ctx *mlp_context(const bh_mlp *m) { return m->ctx; }
+int mlp_out_cols(const bh_mlp *m) { return m->n_out; }
+
+int mlp_forward_enqueue(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int transposed) {
+    const int chunk = 1 << 18;
+    int st;
+    if (rows > 0 && (st = ensure_cap(m, int(rows < chunk ? rows : chunk), false))) return st;
+    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+        const int n = int(rows - r0 < chunk ? rows - r0 : chunk);
+        float *dst = transposed ? d_y + r0 : d_y + r0 * m->n_out;
+        if ((st = forward(m, d_x + r0 * m->n_in, n, false, dst, transposed, transposed ? int(rows) : m->n_out)))
+            return st;
+    }
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
+static int mlp_check(bh_mlp *m) {
+    if (!m) return set_error(BH_ERR_VALUE, "NULL MLP handle");
+    return ctx_check(m->ctx);
+}
+
+}  // namespace bh
+
+extern "C" {
+
+int bh_mlp_create(bh_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32_t n_out, bh_mlp **out) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!out) return set_error(BH_ERR_VALUE, "NULL output handle");
+    if (n_in < 1 || n_in > 1024 || width < 8 || width > 4096 || width % 8 || depth < 1 || depth > 16 || n_out < 1 ||
+        n_out > 8192)
+        return set_error(BH_ERR_VALUE, "MLP shape out of range (width must be a multiple of 8)");
+    bh_mlp *m = new bh_mlp();
+    m->ctx = ctx;
+    m->n_in = n_in;
+    m->n_in_pad = (n_in + 7) / 8 * 8;
+    m->width = width;
+    m->depth = depth;
+    m->n_out = n_out;
+    m->n_out_pad = (n_out + 7) / 8 * 8;
+    size_t off = 0;
+    for (int l = 0; l <= depth; ++l) {
+        const int a = l == 0 ? m->n_in_pad : width, b = l == depth ? m->n_out_pad : width;
+        m->fi.push_back(a);
+        m->fo.push_back(b);
+        m->w_off.push_back(off);
+        off += size_t(a) * b;
+        m->b_off.push_back(off);
+        off += b;
+    }
+    m->n_params = off;
+    bool ok = hipMalloc(&m->w32, off * 4) == hipSuccess && hipMalloc(&m->wb, off * 2) == hipSuccess &&
+              hipMalloc(&m->wtb, off * 2) == hipSuccess && hipMalloc(&m->grad, off * 4) == hipSuccess &&
+              hipMalloc(&m->adam_m, off * 4) == hipSuccess && hipMalloc(&m->adam_v, off * 4) == hipSuccess &&
+              hipMalloc(&m->loss, 4) == hipSuccess;
+    ok = ok && hipMemsetAsync(m->w32, 0, off * 4, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(m->adam_m, 0, off * 4, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(m->adam_v, 0, off * 4, ctx->stream) == hipSuccess && refresh_bf16(m) == BH_OK &&
+         hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) {
+        bh_mlp_destroy(m);
+        return set_error(BH_ERR_MEMORY, "MLP parameter allocation failed");
+    }
+    *out = m;
+    return BH_OK;
+}
+
+int bh_mlp_destroy(bh_mlp *m) {
+    if (!m) return BH_OK;
+    if (m->ctx) {
+        (void)hipSetDevice(m->ctx->device);
+        (void)hipStreamSynchronize(m->ctx->stream);
+    }
+    free_acts(m);
+    for (void *p : {(void *)m->w32, (void *)m->wb, (void *)m->wtb, (void *)m->grad, (void *)m->adam_m,
+                    (void *)m->adam_v, (void *)m->loss})
+        if (p) (void)hipFree(p);
+    delete m;
+    return BH_OK;
+}
+
+int bh_mlp_shape(bh_mlp *m, int32_t *n_in_pad, int32_t *n_out_pad, int64_t *n_params) {
+    if (!m) return set_error(BH_ERR_VALUE, "NULL MLP handle");
+    if (n_in_pad) *n_in_pad = m->n_in_pad;
+    if (n_out_pad) *n_out_pad = m->n_out_pad;
+    if (n_params) *n_params = int64_t(m->n_params);
+    return BH_OK;
+}
+
+int bh_mlp_set_params(bh_mlp *m, const float *host) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!host) return set_error(BH_ERR_VALUE, "NULL parameter buffer");
+    BH_HIP(hipMemcpyAsync(m->w32, host, m->n_params * 4, hipMemcpyHostToDevice, m->ctx->stream));
+    if ((st = refresh_bf16(m))) return st;
+    BH_HIP(hipMemsetAsync(m->adam_m, 0, m->n_params * 4, m->ctx->stream));
+    BH_HIP(hipMemsetAsync(m->adam_v, 0, m->n_params * 4, m->ctx->stream));
+    m->adam_t = 0;
+    BH_HIP(hipStreamSynchronize(m->ctx->stream));
+    return BH_OK;
+}
+
+int bh_mlp_get_params(bh_mlp *m, float *host) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!host) return set_error(BH_ERR_VALUE, "NULL parameter buffer");
+    BH_HIP(hipMemcpyAsync(host, m->w32, m->n_params * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    BH_HIP(hipStreamSynchronize(m->ctx->stream));
+    return BH_OK;
+}
+
+int bh_mlp_forward(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int32_t transposed) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!d_x || !d_y || rows < 0 || rows > (int64_t(1) << 30)) return set_error(BH_ERR_VALUE, "bad MLP forward arguments");
+    if (rows > 0 && (st = ensure_cap(m, int(rows < (1 << 18) ? rows : (1 << 18)), false))) return st;
+    begin_timing(m->ctx);
+    if ((st = mlp_forward_enqueue(m, d_x, rows, d_y, transposed))) return st;
+    end_timing(m->ctx, rows > 0 ? 1 : 0);
+    return BH_OK;
+}
+
+int bh_columns_affine(bh_ctx *ctx, float *d, int64_t rows, int32_t cols, const float *mul, const float *add) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!d || !mul || !add || rows < 0 || cols < 1) return set_error(BH_ERR_VALUE, "bad affine arguments");
+    float *dm = nullptr;
+    if (hipMalloc(&dm, size_t(cols) * 8) != hipSuccess) return set_error(BH_ERR_MEMORY, "affine table allocation failed");
+    hipError_t e = hipMemcpyAsync(dm, mul, size_t(cols) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dm + cols, add, size_t(cols) * 4, hipMemcpyHostToDevice, ctx->stream);
+    const size_t n = size_t(rows) * cols;
+    if (e == hipSuccess && n)
+        hipLaunchKernelGGL(columns_affine_kernel, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, d, n, cols, dm, dm + cols);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dm);
+    if (e != hipSuccess) return set_error(BH_ERR_DEVICE, "affine failed: %s", hipGetErrorString(e));
+    return BH_OK;
+}
+
+// Gradient of the weighted MSE loss (targets d_t [rows][n_out], per-output weights d_w [n_out]) at the
+// current parameters, without an update.  grad_host: n_params values (padded layout) or null.
+int bh_mlp_gradient(bh_mlp *m, const float *d_x, const float *d_t, const float *d_w, const float *d_row_w,
+                    int32_t masked_cols, int32_t rows, float *grad_host, float *loss_host) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!d_x || !d_t || !d_w || rows < 1) return set_error(BH_ERR_VALUE, "bad MLP gradient arguments");
+    if ((st = ensure_cap(m, rows, true))) return st;
+    if ((st = forward(m, d_x, rows, true, nullptr, 0, 0))) return st;
+    if (masked_cols < 0 || masked_cols > m->n_out) return set_error(BH_ERR_VALUE, "masked_cols must lie in [0, n_out]");
+    if ((st = backward(m, rows, d_t, d_w, d_row_w, masked_cols))) return st;
+    if (grad_host) BH_HIP(hipMemcpyAsync(grad_host, m->grad, m->n_params * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    if (loss_host) BH_HIP(hipMemcpyAsync(loss_host, m->loss, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    BH_HIP(hipStreamSynchronize(m->ctx->stream));
+    return BH_OK;
+}
+
+// One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on a batch: forward, weighted-MSE backward,
+// update of the FP32 masters and both BF16 copies.  loss_host (optional) gets the batch loss.
+int bh_mlp_train_step(bh_mlp *m, const float *d_x, const float *d_t, const float *d_w, const float *d_row_w,
+                      int32_t masked_cols, int32_t rows, float lr, float *loss_host) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!d_x || !d_t || !d_w || rows < 1 || !(lr > 0.f)) return set_error(BH_ERR_VALUE, "bad MLP training arguments");
+    if ((st = ensure_cap(m, rows, true))) return st;
+    begin_timing(m->ctx);
+    if ((st = forward(m, d_x, rows, true, nullptr, 0, 0))) return st;
+    if (masked_cols < 0 || masked_cols > m->n_out) return set_error(BH_ERR_VALUE, "masked_cols must lie in [0, n_out]");
+    if ((st = backward(m, rows, d_t, d_w, d_row_w, masked_cols))) return st;
+    m->adam_t += 1;
+    const double b1 = 0.9, b2 = 0.999;
+    const float c1 = float(1.0 - std::pow(b1, double(m->adam_t))), c2 = float(1.0 - std::pow(b2, double(m->adam_t)));
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(m->n_params)), dim3(256), 0, m->ctx->stream, m->w32, m->grad,
+                       m->adam_m, m->adam_v, m->n_params, lr, float(b1), float(b2), 1e-8f, c1, c2, m->wb);
+    refresh_transposed(m);
+    BH_HIP(hipGetLastError());
+    end_timing(m->ctx, 1);
+    if (loss_host) {
+        BH_HIP(hipMemcpyAsync(loss_host, m->loss, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+        BH_HIP(hipStreamSynchronize(m->ctx->stream));
+    }
+    return BH_OK;
+}
+
+}  // extern "C"

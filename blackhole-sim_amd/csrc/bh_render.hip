@@ -74,6 +74,8 @@ struct RenderArgs {
     unsigned *counters;
     unsigned long long *work;   // [0] RK2 steps, [1] in-brick samples of this launch
     const int *tile_order;      // queue slot -> tile, or null for row-major
+    const int *ray_list;        // list mode: queue slot -> pixel index (tile_order unused)
+    const unsigned *n_list;     // list mode: number of listed pixels (device, read at kernel start)
     int tiles_x, n_ids, queues, q_len;
     int steps_per_sync, refill_min;
 };
@@ -590,6 +592,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     RayState<R> s;
     unsigned n_steps = 0, n_samples = 0;   // this lane's work over all its rays
     unsigned long long iters = 0;          // wave loop iterations
+    int n_ids = A.n_ids, q_len = A.q_len;
+    if (A.ray_list) {   // list mode: the ids are the listed pixels, split evenly over the queues
+        n_ids = int(*A.n_list);
+        q_len = (n_ids + A.queues - 1) / A.queues;
+    }
 
     for (;;) {
         unsigned long long idle = __ballot(!active);
@@ -603,17 +610,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
                 unsigned got = 0;
                 if (lane == leader) got = atomicAdd(&A.counters[q], unsigned(need));
                 got = __builtin_amdgcn_readlane(got, leader);
-                const int q_begin = q * A.q_len;
-                const int q_end = min(q_begin + A.q_len, A.n_ids);
+                const int q_begin = q * q_len;
+                const int q_end = min(q_begin + q_len, n_ids);
                 const long long avail = (long long)q_end - ((long long)q_begin + (long long)got);
                 if (avail < need) exhausted |= (1u << q);
                 if (avail <= 0) continue;
                 const unsigned rank = lane_rank(idle);
                 if (!active && (long long)rank < avail) {
                     const int id = q_begin + int(got) + int(rank);
-                    const int tile = A.tile_order ? A.tile_order[id >> 6] : (id >> 6), w = id & 63;
-                    const int px = (tile % A.tiles_x) * 8 + (w & 7);
-                    const int py = (tile / A.tiles_x) * 8 + (w >> 3);
+                    int px, py;
+                    if (A.ray_list) {
+                        const int pix = A.ray_list[id];
+                        px = pix % A.W;
+                        py = pix / A.W;
+                    } else {
+                        const int tile = A.tile_order ? A.tile_order[id >> 6] : (id >> 6), w = id & 63;
+                        px = (tile % A.tiles_x) * 8 + (w & 7);
+                        py = (tile / A.tiles_x) * 8 + (w >> 3);
+                    }
                     if (px < A.W && py < A.H) {
                         ray_init(A, px, py, s);
                         active = true;
@@ -1053,6 +1067,69 @@ static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A)
     return BH_OK;
 }
 
+// Validate, set up and enqueue one render launch.  List mode (ray_list non-null): only the pixels
+// ray_list[0 .. *n_list) are traced, n_list read on the device; max_list bounds it for the grid size.
+static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision, void *d_iquv,
+                          int32_t *d_steps, uint8_t *d_status, const int *ray_list, const unsigned *n_list,
+                          int max_list, bool timed) {
+    int st;
+    if (!scene || !d_iquv) return set_error(BH_ERR_VALUE, "scene and iquv must not be NULL");
+    if ((st = validate_scene(*scene))) return st;
+    if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
+        return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
+    if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32 && precision != BH_PREC_MIXED)
+        return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
+    const BrickDev &b = ctx->bricks[brick_id];
+    RenderFn fn = pick_kernel(precision, b.storage, b.layout);
+    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick in the SoA layout");
+    RenderArgs A{};
+    if ((st = setup_args(*scene, b, A))) return st;
+    A.iquv = d_iquv;
+    A.steps = d_steps;
+    A.status = d_status;
+    A.counters = ctx->d_counters;
+    A.work = reinterpret_cast<unsigned long long *>(ctx->d_counters + kWorkOffset);
+    const Tuning &u = ctx->tuning[precision];
+    A.tiles_x = (A.W + 7) / 8;
+    const int tiles_y = (A.H + 7) / 8;
+    const int n_tiles = A.tiles_x * tiles_y;
+    A.tile_order = nullptr;
+    if (ray_list) {
+        A.ray_list = ray_list;
+        A.n_list = n_list;
+        A.queues = u.queues;
+        A.n_ids = max_list;   // bound only: the kernel reads the count
+        A.q_len = 0;
+    } else {
+        A.queues = u.queues < n_tiles ? u.queues : n_tiles;
+        const int q_tiles = (n_tiles + A.queues - 1) / A.queues;
+        A.q_len = q_tiles * 64;
+        A.n_ids = A.queues * A.q_len;   // padded id space: every queue slot is issued; padding maps outside the image
+        if (u.order == 1) {
+            if ((st = build_schedule(ctx, *scene, A.cam, A.tiles_x, tiles_y, A.queues, q_tiles))) return st;
+            A.tile_order = ctx->d_order;
+        }
+    }
+    A.steps_per_sync = u.steps_per_sync;
+    A.refill_min = u.refill_min;
+    int per_cu = u.blocks_per_cu;
+    if (per_cu <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, u.block_threads, 0) != hipSuccess || nb <= 0) nb = 1;
+        per_cu = nb;
+    }
+    int grid = ctx->cus * per_cu;
+    const int max_useful = (A.n_ids + u.block_threads - 1) / u.block_threads;
+    if (grid > max_useful) grid = max_useful;
+    if (grid < 1) grid = 1;
+    if ((st = fill_u32(ctx, ctx->d_counters, 0u, kCounterWords))) return st;
+    if (timed) begin_timing(ctx);
+    void *args[] = {&A};
+    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(fn), dim3(grid), dim3(u.block_threads), args, 0, ctx->stream));
+    if (timed) end_timing(ctx, 1);
+    return BH_OK;
+}
+
 }  // namespace bh
 
 using namespace bh;
@@ -1193,54 +1270,7 @@ int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, 
     clear_error();
     int st = ctx_check(ctx);
     if (st) return st;
-    if (!scene || !d_iquv) return set_error(BH_ERR_VALUE, "scene and iquv must not be NULL");
-    if ((st = validate_scene(*scene))) return st;
-    if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
-        return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
-    if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32 && precision != BH_PREC_MIXED)
-        return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
-    const BrickDev &b = ctx->bricks[brick_id];
-    RenderFn fn = pick_kernel(precision, b.storage, b.layout);
-    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick in the SoA layout");
-    RenderArgs A{};
-    if ((st = setup_args(*scene, b, A))) return st;
-    A.iquv = d_iquv;
-    A.steps = d_steps;
-    A.status = d_status;
-    A.counters = ctx->d_counters;
-    A.work = reinterpret_cast<unsigned long long *>(ctx->d_counters + kWorkOffset);
-    const Tuning &u = ctx->tuning[precision];
-    A.tiles_x = (A.W + 7) / 8;
-    const int tiles_y = (A.H + 7) / 8;
-    const int n_tiles = A.tiles_x * tiles_y;
-    A.n_ids = n_tiles * 64;
-    A.queues = u.queues < n_tiles ? u.queues : n_tiles;
-    const int q_tiles = (n_tiles + A.queues - 1) / A.queues;
-    A.q_len = q_tiles * 64;
-    A.n_ids = A.queues * A.q_len;   // padded id space: every queue slot is issued; padding maps outside the image
-    A.tile_order = nullptr;
-    if (u.order == 1) {
-        if ((st = build_schedule(ctx, *scene, A.cam, A.tiles_x, tiles_y, A.queues, q_tiles))) return st;
-        A.tile_order = ctx->d_order;
-    }
-    A.steps_per_sync = u.steps_per_sync;
-    A.refill_min = u.refill_min;
-    int per_cu = u.blocks_per_cu;
-    if (per_cu <= 0) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, u.block_threads, 0) != hipSuccess || nb <= 0) nb = 1;
-        per_cu = nb;
-    }
-    int grid = ctx->cus * per_cu;
-    const int max_useful = (A.n_ids + u.block_threads - 1) / u.block_threads;
-    if (grid > max_useful) grid = max_useful;
-    if (grid < 1) grid = 1;
-    if ((st = fill_u32(ctx, ctx->d_counters, 0u, kCounterWords))) return st;
-    begin_timing(ctx);
-    void *args[] = {&A};
-    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(fn), dim3(grid), dim3(u.block_threads), args, 0, ctx->stream));
-    end_timing(ctx, 1);
-    return BH_OK;
+    return render_enqueue(ctx, scene, brick_id, precision, d_iquv, d_steps, d_status, nullptr, nullptr, 0, true);
 }
 
 int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision, void *iquv, int32_t *steps,
@@ -1522,6 +1552,485 @@ int bh_batch_destroy(bh_batch *bt) {
     if (bt->d_grids) (void)hipFree(bt->d_grids);
     if (bt->d_coeffs) (void)hipFree(bt->d_coeffs);
     delete bt;
+    return BH_OK;
+}
+
+}  // extern "C"
+
+// ---- learned surrogate: geodesic knots and the transfer over predicted knots -----------
+
+namespace bh {
+
+constexpr int kCurveMax = 512;   // critical-curve points staged in LDS
+constexpr int kMaxKnots = 512;
+constexpr int kHeads = 3;        // captured, flag, recorded states
+
+// ZAMO tetrad and metric of a camera at (r, th), phi 0, no roll (kerr.py:67-83, 144-148), FP64.
+__device__ __forceinline__ void cam_tetrad(double a, double r, double th, CamConst &C) {
+    double s, c;
+    ::sincos(th, &s, &c);
+    const double sig = r * r + a * a * c * c;
+    const double dlt = r * r - 2.0 * r + a * a;
+    const double Aq = (r * r + a * a) * (r * r + a * a) - a * a * dlt * s * s;
+    const double s2 = s * s;
+    C.g00 = -(1.0 - 2.0 * r / sig);
+    C.g03 = -2.0 * a * r * s2 / sig;
+    C.g11 = sig / dlt;
+    C.g22 = sig;
+    C.g33 = Aq * s2 / sig;
+    const double lapse = ::sqrt(sig * dlt / Aq);
+    const double omega = 2.0 * a * r / Aq;
+    C.e_t0 = 1.0 / lapse;
+    C.e_t3 = omega / lapse;
+    C.e_r1 = ::sqrt(dlt / sig);
+    C.e_th2 = 1.0 / ::sqrt(sig);
+    C.e_ph3 = 1.0 / ::sqrt(C.g33);
+    C.tan_x = C.tan_y = 0.0;
+    C.cos_roll = 1.0;
+    C.sin_roll = 0.0;
+    C.roll = 0;
+    C.r = r;
+    C.th = th;
+    C.ph = 0.0;
+}
+
+// Redshift seen by a zero-angular-momentum observer at (r, th): g = E alpha / (E - omega L).
+__device__ __forceinline__ double zamo_redshift(double a, double r, double th, double E, double L) {
+    double s, c;
+    ::sincos(th, &s, &c);
+    const double sig = r * r + a * a * c * c, dlt = r * r - 2.0 * r + a * a;
+    const double Aq = (r * r + a * a) * (r * r + a * a) - a * a * dlt * s * s;
+    const double lapse = ::sqrt(fmax(sig * dlt / Aq, 0.0));
+    const double omega = 2.0 * a * r / Aq;
+    return E * lapse / (E - omega * L);
+}
+
+struct KnotArgs {
+    const float *params;   // n x 4: alpha, beta, a, cos i
+    long long n;
+    double cam_r, step, r_esc, r_lo, r_hi;
+    int adaptive, max_steps, K;
+    float *rows;           // n x (4K + kHeads)
+    int *steps;
+    float *weight;         // n: 1 for an unflagged ray, else 0
+};
+
+// One thread per ray: the FP64 RK2 geodesic of ray_step (same step rule, theta clamp, pole test and
+// termination order, without the transfer), recording knots from the first step whose midpoint
+// radius lies in the brick's radial range.
+__global__ __launch_bounds__(256) void knots_kernel(KnotArgs A) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const float *p = A.params + i * 4;
+    const double alpha = p[0], beta = p[1], a = p[2];
+    const double ci = fmin(fmax(double(p[3]), -1.0), 1.0);
+    const double th0 = clamp_theta<double>(::acos(ci));
+    CamConst C;
+    cam_tetrad(a, A.cam_r, th0, C);
+    double n_r = -1.0, n_th = -beta / A.cam_r, n_ph = alpha / A.cam_r;
+    const double nrm = ::sqrt(n_r * n_r + n_th * n_th + n_ph * n_ph);
+    n_r /= nrm;
+    n_th /= nrm;
+    n_ph /= nrm;
+    const double pc_t = C.e_t0, pc_r = n_r * C.e_r1, pc_th = n_th * C.e_th2, pc_ph = C.e_t3 + n_ph * C.e_ph3;
+    const double E = -(C.g00 * pc_t + C.g03 * pc_ph), L = C.g03 * pc_t + C.g33 * pc_ph;
+    double r = A.cam_r, th = th0, ph = 0.0, pr = C.g11 * pc_r, pth = C.g22 * pc_th;
+    const RayK<double> k = make_k<double>(a, E, L);
+    const double r_plus = 1.0 + ::sqrt(1.0 - a * a), r_cap = r_plus * (1.0 + 2.0e-4);
+    const double lo = kEpsTheta, hi = 3.14159265358979323846 - kEpsTheta;
+    const int K = A.K;
+    float *row = A.rows + i * (4LL * K + kHeads);
+    int rec = 0, steps = 0, status = 0;
+    bool entered = false, overflow = false, clamped = false;
+    auto put = [&](double kr, double kt, double kp) {
+        float *o = row + 4 * rec;
+        o[0] = float(kr);
+        o[1] = float(kt);
+        o[2] = float(kp);
+        o[3] = float(zamo_redshift(a, kr, kt, E, L));
+        ++rec;
+    };
+    for (;;) {
+        const double h = A.adaptive ? A.step * (r - r_plus) : A.step;
+        double nr, nt_raw, np, npr, npt, mt;
+        geo_step<double>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
+        const double nt = clamp_theta(nt_raw);
+        clamped = clamped || (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi) || (np - ph > 1.0) ||
+                  (ph - np > 1.0);
+        const bool fin = finite_(nr) && finite_(nt) && finite_(np) && finite_(npr) && finite_(npt);
+        const double rm = 0.5 * (r + nr);
+        const bool inb = rm >= A.r_lo && rm <= A.r_hi;
+        if (inb && !entered) {
+            entered = true;
+            put(r, th, ph);
+        }
+        if (entered && fin && !clamped) {   // a pole-clamped ray is flagged: its later states are not recorded
+            if (rec < K) put(nr, nt, np);
+            else if (inb) overflow = true;
+        }
+        const double r_prev = r;
+        r = nr;
+        th = nt;
+        ph = np;
+        pr = npr;
+        pth = npt;
+        ++steps;
+        if (!fin) { status = 3; break; }
+        if (nr <= r_cap) { status = 1; break; }
+        if (nr > A.r_esc && nr > r_prev) { status = 2; break; }
+        if (steps >= A.max_steps) { status = 0; break; }
+    }
+    const int n_rec = rec;
+    if (rec == 0) put(r, th, ph);   // never reached the brick: the end state stands for every knot
+    for (int j = rec; j < K; ++j) {
+        row[4 * j] = row[4 * (rec - 1)];
+        row[4 * j + 1] = row[4 * (rec - 1) + 1];
+        row[4 * j + 2] = row[4 * (rec - 1) + 2];
+        row[4 * j + 3] = row[4 * (rec - 1) + 3];
+    }
+    const bool flag = overflow || clamped || status == 3 || status == 0;
+    row[4 * K] = status == 1 ? 1.0f : 0.0f;
+    row[4 * K + 1] = flag ? 1.0f : 0.0f;
+    row[4 * K + 2] = float(n_rec);
+    if (A.steps) A.steps[i] = steps;
+    if (A.weight) A.weight[i] = flag ? 0.0f : 1.0f;
+}
+
+// Network input of every pixel: ((alpha, beta, a, cos i) - add) * mul, alpha/beta the image-plane
+// offsets (M) of the pixel's camera direction.
+__global__ void surrogate_input_kernel(RenderArgs A, float4 mul, float4 add, float *x) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= A.W * A.H) return;
+    const int px = pix % A.W, py = pix / A.W;
+    const double ndc_x = 2.0 * (double(px) + 0.5) / double(A.W) - 1.0;
+    const double ndc_y = 1.0 - 2.0 * (double(py) + 0.5) / double(A.H);
+    const CamConst &C = A.cam;
+    double n_th = -ndc_y * C.tan_y, n_ph = ndc_x * C.tan_x;
+    if (C.roll) {
+        const double t0 = C.cos_roll * n_th - C.sin_roll * n_ph;
+        const double t1 = C.sin_roll * n_th + C.cos_roll * n_ph;
+        n_th = t0;
+        n_ph = t1;
+    }
+    float *o = x + size_t(pix) * 4;
+    o[0] = (float(n_ph * C.r) - add.x) * mul.x;
+    o[1] = (float(-n_th * C.r) - add.y) * mul.y;
+    o[2] = (float(A.a) - add.z) * mul.z;
+    o[3] = (float(::cos(C.th)) - add.w) * mul.w;
+}
+
+struct SurArgs {
+    RenderArgs R;            // scene, brick view and outputs (FP32 IQUV)
+    const float *y;          // network outputs, [n_out][ld]
+    long long ld;
+    const float *mul, *add;  // denormalisation per output (device)
+    const float *curve;      // critical curve (alpha, beta) pairs
+    int n_curve, K;
+    float crit2, flag_max, cap_lo, cap_hi;
+    int *list;               // fallback pixels
+    unsigned *n_list;
+};
+
+// One wave per 8x8 pixel tile.  Pixels near the critical curve or flagged by the heads are
+// appended to the fallback list; the others integrate the Stokes transfer over the predicted
+// knots with the RK2 transfer rule (midpoint sample, Euclidean ds, accelerated_renderer.py:113-125).
+template <int STORE, int LAYOUT>
+__global__ __launch_bounds__(256) void surrogate_kernel(SurArgs S) {
+    __shared__ float s_grid[kMaxGrid + 1];
+    __shared__ float s_curve[2 * kCurveMax];
+    __shared__ float s_mul[4 * kMaxKnots + kHeads], s_add[4 * kMaxKnots + kHeads];
+    const RenderArgs &A = S.R;
+    const int ng = A.nr + A.nt + A.np;
+    for (int j = threadIdx.x; j < ng; j += blockDim.x) s_grid[j] = A.grids[j];
+    if (threadIdx.x == 0) s_grid[ng] = __builtin_huge_valf();
+    for (int j = threadIdx.x; j < 2 * S.n_curve; j += blockDim.x) s_curve[j] = S.curve[j];
+    const int n_out = 4 * S.K + kHeads;
+    for (int j = threadIdx.x; j < n_out; j += blockDim.x) {
+        s_mul[j] = S.mul[j];
+        s_add[j] = S.add[j];
+    }
+    __syncthreads();
+    const float *gr = s_grid, *gt = s_grid + A.nr, *gp = gt + A.nt;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(A.coeffs), (short)0, int(A.total_bytes), int(kRsrcWord3));
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int px = (tile % A.tiles_x) * 8 + (lane & 7), py = (tile / A.tiles_x) * 8 + (lane >> 3);
+    const bool inside = px < A.W && py < A.H;
+    const int pix = py * A.W + px;
+    const float *yc = S.y + pix;
+    const long long ld = S.ld;
+    bool fb = false;
+    float cap = 0.f;
+    if (inside) {
+        // Bardeen image coordinates of the ray (as the tile schedule) and distance to the critical curve
+        double pr, pth, pt, pph;
+        camera_ray(A, px, py, pr, pth, pt, pph);
+        const double E = -pt, L = pph, a = A.a;
+        double s0, c0;
+        ::sincos(A.cam.th, &s0, &c0);
+        const double Q = pth * pth + c0 * c0 * (-a * a * E * E + L * L / (s0 * s0));
+        const double xi = L / E;
+        const float al = float(-xi / s0);
+        const float be = float(::sqrt(fmax(Q / (E * E) + a * a * c0 * c0 - xi * xi * (c0 / s0) * (c0 / s0), 0.0)));
+        float d2 = __builtin_huge_valf();
+        for (int j = 0; j < S.n_curve; ++j) {
+            const float dx = al - s_curve[2 * j], dy = be - s_curve[2 * j + 1];
+            d2 = fminf(d2, dx * dx + dy * dy);
+        }
+        const int hc = 4 * S.K;
+        cap = yc[hc * ld] * s_mul[hc] + s_add[hc];
+        const float flag = yc[(hc + 1) * ld] * s_mul[hc + 1] + s_add[hc + 1];
+        fb = d2 < S.crit2 || flag > S.flag_max || (cap > S.cap_lo && cap < S.cap_hi) || !(cap == cap);
+    }
+    const unsigned long long m = __ballot(fb);
+    if (m) {
+        const int leader = __builtin_ctzll(m);
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(S.n_list, unsigned(__popcll(m)));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (fb) S.list[base + lane_rank(m)] = pix;
+    }
+    if (!inside || fb) return;
+
+    const HotParams<float> P = hot_params<float>(A);
+    const float r_lo = gr[0], r_hi = gr[A.nr - 1], ph0 = float(A.cam.ph);
+    auto knot = [&](int j, float &kr, float &kt, float &kp) {
+        const int c = 4 * j;
+        kr = yc[c * ld] * s_mul[c] + s_add[c];
+        kt = clamp_theta(yc[(c + 1) * ld] * s_mul[c + 1] + s_add[c + 1]);
+        kp = yc[(c + 2) * ld] * s_mul[c + 2] + s_add[c + 2] + ph0;
+    };
+    float St[4] = {0.f, 0.f, 0.f, 0.f}, tau = 0.f;
+    float r0, t0, p0;
+    knot(0, r0, t0, p0);
+    int code = -1, seg = 0;
+    for (int j = 1; j < S.K; ++j) {
+        float r1, t1, p1;
+        knot(j, r1, t1, p1);
+        const float rm = 0.5f * (r0 + r1);
+        // outside the brick's radial range and moving away from it: no later segment samples it
+        if ((rm > r_hi && r1 > r0) || (rm < r_lo && r1 < r0)) {
+            code = r1 < r0 ? 1 : 2;
+            break;
+        }
+        float c[kCoeffs];
+        if (sample_brick<float, STORE, LAYOUT>(A, rs, gr, gt, gp, rm, 0.5f * (t0 + t1), 0.5f * (p0 + p1), c)) {
+            const float dx = r1 - r0, dy = t1 - t0, dz = p1 - p0;
+            const float dlam = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+            const float ds = pymax(P.L * pymax(dlam, 1.0e-14f), 0.0f);
+            stokes_rk2(St, c, ds);
+            tau = tau + pymax(c[4], 0.0f) * ds;
+        }
+        ++seg;
+        if (r1 <= P.r_cap) { code = 1; break; }
+        if (tau >= P.tau_max) { code = 4; break; }
+        r0 = r1;
+        t0 = t1;
+        p0 = p1;
+    }
+    if (code < 0) code = cap >= 0.5f ? 1 : 2;
+    float *o = static_cast<float *>(A.iquv) + size_t(pix) * 4;
+    o[0] = nan_to_zero(St[0]);
+    o[1] = nan_to_zero(St[1]);
+    o[2] = nan_to_zero(St[2]);
+    o[3] = nan_to_zero(St[3]);
+    if (A.steps) A.steps[pix] = seg;
+    if (A.status) A.status[pix] = uint8_t(code | 16);
+}
+
+__global__ void fill_u32_once(unsigned *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+}
+
+typedef void (*SurFn)(SurArgs);
+
+static SurFn pick_surrogate(int storage, int layout) {
+    if (layout == BH_LAYOUT_CORNER) {
+        if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_CORNER>;
+        if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_CORNER>;
+        return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_CORNER>;
+    }
+    if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_SOA>;
+    if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_SOA>;
+    return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_SOA>;
+}
+
+static size_t surrogate_work_bytes(int64_t rows, int n_knots) {
+    const size_t n_out = size_t(4 * n_knots + kHeads);
+    const size_t al = 256;
+    auto up = [&](size_t b) { return (b + al - 1) / al * al; };
+    return 2 * up(n_out * 4) + up(2 * kCurveMax * 4) + up(64 * 4) + up(size_t(rows) * 4) + up(size_t(rows) * 16) +
+           up(size_t(rows) * n_out * 4);
+}
+
+
+// Shared body of the surrogate entry points: knot rows come from the MLP (mlp non-null; network
+// input and output live in the work buffer) or from the caller (rows, [n_out][ld]).
+static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, bh_mlp *mlp, const float *rows_in,
+                       int64_t ld_in, const bh_surrogate_desc *sd, float *d_work, size_t work_bytes, void *d_iquv,
+                       int32_t *d_steps, uint8_t *d_status) {
+    int st;
+    if (!scene || !sd || !d_work || !d_iquv || !sd->out_mul || !sd->out_add)
+        return set_error(BH_ERR_VALUE, "NULL surrogate argument");
+    if ((st = validate_scene(*scene))) return st;
+    if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
+        return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
+    const int K = sd->n_knots;
+    if (K < 2 || K > kMaxKnots) return set_error(BH_ERR_VALUE, "n_knots must lie in [2, %d]", kMaxKnots);
+    const int n_out = 4 * K + kHeads;
+    if (mlp) {
+        if (mlp_context(mlp) != ctx) return set_error(BH_ERR_VALUE, "the MLP belongs to another context");
+        if (mlp_in_cols(mlp) != 4 || mlp_out_cols(mlp) != n_out)
+            return set_error(BH_ERR_VALUE, "the MLP must map 4 inputs to 4 * n_knots + %d outputs", kHeads);
+    }
+    if (sd->fallback_precision != BH_PREC_FP32 && sd->fallback_precision != BH_PREC_MIXED)
+        return set_error(BH_ERR_VALUE, "fallback_precision must be BH_PREC_FP32 or BH_PREC_MIXED");
+    const int64_t rows = int64_t(scene->width) * scene->height;
+    const size_t need = surrogate_work_bytes(mlp ? rows : 0, K) + (mlp ? 0 : size_t(rows) * 4 + 256);
+    if (work_bytes < need) return set_error(BH_ERR_VALUE, "work buffer holds %zu bytes, %zu needed", work_bytes, need);
+    const BrickDev &b = ctx->bricks[brick_id];
+    SurArgs S{};
+    if ((st = setup_args(*scene, b, S.R))) return st;
+    S.R.iquv = d_iquv;
+    S.R.steps = d_steps;
+    S.R.status = d_status;
+    // work layout: mul | add | curve | counter | fallback list | network input | network output
+    char *w = reinterpret_cast<char *>(d_work);
+    auto take = [&](size_t bytes) {
+        char *p = w;
+        w += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    float *d_mul = reinterpret_cast<float *>(take(size_t(n_out) * 4));
+    float *d_add = reinterpret_cast<float *>(take(size_t(n_out) * 4));
+    float *d_curve = reinterpret_cast<float *>(take(2 * kCurveMax * 4));
+    unsigned *d_count = reinterpret_cast<unsigned *>(take(64 * 4));
+    int *d_list = reinterpret_cast<int *>(take(size_t(rows) * 4));
+    float *d_x = mlp ? reinterpret_cast<float *>(take(size_t(rows) * 16)) : nullptr;
+    float *d_y = mlp ? reinterpret_cast<float *>(take(size_t(rows) * n_out * 4)) : nullptr;
+    // critical curve of this spin and inclination, thinned to kCurveMax points
+    std::vector<double> al, be;
+    if (std::sin(S.R.cam.th) > 1e-3) critical_curve(scene->spin, S.R.cam.th, al, be);
+    std::vector<float> curve;
+    const size_t stride = al.empty() ? 1 : (al.size() + kCurveMax - 1) / kCurveMax;
+    for (size_t j = 0; j < al.size(); j += stride) {
+        curve.push_back(float(al[j]));
+        curve.push_back(float(be[j]));
+    }
+    S.n_curve = int(curve.size() / 2);
+    BH_HIP(hipMemcpyAsync(d_mul, sd->out_mul, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipMemcpyAsync(d_add, sd->out_add, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (S.n_curve) BH_HIP(hipMemcpyAsync(d_curve, curve.data(), curve.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    S.y = mlp ? d_y : rows_in;
+    S.ld = mlp ? rows : ld_in;
+    S.mul = d_mul;
+    S.add = d_add;
+    S.curve = d_curve;
+    S.K = K;
+    S.crit2 = sd->crit_distance * sd->crit_distance;
+    S.flag_max = sd->flag_max;
+    S.cap_lo = sd->cap_lo;
+    S.cap_hi = sd->cap_hi;
+    S.list = d_list;
+    S.n_list = d_count;
+    ctx->sur_count = d_count;
+
+    begin_timing(ctx);
+    hipLaunchKernelGGL(fill_u32_once, dim3(1), dim3(64), 0, ctx->stream, d_count);
+    if (mlp) {
+        const float4 imul = make_float4(sd->in_mul[0], sd->in_mul[1], sd->in_mul[2], sd->in_mul[3]);
+        const float4 iadd = make_float4(sd->in_add[0], sd->in_add[1], sd->in_add[2], sd->in_add[3]);
+        hipLaunchKernelGGL(surrogate_input_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, ctx->stream, S.R,
+                           imul, iadd, d_x);
+        if ((st = mlp_forward_enqueue(mlp, d_x, rows, d_y, 1))) return st;
+    }
+    const int tiles_y = (S.R.H + 7) / 8;
+    const int n_tiles = S.R.tiles_x * tiles_y;
+    hipLaunchKernelGGL(pick_surrogate(b.storage, b.layout), dim3(unsigned((n_tiles + 3) / 4)), dim3(256), 0,
+                       ctx->stream, S);
+    BH_HIP(hipGetLastError());
+    if ((st = render_enqueue(ctx, scene, brick_id, sd->fallback_precision, d_iquv, d_steps, d_status, d_list, d_count,
+                             int(rows), false)))
+        return st;
+    end_timing(ctx, 1);
+    return BH_OK;
+}
+
+}  // namespace bh
+
+extern "C" {
+
+int bh_geodesic_knots(bh_ctx *ctx, const bh_knot_desc *desc, const float *d_params, int64_t n, float *d_rows,
+                      int32_t *d_steps, float *d_weight) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!desc || !d_params || !d_rows || n < 0) return set_error(BH_ERR_VALUE, "bad knot arguments");
+    const bh_knot_desc &d = *desc;
+    if (d.n_knots < 2 || d.n_knots > kMaxKnots) return set_error(BH_ERR_VALUE, "n_knots must lie in [2, %d]", kMaxKnots);
+    if (!(d.cam_r > 2.0) || !(d.step > 0.0) || d.max_steps < 1 || !(d.r_hi > d.r_lo))
+        return set_error(BH_ERR_VALUE, "bad knot scene (cam_r > 2, step > 0, max_steps >= 1, r_hi > r_lo)");
+    KnotArgs A{};
+    A.params = d_params;
+    A.n = n;
+    A.cam_r = d.cam_r;
+    A.step = d.step;
+    A.r_esc = d.escape_radius;
+    A.r_lo = d.r_lo;
+    A.r_hi = d.r_hi;
+    A.adaptive = d.step_mode;
+    A.max_steps = d.max_steps;
+    A.K = d.n_knots;
+    A.rows = d_rows;
+    A.steps = d_steps;
+    A.weight = d_weight;
+    if (n == 0) return BH_OK;
+    begin_timing(ctx);
+    hipLaunchKernelGGL(knots_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, A);
+    BH_HIP(hipGetLastError());
+    end_timing(ctx, 1);
+    return BH_OK;
+}
+
+int bh_surrogate_work_bytes(int64_t rows, int32_t n_knots, size_t *bytes) {
+    if (!bytes || rows < 0 || n_knots < 2 || n_knots > kMaxKnots) return set_error(BH_ERR_VALUE, "bad arguments");
+    *bytes = surrogate_work_bytes(rows, n_knots);
+    return BH_OK;
+}
+
+int bh_surrogate_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, bh_mlp *mlp,
+                               const bh_surrogate_desc *sd, float *d_work, size_t work_bytes, void *d_iquv,
+                               int32_t *d_steps, uint8_t *d_status) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!mlp) return set_error(BH_ERR_VALUE, "NULL MLP");
+    return knot_render(ctx, scene, brick_id, mlp, nullptr, 0, sd, d_work, work_bytes, d_iquv, d_steps, d_status);
+}
+
+int bh_knot_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, const float *d_rows, int64_t ld,
+                          const bh_surrogate_desc *sd, float *d_work, size_t work_bytes, void *d_iquv, int32_t *d_steps,
+                          uint8_t *d_status) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!d_rows) return set_error(BH_ERR_VALUE, "NULL knot rows");
+    if (scene && ld < int64_t(scene->width) * scene->height) return set_error(BH_ERR_VALUE, "ld must be >= width * height");
+    return knot_render(ctx, scene, brick_id, nullptr, d_rows, ld, sd, d_work, work_bytes, d_iquv, d_steps, d_status);
+}
+
+int bh_surrogate_last_fallback(bh_ctx *ctx, int64_t *n_fallback) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!n_fallback) return set_error(BH_ERR_VALUE, "NULL output");
+    if (!ctx->sur_count) return set_error(BH_ERR_STATE, "no surrogate render on this context");
+    unsigned v = 0;
+    BH_HIP(hipMemcpyAsync(&v, ctx->sur_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    *n_fallback = int64_t(v);
     return BH_OK;
 }
 

@@ -206,6 +206,97 @@ int bh_batch_launch(bh_batch *batch, void *d_iquv, int32_t *d_steps, uint8_t *d_
 int bh_batch_launch_graph(bh_batch *batch, void *d_iquv, int32_t *d_steps, uint8_t *d_status);
 int bh_batch_destroy(bh_batch *batch);
 
+/* ---- learned surrogate: MLP on BF16 MFMA ------------------------------------
+ * No counterpart in the reference (its renderers have no learned part); this is
+ * the surrogate of the Kerr ray tracer.  x (n_in) -> [Linear width, SiLU] x depth
+ * -> Linear n_out.  FP32 master parameters, BF16 copies feed the MFMA GEMMs, FP32
+ * accumulation.  n_in and n_out are padded to multiples of 8 inside; the
+ * parameter vector (get/set, gradients) uses the padded layout: per layer l,
+ * W_l[fi][fo] row-major then b_l[fo], with fi/fo the padded fan-in/fan-out.
+ * Padded rows and columns must be zero.  Device pointers are FP32. */
+typedef struct bh_mlp bh_mlp;
+int bh_mlp_create(bh_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32_t n_out, bh_mlp **out);
+int bh_mlp_destroy(bh_mlp *mlp);
+int bh_mlp_shape(bh_mlp *mlp, int32_t *n_in_pad, int32_t *n_out_pad, int64_t *n_params);
+int bh_mlp_set_params(bh_mlp *mlp, const float *host_params);   /* also resets Adam state */
+int bh_mlp_get_params(bh_mlp *mlp, float *host_params);
+/* Inference: d_x [rows][n_in] -> d_y [rows][n_out], or [n_out][rows] when transposed. */
+int bh_mlp_forward(bh_mlp *mlp, const float *d_x, int64_t rows, float *d_y, int32_t transposed);
+/* Loss = sum_{i,o} w_io (y_io - t_io)^2 / rows with d_t [rows][n_out]; w_io =
+ * d_w[o] * d_row_w[i] for o < masked_cols and d_w[o] otherwise (d_row_w may be
+ * NULL: all ones).  bh_mlp_gradient returns dLoss/dparams (n_params floats, may
+ * be NULL) without an update; bh_mlp_train_step takes one Adam step (beta1 0.9,
+ * beta2 0.999, eps 1e-8). */
+int bh_mlp_gradient(bh_mlp *mlp, const float *d_x, const float *d_t, const float *d_w, const float *d_row_w,
+                    int32_t masked_cols, int32_t rows, float *grad_host, float *loss_host);
+int bh_mlp_train_step(bh_mlp *mlp, const float *d_x, const float *d_t, const float *d_w, const float *d_row_w,
+                      int32_t masked_cols, int32_t rows, float lr, float *loss_host);
+/* d[i][c] = d[i][c] * mul[c] + add[c] for a device [rows][cols] FP32 matrix (host mul/add). */
+int bh_columns_affine(bh_ctx *ctx, float *d, int64_t rows, int32_t cols, const float *mul, const float *add);
+
+/* ---- geodesic knots (training data of the surrogate) -----------------------
+ * For each ray (alpha, beta, a, cos i) — image-plane offsets in M of a camera at
+ * radius cam_r, inclination acos(cos i), phi 0, no roll: the direction is
+ * (-1, -beta / cam_r, alpha / cam_r) in the ZAMO frame, as a pixel whose NDC
+ * gives ndc_x * tan_x = alpha / cam_r — the FP64 RK2 geodesic of bh_render is
+ * traced and the states from the start of the first step whose midpoint radius
+ * lies in [r_lo, r_hi] are recorded as n_knots knots (r, theta, phi, g), where
+ * g = E alpha_lapse / (E - omega L) is the redshift seen by a ZAMO; after the
+ * ray ends (or a pole clamp fires) the last state repeats.  Output rows
+ * (n_knots * 4 + 3 floats): knots, then captured (0/1), flag (1 when the brick
+ * part needs more than n_knots states, a pole clamp fired, the ray failed or hit
+ * max_steps) and the number of recorded states; d_steps (optional) gets the
+ * total RK2 step count and d_weight (optional) 1 for unflagged rays, else 0. */
+typedef struct bh_knot_desc {
+    double cam_r;
+    double step;             /* RK2 step, or factor of (r - r_+) when step_mode = 1 */
+    double escape_radius;
+    double r_lo, r_hi;       /* brick radial range */
+    int32_t step_mode;
+    int32_t max_steps;
+    int32_t n_knots;         /* 2..512 */
+    int32_t reserved;
+} bh_knot_desc;
+int bh_geodesic_knots(bh_ctx *ctx, const bh_knot_desc *desc, const float *d_params, int64_t n,
+                      float *d_rows, int32_t *d_steps, float *d_weight);
+
+/* ---- surrogate render --------------------------------------------------------
+ * One render of `scene` from MLP outputs instead of RK2 geodesics.  The network
+ * input of pixel (px, py) is x = ((alpha, beta, a, cos i) - in_add) * in_mul with
+ * alpha = cam_r * ndc_x * tan_x, beta = cam_r * ndc_y * tan_y; its outputs,
+ * denormalised as y * out_mul + out_add, are rows as written by
+ * bh_geodesic_knots.  The Stokes transfer runs over the predicted knots with the
+ * midpoint / Euclidean-ds rule of the RK2 transfer.  Pixels whose Bardeen image
+ * coordinates lie within crit_distance of the critical curve, whose flag output
+ * exceeds flag_max, or whose captured output lies in (cap_lo, cap_hi) are traced
+ * by the FP32 RK2 kernel instead (bh_render, BH_PREC_FP32 or BH_PREC_MIXED per
+ * fallback_precision).  Status bit 4 marks surrogate pixels; their step count is
+ * the number of knot segments integrated.  d_work (device, bh_surrogate_work_bytes)
+ * holds the network input and output and the fallback list.  The number of
+ * fallback rays of the last call is returned by bh_surrogate_last_fallback
+ * (waits for the call). */
+typedef struct bh_surrogate_desc {
+    float in_mul[4], in_add[4];
+    const float *out_mul, *out_add;   /* host, n_knots * 4 + 3 values */
+    int32_t n_knots;
+    int32_t fallback_precision;
+    float crit_distance;
+    float flag_max, cap_lo, cap_hi;
+} bh_surrogate_desc;
+int bh_surrogate_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, bh_mlp *mlp,
+                               const bh_surrogate_desc *sd, float *d_work, size_t work_bytes,
+                               void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+/* The same render from knot rows the caller supplies ([n_out][ld] FP32, ld >=
+ * width * height, denormalised by out_mul / out_add) instead of the network:
+ * with the rows of bh_geodesic_knots it reproduces the RK2 transfer of every
+ * unflagged ray whose brick part fits in n_knots states. */
+int bh_knot_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, const float *d_rows,
+                          int64_t ld, const bh_surrogate_desc *sd, float *d_work, size_t work_bytes,
+                          void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+int bh_surrogate_last_fallback(bh_ctx *ctx, int64_t *n_fallback);
+/* Bytes of d_work bh_surrogate_render_device needs for `rows` pixels. */
+int bh_surrogate_work_bytes(int64_t rows, int32_t n_knots, size_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

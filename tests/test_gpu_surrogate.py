@@ -1,0 +1,231 @@
+"""GPU parity of the surrogate kernels (BF16 MFMA MLP, Adam, geodesic knots, transfer over knots)."""
+
+import math
+
+import numpy as np
+import pytest
+
+from bhgrrt import scenes
+from bhgrrt import surrogate as bsur
+from bhgrrt.render import PREC_FP32
+from oracle import surrogate as osur
+from oracle import render as orender
+
+pytestmark = pytest.mark.gpu
+
+
+def _params_for(mlp, seed, scale=1.0):
+    p = mlp.init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    out = []
+    for l, (W, b) in enumerate(p):
+        b = b.copy()
+        ro = mlp.n_out if l == len(p) - 1 else b.size
+        b[:ro] = rng.normal(size=ro).astype(np.float32) * 0.1
+        out.append((W * np.float32(scale), b))
+    return out
+
+
+def _upload(renderer, a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    d = renderer.alloc(a.nbytes)
+    d.from_host(a)
+    return d
+
+
+@pytest.mark.parametrize("rows,n_out", [(96, 20), (2048, 515)])
+def test_mlp_forward_matches_bf16_oracle(renderer, rows, n_out):
+    print(f"mlp forward rows={rows} n_out={n_out}", flush=True)
+    mlp = bsur.MLP(renderer, 4, 256, 4, n_out)
+    params = _params_for(mlp, 7)
+    mlp.set_params(params)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1, 1, size=(rows, 4)).astype(np.float32)
+    dx, dy = _upload(renderer, x), renderer.alloc(rows * n_out * 4)
+    mlp.forward(dx, rows, dy)
+    y = dy.to_host(np.empty((rows, n_out), np.float32)).astype(np.float64)
+    ref, _ = osur.mlp_forward([(W.astype(np.float64), b.astype(np.float64)) for W, b in params], x, bf16=True)
+    ref = ref[:, :n_out]
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 2e-3, err
+    # transposed output ([n_out][rows]) holds the same values
+    dyt = renderer.alloc(rows * n_out * 4)
+    mlp.forward(dx, rows, dyt, transposed=True)
+    yt = dyt.to_host(np.empty((n_out, rows), np.float32))
+    assert np.array_equal(yt.T, y.astype(np.float32))
+    # the FP64 network without bf16 rounding agrees to bf16 precision only
+    ref64, _ = osur.mlp_forward([(W.astype(np.float64), b.astype(np.float64)) for W, b in params], x)
+    assert np.abs(y - ref64[:, :n_out]).max() / np.abs(ref64).max() < 5e-2
+    for d in (dx, dy, dyt):
+        d.free()
+    mlp.close()
+
+
+@pytest.mark.parametrize("rows", [64, 8192])
+def test_mlp_gradient_matches_oracle(renderer, rows):
+    print(f"mlp gradient rows={rows}", flush=True)
+    n_out = 515
+    mlp = bsur.MLP(renderer, 4, 256, 4, n_out)
+    params = _params_for(mlp, 3)
+    mlp.set_params(params)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=(rows, 4)).astype(np.float32)
+    t = rng.normal(size=(rows, n_out)).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, size=n_out).astype(np.float32)
+    rw = (rng.uniform(size=rows) > 0.3).astype(np.float32)
+    dx, dt, dw, drw = _upload(renderer, x), _upload(renderer, t), _upload(renderer, w), _upload(renderer, rw)
+    loss, grads = mlp.gradient(dx, dt, dw, rows, drw, 300)
+    p64 = [(W.astype(np.float64), b.astype(np.float64)) for W, b in params]
+    loss_ref, g_ref = osur.mlp_loss_grad(p64, x, t, w, bf16=True, row_w=rw, masked=300)
+    assert loss == pytest.approx(loss_ref, rel=1e-4)
+    for l, ((gW, gb), (rW, rb)) in enumerate(zip(grads, g_ref)):
+        eW = np.linalg.norm(gW - rW) / np.linalg.norm(rW)
+        eb = np.linalg.norm(gb - rb) / np.linalg.norm(rb)
+        assert eW < 2e-3 and eb < 2e-3, (l, eW, eb)
+    # against the FP64 chain rule without rounding: bf16-level agreement, same direction
+    _, g64 = osur.mlp_loss_grad(p64, x, t, w, bf16=False, row_w=rw, masked=300)
+    a, b = osur.flatten(grads), osur.flatten(g64)
+    assert float(a @ b) / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.999
+    # padded entries receive zero gradient
+    assert np.all(grads[0][0][4:] == 0.0) and np.all(grads[-1][0][:, n_out:] == 0.0)
+    for d in (dx, dt, dw, drw):
+        d.free()
+    mlp.close()
+
+
+def test_adam_steps_match_oracle(renderer):
+    print("adam two steps", flush=True)
+    rows, n_out = 512, 64
+    mlp = bsur.MLP(renderer, 4, 256, 4, n_out)
+    params = _params_for(mlp, 9)
+    mlp.set_params(params)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, size=(rows, 4)).astype(np.float32)
+    t = rng.normal(size=(rows, n_out)).astype(np.float32)
+    w = np.ones(n_out, np.float32)
+    dx, dt, dw = _upload(renderer, x), _upload(renderer, t), _upload(renderer, w)
+    p = mlp.flat(params).astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in (1, 2):
+        _, g = mlp.gradient(dx, dt, dw, rows)   # gradient at the current parameters (no update)
+        mlp.train_step(dx, dt, dw, rows, 1e-3)
+        p, m, v = osur.adam(p, mlp.flat(g).astype(np.float64), m, v, step, 1e-3)
+        got = mlp.flat(mlp.get_params()).astype(np.float64)
+        assert np.abs(got - p).max() < 2e-6, (step, np.abs(got - p).max())
+    for d in (dx, dt, dw):
+        d.free()
+    mlp.close()
+
+
+def test_training_reduces_the_loss(renderer):
+    print("short training run", flush=True)
+    rows, n_out = 4096, 32
+    mlp = bsur.MLP(renderer, 4, 256, 4, n_out)
+    mlp.set_params(mlp.init_params(1))
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, size=(rows, 4)).astype(np.float32)
+    # smooth targets: products of low-frequency functions of the inputs
+    t = np.stack([np.sin(0.5 * (k % 4 + 1) * x[:, k % 4]) * np.cos(x[:, (k + 1) % 4]) + 0.1 * k / n_out
+                  for k in range(n_out)], 1).astype(np.float32)
+    dx, dt, dw = _upload(renderer, x), _upload(renderer, t), _upload(renderer, np.ones(n_out, np.float32))
+    first = mlp.train_step(dx, dt, dw, rows, 1e-3, want_loss=True)
+    for _ in range(300):
+        last = mlp.train_step(dx, dt, dw, rows, 1e-3, want_loss=True)
+    assert last < 0.05 * first, (first, last)
+    for d in (dx, dt, dw):
+        d.free()
+    mlp.close()
+
+
+def test_geodesic_knots_match_oracle(renderer):
+    print("geodesic knots vs oracle", flush=True)
+    cfg = bsur.SurrogateConfig(n_knots=64)
+    br = scenes.scale_brick(32, 16, 32)
+    params = bsur.sample_params(48, 77, cfg)
+    dp = _upload(renderer, params)
+    n_out = cfg.n_out
+    drows, dsteps = renderer.alloc(48 * n_out * 4), renderer.alloc(48 * 4)
+    bsur.geodesic_knots(renderer, cfg, br, dp, 48, drows, dsteps)
+    renderer.synchronize()
+    rows = drows.to_host(np.empty((48, n_out), np.float32))
+    steps = dsteps.to_host(np.empty(48, np.int32))
+    ref, ref_steps = osur.knots(params.astype(np.float64), cfg.cam_r, cfg.step, True, cfg.max_steps,
+                                cfg.escape_radius, float(np.float32(br.r[0])), float(np.float32(br.r[-1])), 64)
+    K = 64
+    # heads and step counts agree on nearly all rays (chaotic rays near the photon ring may differ)
+    same = (rows[:, 4 * K] == ref[:, 4 * K]) & (rows[:, 4 * K + 2] == ref[:, 4 * K + 2]) & (steps == ref_steps)
+    assert same.mean() > 0.9, same.mean()
+    kn, kr = rows[same, :4 * K].astype(np.float64), ref[same, :4 * K]
+    assert np.allclose(kn, kr, rtol=2e-6, atol=2e-6 * np.abs(kr).max()), np.abs(kn - kr).max()
+    for d in (dp, drows, dsteps):
+        d.free()
+
+
+def test_knot_render_with_exact_knots_matches_rk2(renderer):
+    """RK2 knots of every pixel (bh_geodesic_knots) rendered by bh_knot_render_device agree with the FP32
+    RK2 render; the fallback rays are traced by the same FP32 kernel and agree bit for bit."""
+    print("knot render from exact knots, 64x64 scale scene", flush=True)
+    sc, br = scenes.scale_plain_scene(64, 64), scenes.scale_brick()
+    cfg = bsur.SurrogateConfig(n_knots=128)
+    n = sc.width * sc.height
+    params = bsur.pixel_params(sc)
+    dp = _upload(renderer, params)
+    drows = renderer.alloc(n * cfg.n_out * 4)
+    bsur.geodesic_knots(renderer, cfg, br, dp, n, drows)
+    renderer.synchronize()
+    rows = drows.to_host(np.empty((n, cfg.n_out), np.float32))
+    dT = _upload(renderer, np.ascontiguousarray(rows.T))
+    db = renderer.upload(br, "f32", "corner")
+    ones, zeros = np.ones(cfg.n_out, np.float32), np.zeros(cfg.n_out, np.float32)
+    cfg0 = bsur.SurrogateConfig(n_knots=128, crit_distance=0.0, cap_lo=0.5, cap_hi=0.5)
+    out = bsur.knot_render(renderer, sc, db, dT, n, cfg0, ones, zeros)
+    ref = renderer.render(sc, db, PREC_FP32)
+    st = out["status"].ravel()
+    sur = (st & 16) != 0
+    flagged = rows[:, 4 * 128 + 1] > 0.5
+    assert np.array_equal(~sur, flagged)
+    assert out["fallback"] == int(flagged.sum())
+    I, I0 = out["iquv"].reshape(-1, 4), ref["iquv"].reshape(-1, 4)
+    # fallback pixels: the same FP32 RK2 kernel on the same rays
+    assert np.array_equal(I[~sur], I0[~sur])
+    assert np.array_equal(out["steps"].ravel()[~sur], ref["steps"].ravel()[~sur])
+    # surrogate pixels: FP32 transfer over FP64 knots stored as float32
+    ok = sur & ((ref["status"].ravel() & 8) == 0)
+    e = np.abs(I[ok, 0] - I0[ok, 0]).sum() / np.abs(I0[ok, 0]).sum()
+    assert e < 5e-4, e
+    assert np.mean((st[ok] & 7) == (ref["status"].ravel()[ok] & 7)) > 0.99
+    renderer.release(db)
+    for d in (dp, drows, dT):
+        d.free()
+
+
+def test_surrogate_render_gains_over_the_untrained_network(renderer):
+    """A short training run on seeded RK2 geodesics makes the surrogate image far closer to the RK2 image
+    than the same pipeline with the untrained network (what the learned part contributes)."""
+    print("surrogate short training + render", flush=True)
+    sc, br = scenes.scale_plain_scene(64, 64), scenes.scale_brick()
+    cfg = bsur.SurrogateConfig(spin=(0.9, 0.98), inclination_deg=(10.0, 25.0))
+    sur = bsur.Surrogate(renderer, br, cfg, seed=1)
+    data = sur.generate(1 << 15, 99)
+    db = renderer.upload(br, "f32", "corner")
+    ref = renderer.render(sc, db, PREC_FP32)
+    out0 = sur.render(sc, db)
+    info = sur.train(data, 400, batch=8192, lr=1e-3, log_every=50)
+    out1 = sur.render(sc, db)
+    losses = [l for _, l in sur.history]
+    assert losses[-1] < 0.5 * losses[0], losses
+    I0 = ref["iquv"][..., 0].astype(np.float64).ravel()
+    ok = (ref["status"].ravel() & 8) == 0
+
+    def err(o):
+        return np.abs(o["iquv"][..., 0].astype(np.float64).ravel()[ok] - I0[ok]).sum() / np.abs(I0[ok]).sum()
+
+    e0, e1 = err(out0), err(out1)
+    print(f"rel L1 on I: untrained {e0:.3e}, trained ({info['steps']} steps) {e1:.3e}, fallback {out1['fallback']}")
+    assert e1 < 0.5 * e0
+    assert out1["fallback"] < sc.width * sc.height
+    renderer.release(db)
+    for k in ("x", "t", "w"):
+        data[k].free()
+    sur.close()
