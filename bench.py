@@ -101,6 +101,96 @@ def time_path(rr, scene, db, precision, bufs, steps: int, warmup: int) -> dict:
     return {"ms_per_step": (t1 - t0) * 1e3 / steps, "kernel_ms": float(np.mean(kms)), "work": work[-1]}
 
 
+def sampler_path(rr, brick, n: int, reps: int) -> dict:
+    """FP64 trilinear sampler (sample_brick_trilinear, device-resident) on n uniform points inside the
+    brick: HBM-bound; algorithmic bytes per point = 24 (coordinates) + 88 (11 FP64 channels out)."""
+    import ctypes as C
+    from bhgrrt import _lib
+    from oracle import micro
+
+    rng = np.random.default_rng(20240607)
+    pts = np.empty((n, 3))
+    pts[:, 0] = rng.uniform(float(brick.r[0]), float(brick.r[-1]), n)
+    pts[:, 1] = rng.uniform(float(brick.theta[0]), float(brick.theta[-1]), n)
+    pts[:, 2] = rng.uniform(0.0, 2.0 * math.pi, n)
+    coeffs = np.ascontiguousarray(brick.coeffs, dtype=np.float64)
+    grids = [np.ascontiguousarray(g, dtype=np.float64) for g in (brick.r, brick.theta, brick.phi)]
+    bufs = [rr.alloc(a.nbytes) for a in [coeffs, *grids, pts]]
+    for b, a in zip(bufs, [coeffs, *grids, pts]):
+        b.from_host(a)
+    d_out = rr.alloc(n * 11 * 8)
+    lib = rr._lib
+
+    def run():
+        _lib.check(lib.bh_sample_brick_trilinear_device(rr.ctx, bufs[0].ptr, bufs[1].ptr, grids[0].size, bufs[2].ptr,
+                                                        grids[1].size, bufs[3].ptr, grids[2].size, bufs[4].ptr,
+                                                        C.c_int64(n), d_out.ptr))
+
+    run()
+    rr.synchronize()
+    kms = []
+    for _ in range(max(1, reps)):
+        run()
+        kms.append(rr.last_kernel_ms())
+    rr.synchronize()
+    out = d_out.to_host(np.empty((n, 11)))
+    pick = rng.choice(n, 4096, replace=False)
+    ref, _ = micro.sample_vec(coeffs, *grids, pts[pick, 0], pts[pick, 1], pts[pick, 2])
+    err = float(np.max(np.abs(out[pick] - ref) / np.maximum(np.abs(ref), 1e-300)))
+    ms = float(np.mean(kms))
+    gbs = n * 112.0 / (ms * 1e-3) / 1e9
+    for b in bufs + [d_out]:
+        b.free()
+    return {"points": n, "kernel_ms": ms, "gpoints_s": n / (ms * 1e-3) / 1e9, "algorithmic_GBs": gbs,
+            "hbm_frac": gbs / PEAK_HBM_GBS, "bytes_per_point": 112, "max_rel_err_vs_oracle": err}
+
+
+def surrogate_path(rr, scene, brick, ref, I64, args) -> dict:
+    """Train the MLP surrogate on seeded RK2 geodesics, then time surrogate renders of the scene
+    (bf16 corner brick, mixed-precision RK2 fallback) and measure their error against FP64."""
+    from bhgrrt import surrogate as bsur
+    from bhgrrt.render import PREC_MIXED
+
+    cfg = bsur.SurrogateConfig(fallback_precision=PREC_MIXED)
+    log(f"[bench] surrogate: {args.surrogate_rays} RK2 geodesics, Adam lr 1e-3, batch 8192, "
+        f"<= {args.surrogate_epochs} epochs / {args.surrogate_seconds} s")
+    t0 = time.perf_counter()
+    sur, info = bsur.train_surrogate(rr, brick, cfg, n_rays=args.surrogate_rays, epochs=args.surrogate_epochs,
+                                     time_limit_s=args.surrogate_seconds)
+    train_s = time.perf_counter() - t0
+    log(f"[bench] surrogate trained: {info['steps']} steps in {info['seconds']:.1f} s, loss {info['loss']:.3f}, "
+        f"flagged {info['flagged']:.3f}")
+    dbx = rr.upload(brick, "bf16", "corner")
+    n = scene.rays
+    bufs = (rr.alloc(n * 16), rr.alloc(n * 4), rr.alloc(n))
+    for _ in range(max(1, args.warmup)):
+        sur.render_device(scene, dbx, *bufs)
+    rr.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sur.render_device(scene, dbx, *bufs)
+        kms.append(rr.last_kernel_ms())
+    rr.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    fallback = sur.last_fallback()
+    iquv = bufs[0].to_host(np.empty((scene.height, scene.width, 4), np.float32))
+    status = bufs[2].to_host(np.empty((scene.height, scene.width), np.uint8))
+    ok = ((ref["status"] | status) & 8) == 0
+    out = {"mrays_s": n / dt / 1e6, "ms_per_render": dt * 1e3, "kernel_ms": float(np.mean(kms)),
+           "rel_l1_I": rel_l1(iquv[..., 0][ok].astype(np.float64), I64[ok]),
+           "max_abs_QUV": float(np.max(np.abs(iquv[..., 1:][ok].astype(np.float64) - ref["iquv"][..., 1:][ok]))),
+           "fallback_frac": fallback / n, "train_steps": info["steps"], "train_s": round(train_s, 2),
+           "train_loss": info["loss"], "train_rays": info["n_rays"], "flagged_train_frac": info["flagged"],
+           "knots": cfg.n_knots, "width": cfg.width, "depth": cfg.depth}
+    log(f"[bench] surrogate_mixed_bf16: {out}")
+    for b in bufs:
+        b.free()
+    rr.release(dbx)
+    sur.close()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +200,11 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--frames", type=int, default=8, help="1024^2 frames per batched launch")
+    ap.add_argument("--surrogate-rays", type=int, default=1 << 20, help="RK2 geodesics the surrogate trains on")
+    ap.add_argument("--surrogate-epochs", type=float, default=16.0)
+    ap.add_argument("--surrogate-seconds", type=float, default=90.0, help="training time limit")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sampler-points", type=int, default=1 << 24)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -133,13 +228,14 @@ def main(argv=None) -> int:
     log("[bench] FP64 reference render")
     ref = rr.render(scene, db32, PREC_FP64)
     I64 = ref["iquv"][..., 0]
+    fp64_ms = rr.last_kernel_ms()
 
     log("[bench] plain FP32 path (corner-packed f32 brick)")
     dbc = rr.upload(brick, "f32", "corner")
     plain = time_path(rr, scene, dbc, PREC_FP32, bufs, args.steps, args.warmup)
     out32 = rr.render(scene, dbc, PREC_FP32)
     # rays flagged (status bit 3) in either render left the geodesic at a pole: excluded, counted
-    ok = ((ref["status"] | out32["status"]) & 8) == 0
+    ok_plain = ok = ((ref["status"] | out32["status"]) & 8) == 0
     err32 = rel_l1(out32["iquv"][..., 0][ok].astype(np.float64), I64[ok])
     quv32 = float(np.max(np.abs(out32["iquv"][..., 1:][ok].astype(np.float64) - ref["iquv"][..., 1:][ok])))
     steps_total, samples_total = plain["work"]
@@ -150,7 +246,8 @@ def main(argv=None) -> int:
         f"{value:.1f} Mrays/s, rel L1 I {err32:.2e}")
 
     paths = {"plain_fp32": {"mrays_s": value, "ms_per_render": plain["ms_per_step"], "kernel_ms": plain["kernel_ms"],
-                            "rel_l1_I": err32, "max_abs_QUV": quv32, "tflops": achieved, "layout": "corner"}}
+                            "rel_l1_I": err32, "max_abs_QUV": quv32, "tflops": achieved, "layout": "corner"},
+             "fp64_reference": {"kernel_ms": fp64_ms, "mrays_s": n / (fp64_ms * 1e-3) / 1e6}}
     log("[bench] plain FP32 path (SoA f32 brick)")
     soa = time_path(rr, scene, db32, PREC_FP32, bufs, args.steps, args.warmup)
     paths["plain_fp32_soa"] = {"mrays_s": n / (soa["ms_per_step"] * 1e-3) / 1e6, "ms_per_render": soa["ms_per_step"],
@@ -242,6 +339,31 @@ def main(argv=None) -> int:
     for b in bricks:
         rr.release(b)
 
+    # ---- sampler at large N ----------------------------------------------------------------
+    log(f"[bench] FP64 sampler, {args.sampler_points} points")
+    paths["sampler_fp64"] = sampler_path(rr, brick, args.sampler_points, args.steps)
+    log(f"[bench] sampler: {paths['sampler_fp64']}")
+
+    # ---- learned surrogate + mixed precision --------------------------------------------
+    if not args.no_surrogate:
+        paths["surrogate_mixed_bf16"] = surrogate_path(rr, scene, brick, ref, I64, args)
+
+    # headline of the layered method: surrogate + mixed (+ batched) vs the plain FP32 render
+    target_err = 1e-3
+    sur = paths.get("surrogate_mixed_bf16")
+    compliant = {k: v for k, v in paths.items() if v.get("rel_l1_I") is not None and v["rel_l1_I"] <= target_err
+                 and "mrays_s" in v}
+    best_name = max(compliant, key=lambda k: compliant[k]["mrays_s"]) if compliant else None
+    headline = {
+        "target": {"ratio": 10.0, "rel_l1_I": target_err},
+        "surrogate_ratio": (sur["mrays_s"] / value) if sur else None,
+        "surrogate_rel_l1_I": sur["rel_l1_I"] if sur else None,
+        "meets_target": bool(sur and sur["mrays_s"] / value >= 10.0 and sur["rel_l1_I"] <= target_err),
+        "best_path_within_error": best_name,
+        "best_ratio_within_error": (compliant[best_name]["mrays_s"] / value) if best_name else None,
+    }
+    log(f"[bench] headline: {headline}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (FP64 NumPy oracle)")
@@ -271,7 +393,8 @@ def main(argv=None) -> int:
                               "F_sample": F_SAMPLE, "kernel_ms": plain["kernel_ms"]}},
         "cpu_baseline": cpu,
         "error": {"plain_fp32": {"rel_l1_I": err32, "max_abs_QUV": quv32,
-                                 "pole_flagged_frac": float(1 - ok.mean())}},
+                                 "pole_flagged_frac": float(1 - ok_plain.mean())}},
+        "headline": headline,
         "paths": paths,
         "host": platform.node(),
     }

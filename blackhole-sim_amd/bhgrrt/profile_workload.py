@@ -1,18 +1,24 @@
-"""Fixed workload for rocprofv3 runs: renders of the 1024^2 Kerr Stokes scene plus a calibration copy.
+"""Fixed workload for rocprofv3 runs: the 1024^2 Kerr Stokes scene on every path, plus a calibration copy.
 
-    rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \
-        python3 -m bhgrrt.profile_workload --paths f32:corner:fp32,bf16:corner:mixed
-    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- python3 -m bhgrrt.profile_workload
+    rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \\
+        python3 blackhole-sim_amd/bhgrrt/profile_workload.py
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
+        python3 blackhole-sim_amd/bhgrrt/profile_workload.py
 
-Each path is storage:layout:precision; every path is rendered twice (the second
-launch is the measured one).  A 256 MiB device-to-device copy follows, so the
-FETCH_SIZE / WRITE_SIZE counters can be scaled by what they read for a copy of
-known size.
+Render paths are storage:layout:precision; each is rendered twice (the second launch is the
+measured one).  ``--surrogate`` adds the learned surrogate: FP64 geodesic knots for
+``--surrogate-rays`` seeded rays, ``--train-steps`` Adam steps on the BF16 MFMA kernels and two
+1024^2 surrogate renders (bf16 corner brick, mixed-precision fallback).  ``--sampler N`` adds two
+launches of the FP64 trilinear sampler on N uniform points.  A 256 MiB device-to-device copy comes
+last, so the FETCH_SIZE / WRITE_SIZE counters can be scaled by what they read for a copy of known
+size.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes as C
+import math
 import sys
 from pathlib import Path
 
@@ -21,6 +27,8 @@ for _p in (_ROOT, _ROOT / "blackhole-sim_amd"):
     if str(_p) not in sys.path:
         sys.path.insert(0, str(_p))
 
+import numpy as np  # noqa: E402
+
 from bhgrrt import _lib, scenes  # noqa: E402
 from bhgrrt.render import Renderer  # noqa: E402
 
@@ -28,16 +36,67 @@ PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
 COPY_BYTES = 256 << 20
 
 
+def run_sampler(r: Renderer, brick, n: int) -> None:
+    rng = np.random.default_rng(20240607)
+    pts = np.stack([rng.uniform(float(brick.r[0]), float(brick.r[-1]), n),
+                    rng.uniform(float(brick.theta[0]), float(brick.theta[-1]), n),
+                    rng.uniform(0.0, 2.0 * math.pi, n)], axis=1)
+    arrs = [np.ascontiguousarray(brick.coeffs, np.float64)] + [np.ascontiguousarray(g, np.float64)
+                                                                for g in (brick.r, brick.theta, brick.phi)] + [pts]
+    bufs = [r.alloc(a.nbytes) for a in arrs]
+    for b, a in zip(bufs, arrs):
+        b.from_host(a)
+    out = r.alloc(n * 88)
+    for _ in range(2):
+        _lib.check(r._lib.bh_sample_brick_trilinear_device(r.ctx, bufs[0].ptr, bufs[1].ptr, brick.r.size, bufs[2].ptr,
+                                                           brick.theta.size, bufs[3].ptr, brick.phi.size, bufs[4].ptr,
+                                                           C.c_int64(n), out.ptr))
+        r.synchronize()
+    print(f"sampler: {n} points, {r.last_kernel_ms():.3f} ms", flush=True)
+    for b in bufs + [out]:
+        b.free()
+
+
+def run_surrogate(r: Renderer, sc, brick, rays: int, steps: int) -> None:
+    from bhgrrt import surrogate as bsur
+
+    cfg = bsur.SurrogateConfig(fallback_precision=PREC["mixed"])
+    sur = bsur.Surrogate(r, brick, cfg, seed=1)
+    data = sur.generate(rays, 20240605)
+    print(f"knots: {rays} rays, {data['ms']:.3f} ms", flush=True)
+    info = sur.train(data, steps, log_every=max(steps, 1))
+    print(f"training: {info['steps']} Adam steps, loss {info['loss']:.3f}", flush=True)
+    for k in ("x", "t", "w"):
+        data[k].free()
+    db = r.upload(brick, "bf16", "corner")
+    n = sc.rays
+    bufs = (r.alloc(n * 16), r.alloc(n * 4), r.alloc(n))
+    for _ in range(2):
+        sur.render_device(sc, db, *bufs)
+        r.synchronize()
+    print(f"surrogate render: {r.last_kernel_ms():.3f} ms, fallback {sur.last_fallback()} rays", flush=True)
+    for b in bufs:
+        b.free()
+    r.release(db)
+    sur.close()
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--paths", default="f32:corner:fp32,bf16:corner:mixed,f16:corner:mixed,f32:soa:fp32")
     ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--surrogate", action="store_true")
+    ap.add_argument("--surrogate-rays", type=int, default=1 << 18)
+    ap.add_argument("--train-steps", type=int, default=64)
+    ap.add_argument("--sampler", type=int, default=0, help="points of the FP64 sampler launch (0: none)")
+    ap.add_argument("--latency-sizes", default="", help="e.g. 16,64,256,1024: plain FP32 renders of the scene at "
+                    "these sizes, printing kernel time, longest ray and time per step of the longest ray")
     args = ap.parse_args(argv)
     sc, br = scenes.scale_plain_scene(args.size, args.size), scenes.scale_brick()
     r = Renderer(0)
     n = sc.rays
     bufs = (r.alloc(n * 32), r.alloc(n * 4), r.alloc(n))
-    for spec in args.paths.split(","):
+    for spec in [p for p in args.paths.split(",") if p]:
         storage, layout, prec = spec.split(":")
         db = r.upload(br, storage, layout)
         for _ in range(2):
@@ -46,6 +105,26 @@ def main(argv=None) -> int:
         steps, samples = r.last_work()
         print(f"{spec}: {r.last_kernel_ms():.3f} ms, {steps} RK2 steps, {samples} brick samples", flush=True)
         r.release(db)
+    for b in bufs:
+        b.free()
+    if args.latency_sizes:
+        db = r.upload(br, "f32", "corner")
+        for size in [int(v) for v in args.latency_sizes.split(",")]:
+            scn = scenes.scale_plain_scene(size, size)
+            ms = []
+            for _ in range(3):
+                out = r.render(scn, db, PREC["fp32"])
+                ms.append(r.last_kernel_ms())
+            t = float(np.median(ms))
+            longest = int(out["steps"].max())
+            print(f"latency {size}x{size}: kernel {t:.3f} ms, longest ray {longest} steps, "
+                  f"{1e3 * t / longest:.2f} us per step of the longest ray, {int(out['steps'].sum())} steps in all",
+                  flush=True)
+        r.release(db)
+    if args.sampler:
+        run_sampler(r, br, args.sampler)
+    if args.surrogate:
+        run_surrogate(r, sc, br, args.surrogate_rays, args.train_steps)
     a, b = r.alloc(COPY_BYTES), r.alloc(COPY_BYTES)
     _lib.check(_lib.lib().bh_copy_device(r.ctx, b.ptr, a.ptr, COPY_BYTES))
     print(f"calibration copy: {COPY_BYTES} bytes", flush=True)

@@ -152,11 +152,13 @@ __global__ void transpose_bf16_kernel(const float *w, int K, int N, __bf16 *wt) 
     wt[size_t(n) * K + k] = __bf16(w[i]);
 }
 
-// Column sums of an [M][N] FP32 matrix into out (zeroed beforehand), 256 rows per block row.
+// Column sums of an [M][N] FP32 matrix into out (zeroed beforehand): block (x, y) sums columns
+// x*256 .. +255 over rows y*kColRows .. +kColRows-1, one atomic per column per block.
+constexpr int kColRows = 32;
 __global__ void colsum_kernel(const float *x, int M, int N, float *out) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
-    const int r0 = blockIdx.y * 256, r1 = min(M, r0 + 256);
+    const int r0 = blockIdx.y * kColRows, r1 = min(M, r0 + kColRows);
     float s = 0.f;
     for (int m = r0; m < r1; ++m) s += x[size_t(m) * N + n];
     atomicAdd(&out[n], s);
@@ -164,26 +166,35 @@ __global__ void colsum_kernel(const float *x, int M, int N, float *out) {
 
 // Weighted squared error: L = sum_{i,o} w_io (y_io - t_io)^2 / rows,  dY = 2 w_io (y - t) / rows, with
 // w_io = w_o * rw_i for o < masked (rw null: 1) and w_o otherwise.  y is [rows][n_pad]; t is [rows][n];
-// padded columns get zero gradient.
-__global__ void mse_grad_kernel(const float *y, const float *t, const float *w, const float *rw, int masked, int rows,
-                                int n, int n_pad, float *dy, __bf16 *dyb, float *loss) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// padded columns get zero gradient.  Block (x, r) covers columns x*256 .. of rows r*kMseRows ..; the
+// loss is reduced in the block and added with one atomic per block.
+constexpr int kMseRows = 16;
+__global__ __launch_bounds__(256) void mse_grad_kernel(const float *y, const float *t, const float *w, const float *rw,
+                                                       int masked, int rows, int n, int n_pad, float *dy, __bf16 *dyb,
+                                                       float *loss) {
+    __shared__ float part[4];
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.f;
-    if (i < size_t(rows) * n_pad) {
-        const int o = int(i % n_pad);
-        const size_t r = i / n_pad;
-        float gr = 0.f;
-        if (o < n) {
-            const float wo = (rw && o < masked) ? w[o] * rw[r] : w[o];
-            const float d = y[i] - t[r * n + o];
-            gr = wo == 0.f ? 0.f : 2.0f * wo * d / float(rows);
-            l = wo == 0.f ? 0.f : wo * d * d / float(rows);
+    if (o < n_pad) {
+        const float wo_col = o < n ? w[o] : 0.f;
+        const int r0 = blockIdx.y * kMseRows, r1 = min(rows, r0 + kMseRows);
+        for (int r = r0; r < r1; ++r) {
+            const size_t i = size_t(r) * n_pad + o;
+            float gr = 0.f;
+            if (o < n) {
+                const float wo = (rw && o < masked) ? wo_col * rw[r] : wo_col;
+                const float d = y[i] - t[size_t(r) * n + o];
+                gr = wo == 0.f ? 0.f : 2.0f * wo * d / float(rows);
+                l += wo == 0.f ? 0.f : wo * d * d / float(rows);
+            }
+            dy[i] = gr;
+            dyb[i] = __bf16(gr);
         }
-        dy[i] = gr;
-        dyb[i] = __bf16(gr);
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
 }
 
 __global__ void adam_kernel(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2,
@@ -351,12 +362,12 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
     bh_ctx *ctx = m->ctx;
     BH_HIP(hipMemsetAsync(m->grad, 0, m->n_params * 4, ctx->stream));
     BH_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), ctx->stream));
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(size_t(rows) * m->n_out_pad)), dim3(256), 0, ctx->stream, m->y,
-                       d_t, d_w, d_rw, masked, rows, m->n_out, m->n_out_pad, m->dy, m->dyb, m->loss);
+    hipLaunchKernelGGL(mse_grad_kernel, dim3((m->n_out_pad + 255) / 256, (rows + kMseRows - 1) / kMseRows), dim3(256), 0,
+                       ctx->stream, m->y, d_t, d_w, d_rw, masked, rows, m->n_out, m->n_out_pad, m->dy, m->dyb, m->loss);
     int st;
     const __bf16 *dcur = m->dyb;   // gradient w.r.t. this layer's pre-activation, BF16
     const float *dcurf = m->dy;    // the same in FP32 (bias gradient)
-    const int splits = rows >= 2048 ? rows / 1024 : 1;
+    const int splits = rows >= 1024 ? rows / 256 : 1;
     for (int l = m->depth; l >= 0; --l) {
         const int K = m->fi[l], N = m->fo[l];
         const __bf16 *xin = l == 0 ? m->xb : m->act[l - 1];
@@ -372,8 +383,8 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         gw.Cf = m->grad + m->w_off[l];
         gw.ldc = N;
         if ((st = run_gemm<false, false, EPI_ATOMIC>(ctx, gw, splits))) return st;
-        hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256, (rows + 255) / 256), dim3(256), 0, ctx->stream, dcurf,
-                           rows, N, m->grad + m->b_off[l]);
+        hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256, (rows + kColRows - 1) / kColRows), dim3(256), 0,
+                           ctx->stream, dcurf, rows, N, m->grad + m->b_off[l]);
         if (l == 0) break;
         // dZ_prev = (dZ W^T) * silu'(z_prev)
         __bf16 *dnext = (dcur == m->dab) ? m->dab2 : m->dab;
