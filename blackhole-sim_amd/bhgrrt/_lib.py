@@ -110,6 +110,7 @@ SIGNATURES = [
     ("bh_mlp_set_params", C.c_int, [C.c_void_p, _fp]),
     ("bh_mlp_get_params", C.c_int, [C.c_void_p, _fp]),
     ("bh_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
+    ("bh_mlp_set_fused", C.c_int, [C.c_void_p, C.c_int32]),
     ("bh_mlp_gradient", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, _fp, _fp]),
     ("bh_mlp_train_step", C.c_int,

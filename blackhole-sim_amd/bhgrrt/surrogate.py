@@ -20,7 +20,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import time
-from dataclasses import asdict, dataclass, field
+from dataclasses import asdict, dataclass
 
 import numpy as np
 
@@ -29,10 +29,6 @@ from .render import PREC_FP32, DeviceBrick, Renderer, _ptr, scene_desc
 from .scenes import Brick, Scene
 
 HEADS = 3
-
-
-def _pad8(n: int) -> int:
-    return (int(n) + 7) // 8 * 8
 
 
 def _off(buf, nbytes: int):
@@ -103,6 +99,10 @@ class MLP:
         v = np.empty(self.n_params, dtype=np.float32)
         _lib.check(self._lib.bh_mlp_get_params(self.h, _lib.fptr(v)))
         return self.unflat(v)
+
+    def set_fused(self, fused: bool) -> None:
+        """Transposed inference through the fused all-layer kernel (default) or the per-layer GEMMs."""
+        _lib.check(self._lib.bh_mlp_set_fused(self.h, int(bool(fused))))
 
     def forward(self, d_x, rows: int, d_y, transposed: bool = False) -> None:
         _lib.check(self._lib.bh_mlp_forward(self.h, _ptr(d_x), int(rows), _ptr(d_y), int(bool(transposed))))

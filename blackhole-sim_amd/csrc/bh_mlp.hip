@@ -42,9 +42,10 @@ struct GemmArgs {
     int k_split;       // K range of one grid.z slice (multiple of kKStep)
 };
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (1 ulp) instead of a full-precision divide.
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -211,6 +212,142 @@ __global__ void adam_kernel(float *w, const float *g, float *m, float *v, size_t
     wb[i] = __bf16(wi);
 }
 
+// ---- fused inference: all layers of a 128-row tile in one workgroup ------------------------------
+// Width-256 networks.  Activations stay in LDS as BF16 [128][256] (each wave owns 32 rows, so a wave
+// overwrites only rows it alone reads); each layer's W^T is staged in LDS 32 k-columns at a time
+// (double buffered, one barrier per slice), and each wave computes its 32 x 256 block with 2 x 16
+// MFMA 16x16x32 tiles accumulating in FP32.  The first layer (n_in <= 8 inputs) runs on the VALU with
+// the same BF16 rounding of inputs and weights as the MFMA path.  The output layer (up to 1024
+// columns) runs in passes of 256 columns and writes FP32 transposed ([n_out][ld]).
+constexpr int kFRows = 128;             // rows per workgroup
+constexpr int kFWidth = 256;            // hidden width
+constexpr int kFActLd = kFWidth + 8;    // LDS row stride (bf16) of the activations
+constexpr int kFBk = 32;                // k per staged weight slice
+constexpr int kFBLd = kFBk + 8;         // LDS row stride (bf16) of a staged slice
+constexpr int kFMaxLayers = 10;
+
+struct FusedArgs {
+    const float *x;          // [rows][n_in]
+    int rows, n_in, depth, n_out, n_out_pad;
+    const float *w32;        // FP32 parameters (layer-0 weights, biases)
+    const __bf16 *wtb;       // BF16 W^T per layer
+    long long w_off[kFMaxLayers], b_off[kFMaxLayers];
+    float *y;                // [n_out][ld], rows r0 .. of this launch at column offset 0
+    long long ld;
+};
+
+__global__ __launch_bounds__(256) void mlp_fused_kernel(FusedArgs F) {
+    __shared__ __align__(16) __bf16 act[kFRows][kFActLd];
+    __shared__ __align__(16) __bf16 bs[2][kFWidth][kFBLd];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r0 = blockIdx.x * kFRows;
+    const int rl = lane & 15, kq = (lane >> 4) * 8;
+
+    // layer 0 on the VALU: x staged as BF16-rounded floats in the (not yet used) slice buffer
+    float *xs = reinterpret_cast<float *>(&bs[0][0][0]);   // kFRows x 8 floats
+    for (int i = tid; i < kFRows * 8; i += 256) {
+        const int r = i >> 3, k = i & 7;
+        const float v = (r0 + r < F.rows && k < F.n_in) ? F.x[size_t(r0 + r) * F.n_in + k] : 0.f;
+        xs[i] = float(__bf16(v));
+    }
+    __syncthreads();
+    {
+        const float *W0 = F.w32 + F.w_off[0];   // [8][256]
+        const float *b0 = F.w32 + F.b_off[0];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const int c = lane + 64 * cc;
+            float w[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w[k] = k < F.n_in ? float(__bf16(W0[k * kFWidth + c])) : 0.f;
+            const float bias = b0[c];
+            for (int r = wave * 32; r < wave * 32 + 32; ++r) {
+                float v = 0.f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v += xs[r * 8 + k] * w[k];
+                act[r][c] = __bf16(silu(v + bias));
+            }
+        }
+    }
+    __syncthreads();   // xs (in bs[0]) is dead from here on
+
+    for (int l = 1; l <= F.depth; ++l) {
+        const bool last = l == F.depth;
+        const int n_cols = last ? F.n_out_pad : kFWidth;
+        const __bf16 *Wt = F.wtb + F.w_off[l];   // [n_cols][256]
+        const float *bias = F.w32 + F.b_off[l];
+        for (int n0 = 0; n0 < n_cols; n0 += kFWidth) {
+            f32x4 acc[2][16];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            // thread t stages row n0 + t of the slice: 32 k-values = 4 x 16 bytes
+            const bool row_ok = n0 + tid < n_cols;
+            const uint4 *src = reinterpret_cast<const uint4 *>(Wt + size_t(n0 + tid) * kFWidth);
+            uint4 pre[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pre[q] = row_ok ? src[q] : uint4{0u, 0u, 0u, 0u};
+            for (int kt = 0; kt < kFWidth / kFBk; ++kt) {
+                const int buf = kt & 1;
+                uint4 *dst = reinterpret_cast<uint4 *>(&bs[buf][tid][0]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) dst[q] = pre[q];
+                __syncthreads();
+                if (kt + 1 < kFWidth / kFBk) {   // next slice in flight during this slice's MFMAs
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) pre[q] = row_ok ? src[(kt + 1) * 4 + q] : uint4{0u, 0u, 0u, 0u};
+                }
+                bf16x8 a[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    a[i] = *reinterpret_cast<const bf16x8 *>(&act[wave * 32 + i * 16 + rl][kt * kFBk + kq]);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    if (n0 + j * 16 >= n_cols) break;   // wave-uniform: the last output pass is partial
+                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&bs[buf][j * 16 + rl][kq]);
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b, acc[i][j], 0, 0, 0);
+                }
+            }
+            // epilogue.  Lane l holds rows (l/16)*4 + q, column l%16 of each 16 x 16 tile.
+            if (!last) {
+                // every wave has read its last A operand of this layer (program order within the wave)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int c = j * 16 + rl;
+                        const float bv = bias[c];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            act[wave * 32 + i * 16 + (lane >> 4) * 4 + q][c] = __bf16(silu(acc[i][j][q] + bv));
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int c = n0 + j * 16 + rl;
+                        if (c >= F.n_out) continue;
+                        const float bv = bias[c];
+                        const int row = r0 + wave * 32 + i * 16 + (lane >> 4) * 4;
+                        float *o = F.y + size_t(c) * F.ld + row;
+                        if (row + 3 < F.rows && (F.ld & 3) == 0) {
+                            *reinterpret_cast<float4 *>(o) =
+                                make_float4(acc[i][j][0] + bv, acc[i][j][1] + bv, acc[i][j][2] + bv, acc[i][j][3] + bv);
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+                                if (row + q < F.rows) o[q] = acc[i][j][q] + bv;
+                        }
+                    }
+            }
+            __syncthreads();   // slice buffers and (for the next layer) activations are reused
+        }
+    }
+}
+
 }  // namespace bh
 
 using namespace bh;
@@ -235,6 +372,7 @@ struct bh_mlp {
     __bf16 *dyb = nullptr, *dab = nullptr, *dab2 = nullptr;
     float *dz = nullptr;
     float *loss = nullptr;
+    bool fused = true;                         // inference through mlp_fused_kernel when the shape allows
 };
 
 namespace bh {
@@ -420,7 +558,35 @@ int mlp_in_cols(const bh_mlp *m) { return m->n_in; }
 bh_ctx *mlp_context(const bh_mlp *m) { return m->ctx; }
 int mlp_out_cols(const bh_mlp *m) { return m->n_out; }
 
+static bool fused_ok(const bh_mlp *m) {
+    return m->fused && m->width == kFWidth && m->n_in_pad == 8 && m->depth + 1 <= kFMaxLayers &&
+           m->n_out_pad <= 4 * kFWidth;
+}
+
+// Fused inference of `rows` rows into y [n_out][ld] (transposed).
+static int forward_fused(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, long long ld) {
+    FusedArgs F{};
+    F.x = d_x;
+    F.rows = int(rows);
+    F.n_in = m->n_in;
+    F.depth = m->depth;
+    F.n_out = m->n_out;
+    F.n_out_pad = m->n_out_pad;
+    F.w32 = m->w32;
+    F.wtb = m->wtb;
+    for (int l = 0; l <= m->depth; ++l) {
+        F.w_off[l] = (long long)m->w_off[l];
+        F.b_off[l] = (long long)m->b_off[l];
+    }
+    F.y = d_y;
+    F.ld = ld;
+    hipLaunchKernelGGL(mlp_fused_kernel, dim3(unsigned((rows + kFRows - 1) / kFRows)), dim3(256), 0, m->ctx->stream, F);
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
 int mlp_forward_enqueue(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int transposed) {
+    if (transposed && fused_ok(m)) return rows > 0 ? forward_fused(m, d_x, rows, d_y, rows) : BH_OK;
     const int chunk = 1 << 18;
     int st;
     if (rows > 0 && (st = ensure_cap(m, int(rows < chunk ? rows : chunk), false))) return st;
@@ -541,6 +707,12 @@ int bh_mlp_forward(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int32_
     begin_timing(m->ctx);
     if ((st = mlp_forward_enqueue(m, d_x, rows, d_y, transposed))) return st;
     end_timing(m->ctx, rows > 0 ? 1 : 0);
+    return BH_OK;
+}
+
+int bh_mlp_set_fused(bh_mlp *m, int32_t fused) {
+    if (!m) return set_error(BH_ERR_VALUE, "NULL MLP handle");
+    m->fused = fused != 0;
     return BH_OK;
 }
 

@@ -220,8 +220,11 @@ int bh_mlp_destroy(bh_mlp *mlp);
 int bh_mlp_shape(bh_mlp *mlp, int32_t *n_in_pad, int32_t *n_out_pad, int64_t *n_params);
 int bh_mlp_set_params(bh_mlp *mlp, const float *host_params);   /* also resets Adam state */
 int bh_mlp_get_params(bh_mlp *mlp, float *host_params);
-/* Inference: d_x [rows][n_in] -> d_y [rows][n_out], or [n_out][rows] when transposed. */
+/* Inference: d_x [rows][n_in] -> d_y [rows][n_out], or [n_out][rows] when transposed.
+ * Transposed inference of width-256 networks (n_in <= 8, n_out <= 1024) runs one fused kernel
+ * (activations kept in LDS) unless bh_mlp_set_fused(mlp, 0) selects the per-layer GEMMs. */
 int bh_mlp_forward(bh_mlp *mlp, const float *d_x, int64_t rows, float *d_y, int32_t transposed);
+int bh_mlp_set_fused(bh_mlp *mlp, int32_t fused);
 /* Loss = sum_{i,o} w_io (y_io - t_io)^2 / rows with d_t [rows][n_out]; w_io =
  * d_w[o] * d_row_w[i] for o < masked_cols and d_w[o] otherwise (d_row_w may be
  * NULL: all ones).  bh_mlp_gradient returns dLoss/dparams (n_params floats, may

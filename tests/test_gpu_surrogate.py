@@ -1,6 +1,5 @@
 """GPU parity of the surrogate kernels (BF16 MFMA MLP, Adam, geodesic knots, transfer over knots)."""
 
-import math
 
 import numpy as np
 import pytest
@@ -9,7 +8,6 @@ from bhgrrt import scenes
 from bhgrrt import surrogate as bsur
 from bhgrrt.render import PREC_FP32
 from oracle import surrogate as osur
-from oracle import render as orender
 
 pytestmark = pytest.mark.gpu
 
@@ -46,8 +44,10 @@ def test_mlp_forward_matches_bf16_oracle(renderer, rows, n_out):
     y = dy.to_host(np.empty((rows, n_out), np.float32)).astype(np.float64)
     ref, _ = osur.mlp_forward([(W.astype(np.float64), b.astype(np.float64)) for W, b in params], x, bf16=True)
     ref = ref[:, :n_out]
-    err = np.abs(y - ref).max() / np.abs(ref).max()
-    assert err < 2e-3, err
+    # BF16 hidden activations: an FP32 difference can flip one activation's rounding (2^-8 relative),
+    # so the bound is on the mean error and loose on the worst value
+    assert np.abs(y - ref).sum() / np.abs(ref).sum() < 1e-3
+    assert np.abs(y - ref).max() / np.abs(ref).max() < 1e-2
     # transposed output ([n_out][rows]) holds the same values
     dyt = renderer.alloc(rows * n_out * 4)
     mlp.forward(dx, rows, dyt, transposed=True)
@@ -229,3 +229,36 @@ def test_surrogate_render_gains_over_the_untrained_network(renderer):
     for k in ("x", "t", "w"):
         data[k].free()
     sur.close()
+
+
+@pytest.mark.parametrize("rows", [1, 131, 1000, 4096 + 5])
+def test_fused_inference_matches_layer_gemms_and_oracle(renderer, rows):
+    """The fused all-layer inference kernel (activations in LDS) against the per-layer GEMM path and
+    the BF16-emulating FP64 oracle, for ragged row counts and the surrogate's 515 outputs."""
+    print(f"fused inference rows={rows}", flush=True)
+    n_out = 515
+    mlp = bsur.MLP(renderer, 4, 256, 4, n_out)
+    params = _params_for(mlp, 21)
+    mlp.set_params(params)
+    rng = np.random.default_rng(rows)
+    x = rng.uniform(-1, 1, size=(rows, 4)).astype(np.float32)
+    dx = _upload(renderer, x)
+    dy_f, dy_g = renderer.alloc(rows * n_out * 4), renderer.alloc(rows * n_out * 4)
+    mlp.set_fused(True)
+    mlp.forward(dx, rows, dy_f, transposed=True)
+    mlp.set_fused(False)
+    mlp.forward(dx, rows, dy_g, transposed=True)
+    yf = dy_f.to_host(np.empty((n_out, rows), np.float32)).astype(np.float64)
+    yg = dy_g.to_host(np.empty((n_out, rows), np.float32)).astype(np.float64)
+    ref, _ = osur.mlp_forward([(W.astype(np.float64), b.astype(np.float64)) for W, b in params], x, bf16=True)
+    ref = ref[:, :n_out].T
+    scale = np.abs(ref).max()
+    # BF16 hidden activations: an accumulation-order difference can flip one activation's rounding
+    # (one BF16 step, 2^-8 relative), so the bound is on the mean error and loose on the worst value
+    for y in (yf, yg):
+        assert np.abs(y - ref).sum() / np.abs(ref).sum() < 1e-3
+        assert np.abs(y - ref).max() / scale < 1e-2
+    assert np.abs(yf - yg).sum() / np.abs(yg).sum() < 1e-3
+    for d in (dx, dy_f, dy_g):
+        d.free()
+    mlp.close()
