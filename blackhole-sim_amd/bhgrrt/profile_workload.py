@@ -9,7 +9,8 @@ Render paths are storage:layout:precision; each is rendered twice (the second la
 measured one).  ``--surrogate`` adds the learned surrogate: FP64 geodesic knots for
 ``--surrogate-rays`` seeded rays, ``--train-steps`` Adam steps on the BF16 MFMA kernels and two
 1024^2 surrogate renders (bf16 corner brick, mixed-precision fallback).  ``--sampler N`` adds two
-launches of the FP64 trilinear sampler on N uniform points.  A 256 MiB device-to-device copy comes
+launches of the FP64 trilinear sampler on N uniform points; ``--mlp-rows N`` times the surrogate
+network's inference on N rows (fused kernel, then per-layer GEMMs).  A 256 MiB device-to-device copy comes
 last, so the FETCH_SIZE / WRITE_SIZE counters can be scaled by what they read for a copy of known
 size.
 """
@@ -57,6 +58,28 @@ def run_sampler(r: Renderer, brick, n: int) -> None:
         b.free()
 
 
+def run_mlp(r: Renderer, rows: int) -> None:
+    from bhgrrt import surrogate as bsur
+
+    mlp = bsur.MLP(r, 4, 256, 4, 515)
+    mlp.set_params(mlp.init_params(3))
+    x, y = r.alloc(rows * 16), r.alloc(rows * 515 * 4)
+    x.from_host(np.random.default_rng(1).uniform(-1, 1, (rows, 4)).astype(np.float32))
+    for fused in (True, False):
+        mlp.set_fused(fused)
+        ms = []
+        for _ in range(4):
+            mlp.forward(x, rows, y, transposed=True)
+            r.synchronize()
+            ms.append(r.last_kernel_ms())
+        t = float(np.median(ms[1:]))
+        print(f"mlp inference {'fused' if fused else 'per-layer'}: {rows} rows, {t:.3f} ms, "
+              f"{2 * 3.31e5 * rows / t / 1e9:.1f} TFLOP/s", flush=True)
+    x.free()
+    y.free()
+    mlp.close()
+
+
 def run_surrogate(r: Renderer, sc, brick, rays: int, steps: int) -> None:
     from bhgrrt import surrogate as bsur
 
@@ -89,6 +112,8 @@ def main(argv=None) -> int:
     ap.add_argument("--surrogate-rays", type=int, default=1 << 18)
     ap.add_argument("--train-steps", type=int, default=64)
     ap.add_argument("--sampler", type=int, default=0, help="points of the FP64 sampler launch (0: none)")
+    ap.add_argument("--mlp-rows", type=int, default=0, help="time transposed inference of the 4->256x4->515 "
+                    "network on this many rows, fused kernel and per-layer GEMMs (0: skip)")
     ap.add_argument("--latency-sizes", default="", help="e.g. 16,64,256,1024: plain FP32 renders of the scene at "
                     "these sizes, printing kernel time, longest ray and time per step of the longest ray")
     args = ap.parse_args(argv)
@@ -121,6 +146,8 @@ def main(argv=None) -> int:
                   f"{1e3 * t / longest:.2f} us per step of the longest ray, {int(out['steps'].sum())} steps in all",
                   flush=True)
         r.release(db)
+    if args.mlp_rows:
+        run_mlp(r, args.mlp_rows)
     if args.sampler:
         run_sampler(r, br, args.sampler)
     if args.surrogate:
