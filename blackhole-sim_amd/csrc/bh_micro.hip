@@ -148,6 +148,85 @@ __global__ void sample_kernel(const double *__restrict__ coeffs, const double *_
     }
 }
 
+// Same sampler with a cooperative gather: a group of 16 lanes serves one point.  The group loads the
+// point's 8 corner records (88 doubles) with consecutive lanes on consecutive doubles of a record, so
+// one load instruction touches a few whole sectors instead of one sector per lane; the values go
+// through LDS, and lane k < 11 interpolates channel k with the nested lerps of sample_aos_d and writes
+// it, so a point's 11 outputs are one contiguous 88-byte store.
+constexpr int kGatherLanes = 16;
+constexpr int kGatherGrid = 2048;   // grid values staged in LDS (r + theta + phi)
+
+template <typename G>
+__device__ __forceinline__ void gather_points(const double *__restrict__ coeffs, const G *rg, int nr, const G *tg,
+                                              int nt, const G *pg, int np, const double *__restrict__ pts, int64_t n,
+                                              double *__restrict__ out, double (*vals)[8 * kCoeffs]) {
+    constexpr int kGroups = 256 / kGatherLanes;
+    constexpr int kVals = 8 * kCoeffs;
+    const int g = threadIdx.x / kGatherLanes, gl = threadIdx.x % kGatherLanes;
+    const int64_t stride = int64_t(gridDim.x) * kGroups;
+    for (int64_t i = int64_t(blockIdx.x) * kGroups + g; i < n; i += stride) {
+        const double *p = pts + i * 3;
+        const double r = p[0], th = p[1], ph = p[2];
+        int r0 = 0, t0 = 0, p0 = 0, p1 = 0;
+        double wr = 0.0, wt = 0.0, wp = 0.0;
+        const bool ok = bracket_linear_d(rg, nr, r, r0, wr) && bracket_linear_d(tg, nt, th, t0, wt);
+        if (ok) {
+            bracket_phi_d(pg, np, ph, p0, p1, wp);
+#pragma unroll
+            for (int round = 0; round < (kVals + kGatherLanes - 1) / kGatherLanes; ++round) {
+                const int f = round * kGatherLanes + gl;   // value index: corner q = f / 11, channel f % 11
+                if (f < kVals) {
+                    const int q = f / kCoeffs, k = f - q * kCoeffs;
+                    const int ri = r0 + (q >> 2), ti = t0 + ((q >> 1) & 1), pi = (q & 1) ? p1 : p0;
+                    vals[g][f] = coeffs[((size_t(ri) * nt + ti) * np + pi) * kCoeffs + k];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (gl < kCoeffs) {
+            double res = __builtin_nan("");
+            if (ok) {
+                const double *v = vals[g];
+                const int k = gl;
+                // corners in order (dr, dt, dp) = q >> 2, (q >> 1) & 1, q & 1, as o000 .. o111
+                const double c00 = (1.0 - wp) * v[0 * kCoeffs + k] + wp * v[1 * kCoeffs + k];
+                const double c01 = (1.0 - wp) * v[2 * kCoeffs + k] + wp * v[3 * kCoeffs + k];
+                const double c10 = (1.0 - wp) * v[4 * kCoeffs + k] + wp * v[5 * kCoeffs + k];
+                const double c11 = (1.0 - wp) * v[6 * kCoeffs + k] + wp * v[7 * kCoeffs + k];
+                const double c0 = (1.0 - wt) * c00 + wt * c01;
+                const double c1 = (1.0 - wt) * c10 + wt * c11;
+                res = (1.0 - wr) * c0 + wr * c1;
+            }
+            out[i * kCoeffs + gl] = res;
+        }
+        __builtin_amdgcn_wave_barrier();   // the next point reuses this group's LDS values
+    }
+}
+
+// Same sampler with a cooperative gather: a group of 16 lanes serves one point.  The group loads the
+// point's 8 corner records (88 doubles) with consecutive lanes on consecutive doubles of a record, so
+// one load instruction touches a few whole sectors instead of one sector per lane; the values go
+// through LDS, and lane k < 11 interpolates channel k with the nested lerps of sample_aos_d and writes
+// it, so a point's 11 outputs are one contiguous 88-byte store.  Grids of up to 2048 values are
+// searched in LDS.
+__global__ __launch_bounds__(256) void sample_gather_kernel(const double *__restrict__ coeffs,
+                                                            const double *__restrict__ rg, int nr,
+                                                            const double *__restrict__ tg, int nt,
+                                                            const double *__restrict__ pg, int np,
+                                                            const double *__restrict__ pts, int64_t n,
+                                                            double *__restrict__ out) {
+    __shared__ double vals[256 / kGatherLanes][8 * kCoeffs];
+    __shared__ double sgrid[kGatherGrid];
+    if (nr + nt + np <= kGatherGrid) {
+        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
+            sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
+        __syncthreads();
+        gather_points<double>(coeffs, sgrid, nr, sgrid + nr, nt, sgrid + nr + nt, np, pts, n, out, vals);
+    } else {
+        gather_points<double>(coeffs, rg, nr, tg, nt, pg, np, pts, n, out, vals);
+    }
+}
+
 __global__ void sample_step_kernel(const double *__restrict__ coeffs, const double *__restrict__ rg, int nr,
                                    const double *__restrict__ tg, int nt, const double *__restrict__ pg, int np,
                                    const double *__restrict__ pts, int64_t n, double ds,
@@ -225,6 +304,13 @@ static int validate_sampler(const double *coeffs, size_t n_coeffs, const double 
 }
 
 static inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
+
+// Grid of sample_gather_kernel: one group of 16 lanes per point, grid-stride beyond 32 blocks per CU.
+static inline unsigned gather_blocks(bh_ctx *ctx, int64_t n) {
+    const int64_t want = (n + 15) / 16;
+    const int64_t cap = int64_t(ctx->cus > 0 ? ctx->cus : 256) * 32;
+    return unsigned(want < cap ? (want > 0 ? want : 1) : cap);
+}
 
 // Copies host arrays into one staging allocation; returns device pointers in order.
 struct Stage {
@@ -313,7 +399,7 @@ int bh_sample_brick_trilinear(const double *coeffs, size_t n_coeffs, const doubl
     double *dout = sg.put<double>(ctx, nullptr, n * kCoeffs, &st);
     if (st) return st;
     begin_timing(ctx);
-    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(int64_t(n), 256)), dim3(256), 0, ctx->stream, dc, dr, int(nr),
+    hipLaunchKernelGGL(sample_gather_kernel, dim3(gather_blocks(ctx, int64_t(n))), dim3(256), 0, ctx->stream, dc, dr, int(nr),
                        dt, int(nt), dp, int(np), dpts, int64_t(n), dout);
     BH_HIP(hipGetLastError());
     end_timing(ctx, 1);
@@ -409,7 +495,7 @@ int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs, const 
     if (n < 0) return set_error(BH_ERR_VALUE, "n must be non-negative");
     if (n == 0) return BH_OK;
     begin_timing(ctx);
-    hipLaunchKernelGGL(sample_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, d_coeffs, d_r, int(nr),
+    hipLaunchKernelGGL(sample_gather_kernel, dim3(gather_blocks(ctx, n)), dim3(256), 0, ctx->stream, d_coeffs, d_r, int(nr),
                        d_theta, int(nt), d_phi, int(np), d_points, n, d_out);
     BH_HIP(hipGetLastError());
     end_timing(ctx, 1);
