@@ -131,34 +131,11 @@ __global__ void stokes_rk2_kernel(const double *__restrict__ coeffs, int64_t cel
     for (int k = 0; k < 4; ++k) op[k] = s[k];
 }
 
-__global__ void sample_kernel(const double *__restrict__ coeffs, const double *__restrict__ rg, int nr,
-                              const double *__restrict__ tg, int nt, const double *__restrict__ pg, int np,
-                              const double *__restrict__ pts, int64_t n, double *__restrict__ out) {
-    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double v[kCoeffs];
-    const double *p = pts + i * 3;
-    double *o = out + i * kCoeffs;
-    if (sample_aos_d(coeffs, rg, nr, tg, nt, pg, np, p[0], p[1], p[2], v)) {
-#pragma unroll
-        for (int k = 0; k < kCoeffs; ++k) o[k] = v[k];
-    } else {
-#pragma unroll
-        for (int k = 0; k < kCoeffs; ++k) o[k] = __builtin_nan("");
-    }
-}
-
-// Same sampler with a cooperative gather: a group of 16 lanes serves one point.  The group loads the
-// point's 8 corner records (88 doubles) with consecutive lanes on consecutive doubles of a record, so
-// one load instruction touches a few whole sectors instead of one sector per lane; the values go
-// through LDS, and lane k < 11 interpolates channel k with the nested lerps of sample_aos_d and writes
-// it, so a point's 11 outputs are one contiguous 88-byte store.
 constexpr int kGatherLanes = 16;
 constexpr int kGatherGrid = 2048;   // grid values staged in LDS (r + theta + phi)
 
-template <typename G>
-__device__ __forceinline__ void gather_points(const double *__restrict__ coeffs, const G *rg, int nr, const G *tg,
-                                              int nt, const G *pg, int np, const double *__restrict__ pts, int64_t n,
+__device__ __forceinline__ void gather_points(const double *__restrict__ coeffs, const double *rg, int nr,
+                                              const double *tg, int nt, const double *pg, int np, const double *__restrict__ pts, int64_t n,
                                               double *__restrict__ out, double (*vals)[8 * kCoeffs]) {
     constexpr int kGroups = 256 / kGatherLanes;
     constexpr int kVals = 8 * kCoeffs;
@@ -203,11 +180,12 @@ __device__ __forceinline__ void gather_points(const double *__restrict__ coeffs,
     }
 }
 
-// Same sampler with a cooperative gather: a group of 16 lanes serves one point.  The group loads the
+// FP64 trilinear sampler over many points (the rule of sample_aos_d; NaN for a point outside the
+// r/theta range) with a cooperative gather: a group of 16 lanes serves one point.  The group loads the
 // point's 8 corner records (88 doubles) with consecutive lanes on consecutive doubles of a record, so
 // one load instruction touches a few whole sectors instead of one sector per lane; the values go
 // through LDS, and lane k < 11 interpolates channel k with the nested lerps of sample_aos_d and writes
-// it, so a point's 11 outputs are one contiguous 88-byte store.  Grids of up to 2048 values are
+// it, so a point's 11 outputs are one contiguous 88-byte store.  Grids of up to 2048 values in all are
 // searched in LDS.
 __global__ __launch_bounds__(256) void sample_gather_kernel(const double *__restrict__ coeffs,
                                                             const double *__restrict__ rg, int nr,
@@ -221,9 +199,9 @@ __global__ __launch_bounds__(256) void sample_gather_kernel(const double *__rest
         for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
             sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
         __syncthreads();
-        gather_points<double>(coeffs, sgrid, nr, sgrid + nr, nt, sgrid + nr + nt, np, pts, n, out, vals);
+        gather_points(coeffs, sgrid, nr, sgrid + nr, nt, sgrid + nr + nt, np, pts, n, out, vals);
     } else {
-        gather_points<double>(coeffs, rg, nr, tg, nt, pg, np, pts, n, out, vals);
+        gather_points(coeffs, rg, nr, tg, nt, pg, np, pts, n, out, vals);
     }
 }
 
