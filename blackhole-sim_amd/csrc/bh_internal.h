@@ -34,6 +34,7 @@ struct GridHint {      // index guess u = scale * f(x) + offset, f = log2 or ide
     float scale;
     float offset;
     int log;
+    float first, last;   // g[0] and g[n - 1], kept in registers for the range tests
 };
 
 struct BrickDev {

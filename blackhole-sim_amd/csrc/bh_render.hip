@@ -206,53 +206,65 @@ __device__ __forceinline__ int guess_index(float x, GridHint h, int nmax) {
     return int(u);
 }
 
-// native_kernels.py:137-146 via guess + correction.
+// Exact bracket search from a guess i, for grids the hint does not fit: the i with
+// (i == 0 || x >= g[i]) && (i == last || x < g[i + 1]), last = n - 2 for r/theta, n - 1 for phi.
 template <typename R>
-__device__ __forceinline__ bool bracket_lin(const R *g, int n, R x, GridHint h, int &i0, R &w) {
-    if (x < g[0] || x > g[n - 1]) return false;
-    int i = guess_index(float(x), h, n - 2);
-    R lo = g[i], hi = g[i + 1];
-    // the hint is exact or off by one cell for the grids it was built for: one branch-free correction
-    const bool down = (i > 0) && (x < lo);
-    const bool up = (i < n - 2) && (x >= hi);
-    if (down | up) {
-        i += up ? 1 : -1;
-        lo = g[i];
-        hi = g[i + 1];
-        if (((i > 0) && (x < lo)) || ((i < n - 2) && (x >= hi))) {   // any other grid: exact search
-            while (i > 0 && x < g[i]) --i;
-            while (i < n - 2 && x >= g[i + 1]) ++i;
-            lo = g[i];
-            hi = g[i + 1];
-        }
-    }
-    i0 = i;
-    w = (x == g[n - 1]) ? R(1) : Mth<R>::div(x - lo, hi - lo);
-    return true;
+__device__ __forceinline__ int bracket_search(const R *g, int last, R x, int i) {
+    while (i > 0 && x < g[i]) --i;
+    while (i < last && x >= g[i + 1]) ++i;
+    return i;
 }
 
-// native_kernels.py:149-161 via guess + correction.
+// Cell and weights of a point: native_kernels.py:137-146 (r, theta: searchsorted-right - 1
+// clipped to [0, n-2], unit weight at the last grid value) and :149-161 (periodic phi with the
+// seam cell [g[n-1], g[0] + 2 pi)).  The index comes from the hint; the six grid values that check
+// it are read from LDS together, and only a point whose check fails searches.
 template <typename R>
-__device__ __forceinline__ void bracket_phi(const R *g, int n, R phi, GridHint h, int &i0, int &i1, R &w) {
-    const R p = Mth<R>::wrap(phi, g[0]);
-    int i = guess_index(float(p), h, n - 1);
-    // g[n] is a +inf sentinel in LDS, so g[i + 1] is always readable
-    const bool down = (i > 0) && (p < g[i]);
-    const bool up = (i < n - 1) && (p >= g[i + 1]);
-    if (down | up) {
-        i += up ? 1 : -1;
-        if (((i > 0) && (p < g[i])) || ((i < n - 1) && (p >= g[i + 1]))) {
-            while (i > 0 && p < g[i]) --i;
-            while (i < n - 1 && p >= g[i + 1]) ++i;
-        }
+struct CellPos {
+    int r0, t0, p0, p1;
+    R wr, wt, wp;
+};
+
+template <typename R>
+__device__ __forceinline__ bool locate(const RenderArgs &A, const R *gr, const R *gt, const R *gp, R x, R y, R z,
+                                       CellPos<R> &c) {
+    const R r_first = R(A.hr.first), r_last = R(A.hr.last), t_first = R(A.ht.first), t_last = R(A.ht.last);
+    if (x < r_first || x > r_last || y < t_first || y > t_last) return false;   // NaN passes, as in the reference
+    const R p_first = R(A.hp.first);
+    const R p = Mth<R>::wrap(z, p_first);
+    int ir = guess_index(float(x), A.hr, A.nr - 2);
+    int it = guess_index(float(y), A.ht, A.nt - 2);
+    int ip = guess_index(float(p), A.hp, A.np - 1);
+    R rl = gr[ir], rh = gr[ir + 1], tl = gt[it], th = gt[it + 1], pl = gp[ip], ph = gp[ip + 1];   // g[np] = +inf
+    // bitwise, not short-circuit: all six reads are needed, so no branch may split them
+    const bool bad = ((ir > 0) & (x < rl)) | ((ir < A.nr - 2) & (x >= rh)) | ((it > 0) & (y < tl)) |
+                     ((it < A.nt - 2) & (y >= th)) | ((ip > 0) & (p < pl)) | ((ip < A.np - 1) & (p >= ph));
+    if (bad) {
+        ir = bracket_search(gr, A.nr - 2, x, ir);
+        it = bracket_search(gt, A.nt - 2, y, it);
+        ip = bracket_search(gp, A.np - 1, p, ip);
+        rl = gr[ir];
+        rh = gr[ir + 1];
+        tl = gt[it];
+        th = gt[it + 1];
+        pl = gp[ip];
+        ph = gp[ip + 1];
     }
-    if (p < g[0]) i = n - 1;
-    const int j = (i + 1 == n) ? 0 : i + 1;
-    const R hi = (j > i) ? g[j] : g[0] + Mth<R>::two_pi();
-    const R pp = (p >= g[i]) ? p : p + Mth<R>::two_pi();
-    i0 = i;
-    i1 = j;
-    w = Mth<R>::div(pp - g[i], hi - g[i]);
+    if (p < p_first) {   // wrap rounding: the seam cell
+        ip = A.np - 1;
+        pl = gp[ip];
+    }
+    const int jp = (ip + 1 == A.np) ? 0 : ip + 1;
+    const R p_hi = (jp > ip) ? ph : p_first + Mth<R>::two_pi();
+    const R pp = (p >= pl) ? p : p + Mth<R>::two_pi();
+    c.r0 = ir;
+    c.t0 = it;
+    c.p0 = ip;
+    c.p1 = jp;
+    c.wr = (x == r_last) ? R(1) : Mth<R>::div(x - rl, rh - rl);
+    c.wt = (y == t_last) ? R(1) : Mth<R>::div(y - tl, th - tl);
+    c.wp = Mth<R>::div(pp - pl, p_hi - pl);
+    return true;
 }
 
 // Trilinear sample of all 11 channels at (r, theta, phi); false when outside r/theta.
@@ -272,11 +284,10 @@ __device__ __forceinline__ float half_hi(unsigned w, int store) {
 template <typename R, int STORE, int LAYOUT>
 __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
                                              const R *gp, R x, R y, R z, R out[kCoeffs]) {
-    int r0, t0, p0, p1;
-    R wr, wt, wp;
-    if (!bracket_lin(gr, A.nr, x, A.hr, r0, wr)) return false;
-    if (!bracket_lin(gt, A.nt, y, A.ht, t0, wt)) return false;
-    bracket_phi(gp, A.np, z, A.hp, p0, p1, wp);
+    CellPos<R> cp;
+    if (!locate(A, gr, gt, gp, x, y, z, cp)) return false;
+    const int r0 = cp.r0, t0 = cp.t0, p0 = cp.p0, p1 = cp.p1;
+    const R wr = cp.wr, wt = cp.wt, wp = cp.wp;
     constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
     if constexpr (LAYOUT == BH_LAYOUT_CORNER) {
         // one cell record: corners (dr, dt, dp) in order dr*4 + dt*2 + dp, 11 channels each
@@ -293,6 +304,8 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
             w[4 * j + 2] = q.z;
             w[4 * j + 3] = q.w;
         }
+        // all loads of the record leave before the first use, so a lone ray waits for one round trip
+        __builtin_amdgcn_sched_barrier(0);
         const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
@@ -318,6 +331,8 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
     const unsigned o[8] = {(row00 + p0) * esz, (row00 + p1) * esz, (row01 + p0) * esz, (row01 + p1) * esz,
                            (row10 + p0) * esz, (row10 + p1) * esz, (row11 + p0) * esz, (row11 + p1) * esz};
     if constexpr (sizeof(R) == 8) {
+        // FP64 keeps the reference's nested lerps; its loads are left to the scheduler (holding all 88
+        // in flight would spill the FP64 kernel's registers)
         const R ap = R(1) - wp, at = R(1) - wt, ar = R(1) - wr;
 #pragma unroll
         for (int k = 0; k < kCoeffs; ++k) {
@@ -334,22 +349,23 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
             out[k] = ar * c0 + wr * c1;
         }
     } else {
-        // issue all 88 loads before any arithmetic so they are in flight together
-        float v[kCoeffs][8];
+        // all 88 loads leave before any arithmetic, so they are in flight together (one round trip)
+        float raw[kCoeffs][8];
 #pragma unroll
         for (int k = 0; k < kCoeffs; ++k) {
             const unsigned so = unsigned(k) * A.plane_bytes;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[k][q] = load_coeff<STORE>(rs, o[q], so);
+            for (int q = 0; q < 8; ++q) raw[k][q] = load_coeff<STORE>(rs, o[q], so);
         }
+        __builtin_amdgcn_sched_barrier(0);
         const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
 #pragma unroll
         for (int k = 0; k < kCoeffs; ++k) {
-            R acc = w8[0] * v[k][0];
+            R acc = w8[0] * raw[k][0];
 #pragma unroll
-            for (int q = 1; q < 8; ++q) acc += w8[q] * v[k][q];
+            for (int q = 1; q < 8; ++q) acc += w8[q] * raw[k][q];
             out[k] = acc;
         }
     }
@@ -930,6 +946,8 @@ static GridHint make_hint(const float *g, int n, bool periodic) {
         if (std::fabs(q) < 1e-12) geometric = false;
     }
     if (periodic) geometric = false;
+    h.first = g[0];
+    h.last = g[n - 1];
     if (geometric) {
         const double l0 = std::log2(g0), l1 = std::log2(gl);
         h.log = 1;
