@@ -52,6 +52,7 @@ struct RenderArgs;
 template <typename R>
 struct HotParams {
     R step, r_plus, r_cap, r_esc, L, tau_max;
+    int max_steps, adaptive;
 };
 
 struct RenderArgs {
@@ -82,8 +83,39 @@ struct RenderArgs {
 
 template <typename R>
 __device__ __forceinline__ HotParams<R> hot_params(const RenderArgs &A) {
-    if constexpr (sizeof(R) == 4) return {A.step_f, A.r_plus_f, A.r_cap_f, A.r_esc_f, A.L_f, A.tau_max_f};
-    else return {A.step, A.r_plus, A.r_cap, A.r_esc, A.L, A.tau_max};
+    if constexpr (sizeof(R) == 4)
+        return {A.step_f, A.r_plus_f, A.r_cap_f, A.r_esc_f, A.L_f, A.tau_max_f, A.max_steps, A.adaptive};
+    else return {A.step, A.r_plus, A.r_cap, A.r_esc, A.L, A.tau_max, A.max_steps, A.adaptive};
+}
+
+// A copy of a wave-uniform 32-bit value that lives in a VGPR.  The step loop's constants would
+// otherwise compete for SGPRs with the launch arguments and the branch masks, be spilled to VGPR
+// lanes and come back through v_readlane on every step.
+__device__ __forceinline__ float in_vgpr(float x) {
+    float y;
+    asm("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
+    return y;
+}
+__device__ __forceinline__ int in_vgpr(int x) {
+    int y;
+    asm("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
+    return y;
+}
+
+// The step loop's constants; FP32 ones are moved to VGPRs (FP64 ones stay where the compiler puts them).
+template <typename R>
+__device__ __forceinline__ HotParams<R> loop_params(const RenderArgs &A) {
+    HotParams<R> P = hot_params<R>(A);
+    if constexpr (sizeof(R) == 4) {
+        P.step = in_vgpr(P.step);
+        P.r_plus = in_vgpr(P.r_plus);
+        P.r_cap = in_vgpr(P.r_cap);
+        P.r_esc = in_vgpr(P.r_esc);
+        P.L = in_vgpr(P.L);
+        P.tau_max = in_vgpr(P.tau_max);
+        P.max_steps = in_vgpr(P.max_steps);
+    }
+    return P;
 }
 
 // ---- precision traits ------------------------------------------------------
@@ -505,8 +537,8 @@ __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr
 // MIXED: the geodesic step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
 // |p_theta| < pth_island near a theta turning point) and the horizon/escape tests compare in FP64.
 template <typename R, int STORE, int LAYOUT, bool MIXED>
-__device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
-                                        const R *gp, RayState<R> &s, unsigned &n_samples) {
+__device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
+                                        const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
     const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
     R nr, nt_raw, np, npr, npt, mt;
     bool island = false;
@@ -527,8 +559,7 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsr
         npt = R(dpt_);
         mt = R(dm_);
     } else {
-        const HotParams<R> P = hot_params<R>(A);
-        const R h = A.adaptive ? P.step * (s.r - P.r_plus) : P.step;
+        const R h = P.adaptive ? P.step * (s.r - P.r_plus) : P.step;
         geo_step<R>(s.k, s.r, s.th, s.ph, s.pr, s.pth, h, nr, nt_raw, np, npr, npt, mt);
     }
     const R nt = clamp_theta(nt_raw);
@@ -542,7 +573,7 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsr
     if (sample_brick<R, STORE, LAYOUT>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt), R(0.5) * (s.ph + np), c)) {
         const R dx = nr - s.r, dy = nt - s.th, dz = np - s.ph;
         const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
-        const R ds = pymax(hot_params<R>(A).L * pymax(dlam, R(1.0e-14)), R(0));
+        const R ds = pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
         stokes_rk2(s.S, c, ds);
         s.tau = s.tau + pymax(c[4], R(0)) * ds;
         ++n_samples;
@@ -554,17 +585,21 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, __amdgpu_buffer_rsr
     s.pr = npr;
     s.pth = npt;
     s.steps += 1;
-    if (!fin) return 3;
+    // first match wins: invalid, captured, escaped, opaque, out of steps (written as selects, no branches)
+    bool captured, escaped;
     if constexpr (MIXED) {
-        if (double(nr) <= A.r_cap) return 1;
-        if (double(nr) > A.r_esc && nr > r_prev) return 2;
+        captured = double(nr) <= A.r_cap;
+        escaped = (double(nr) > A.r_esc) && (nr > r_prev);
     } else {
-        if (nr <= hot_params<R>(A).r_cap) return 1;
-        if (nr > hot_params<R>(A).r_esc && nr > r_prev) return 2;
+        captured = nr <= P.r_cap;
+        escaped = (nr > P.r_esc) && (nr > r_prev);
     }
-    if (s.tau >= hot_params<R>(A).tau_max) return 4;
-    if (s.steps >= A.max_steps) return 0;
-    return -1;
+    int code = (s.steps >= P.max_steps) ? 0 : -1;
+    code = (s.tau >= P.tau_max) ? 4 : code;
+    code = escaped ? 2 : code;
+    code = captured ? 1 : code;
+    code = fin ? code : 3;
+    return code;
 }
 
 template <typename R>
@@ -600,6 +635,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(A.coeffs), (short)0, int(A.total_bytes), int(kRsrcWord3));
 
+    const HotParams<R> P = loop_params<R>(A);
     const int lane = threadIdx.x & 63;
     const int q_own = blockIdx.x % A.queues;
     const unsigned all_q = (1u << A.queues) - 1u;
@@ -661,7 +697,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < A.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
@@ -762,9 +798,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const R *gr = s_grid + A.grid_off;
         const R *gt = gr + A.nr;
         const R *gp = gt + A.nt;
+        const HotParams<R> P = loop_params<R>(A);
         for (int k = 0; k < B.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
