@@ -232,9 +232,14 @@ __device__ __forceinline__ float load_coeff(__amdgpu_buffer_rsrc_t rs, unsigned 
     }
 }
 
+// Index guess from the hint, clamped to [0, nmax]; NaN -> 0.  Only the r grid may be geometric
+// (theta and phi hints are always linear), so only the r guess evaluates a logarithm.
+template <bool MAYBE_LOG>
 __device__ __forceinline__ int guess_index(float x, GridHint h, int nmax) {
-    float u = h.log ? __builtin_amdgcn_logf(x) * h.scale + h.offset : x * h.scale + h.offset;
-    u = fminf(fmaxf(u, 0.f), float(nmax));   // NaN -> 0
+    float u;
+    if constexpr (MAYBE_LOG) u = h.log ? __builtin_amdgcn_logf(x) * h.scale + h.offset : x * h.scale + h.offset;
+    else u = x * h.scale + h.offset;
+    u = fminf(fmaxf(u, 0.f), float(nmax));
     return int(u);
 }
 
@@ -264,9 +269,9 @@ __device__ __forceinline__ bool locate(const RenderArgs &A, const R *gr, const R
     if (x < r_first || x > r_last || y < t_first || y > t_last) return false;   // NaN passes, as in the reference
     const R p_first = R(A.hp.first);
     const R p = Mth<R>::wrap(z, p_first);
-    int ir = guess_index(float(x), A.hr, A.nr - 2);
-    int it = guess_index(float(y), A.ht, A.nt - 2);
-    int ip = guess_index(float(p), A.hp, A.np - 1);
+    int ir = guess_index<true>(float(x), A.hr, A.nr - 2);
+    int it = guess_index<false>(float(y), A.ht, A.nt - 2);
+    int ip = guess_index<false>(float(p), A.hp, A.np - 1);
     R rl = gr[ir], rh = gr[ir + 1], tl = gt[it], th = gt[it + 1], pl = gp[ip], ph = gp[ip + 1];   // g[np] = +inf
     // bitwise, not short-circuit: all six reads are needed, so no branch may split them
     const bool bad = ((ir > 0) & (x < rl)) | ((ir < A.nr - 2) & (x >= rh)) | ((it > 0) & (y < tl)) |
@@ -313,7 +318,10 @@ __device__ __forceinline__ float half_hi(unsigned w, int store) {
                                   : float(__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)));
 }
 
-template <typename R, int STORE, int LAYOUT>
+// COOP (FP32, corner layout): every lane of the wave holds the same ray; lane k (k < 11) loads the 8 corner
+// values of channel k and evaluates that channel's weighted sum in the same order as the per-lane
+// form, so the eleven results are bit-identical to it, and each result is read back from its lane.
+template <typename R, int STORE, int LAYOUT, bool COOP = false>
 __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
                                              const R *gp, R x, R y, R z, R out[kCoeffs]) {
     CellPos<R> cp;
@@ -321,6 +329,27 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
     const int r0 = cp.r0, t0 = cp.t0, p0 = cp.p0, p1 = cp.p1;
     const R wr = cp.wr, wt = cp.wt, wp = cp.wp;
     constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+    if constexpr (COOP) {
+        static_assert(sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER, "the cooperative sample is FP32, corner layout");
+        (void)p1;
+        const unsigned k = min(unsigned(threadIdx.x & 63), unsigned(kCoeffs - 1));   // lanes >= 11 repeat channel 10
+        const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
+        const unsigned base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, base + (unsigned(q * kCoeffs) + k) * esz, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
+        const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
+        const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
+        R acc = w8[0] * R(v[0]);
+#pragma unroll
+        for (int q = 1; q < 8; ++q) acc = acc + w8[q] * R(v[q]);
+#pragma unroll
+        for (int j = 0; j < kCoeffs; ++j)
+            out[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, acc), j));
+        return true;
+    }
     if constexpr (LAYOUT == BH_LAYOUT_CORNER) {
         // one cell record: corners (dr, dt, dp) in order dr*4 + dt*2 + dp, 11 channels each
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
@@ -536,7 +565,7 @@ __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr
 // One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
 // MIXED: the geodesic step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
 // |p_theta| < pth_island near a theta turning point) and the horizon/escape tests compare in FP64.
-template <typename R, int STORE, int LAYOUT, bool MIXED>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool COOP = false>
 __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                         const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
     const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
@@ -570,7 +599,8 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> 
 
     // transfer segment (accelerated_renderer.py:113-125)
     R c[kCoeffs];
-    if (sample_brick<R, STORE, LAYOUT>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt), R(0.5) * (s.ph + np), c)) {
+    if (sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt),
+                                             R(0.5) * (s.ph + np), c)) {
         const R dx = nr - s.r, dy = nt - s.th, dz = np - s.ph;
         const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
         const R ds = pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
@@ -602,6 +632,46 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> 
     return code;
 }
 
+template <typename T>
+__device__ __forceinline__ T lane_value(T x, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        const long long v = __builtin_bit_cast(long long, x);
+        const int lo = __builtin_amdgcn_readlane(int(v & 0xffffffffll), lane);
+        const int hi = __builtin_amdgcn_readlane(int(v >> 32), lane);
+        return __builtin_bit_cast(T, (long long)(unsigned(lo)) | ((long long)hi << 32));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
+    }
+}
+
+// Copy the state of the ray in lane `owner` to every lane of the wave.
+template <typename R>
+__device__ __forceinline__ void broadcast_ray(RayState<R> &s, int owner) {
+    s.r = lane_value(s.r, owner);
+    s.th = lane_value(s.th, owner);
+    s.ph = lane_value(s.ph, owner);
+    s.pr = lane_value(s.pr, owner);
+    s.pth = lane_value(s.pth, owner);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s.S[i] = lane_value(s.S[i], owner);
+    s.tau = lane_value(s.tau, owner);
+    s.k.a = lane_value(s.k.a, owner);
+    s.k.a2 = lane_value(s.k.a2, owner);
+    s.k.E = lane_value(s.k.E, owner);
+    s.k.L = lane_value(s.k.L, owner);
+    s.k.L2 = lane_value(s.k.L2, owner);
+    s.k.a2E2 = lane_value(s.k.a2E2, owner);
+    s.k.twoaEL = lane_value(s.k.twoaEL, owner);
+    s.k.aE = lane_value(s.k.aE, owner);
+    s.k.aL = lane_value(s.k.aL, owner);
+    s.k.fourE = lane_value(s.k.fourE, owner);
+    s.Ed = lane_value(s.Ed, owner);
+    s.Ld = lane_value(s.Ld, owner);
+    s.steps = lane_value(s.steps, owner);
+    s.pix = lane_value(s.pix, owner);
+    s.clamped = lane_value(int(s.clamped), owner) != 0;
+}
+
 template <typename R>
 __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R> &s, int code) {
     R *o = static_cast<R *>(A.iquv) + size_t(s.pix) * 4;
@@ -611,6 +681,29 @@ __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R>
     o[3] = nan_to_zero(s.S[3]);
     if (A.steps) A.steps[s.pix] = s.steps;
     if (A.status) A.status[s.pix] = uint8_t(code | (s.clamped ? 8 : 0));
+}
+
+// A wave left with one live ray (FP32 kernels, corner layout): every lane takes a copy of it, the brick sample of
+// each step is split over the lanes (COOP), and the ray runs to its end; then its owner lane writes
+// the result.  The copies stay identical, so the wave leaves the loop together.
+template <typename R, int STORE, int LAYOUT, bool MIXED>
+__device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
+                                         const R *gr, const R *gt, const R *gp, RayState<R> &s, int lane, int owner,
+                                         unsigned &n_steps, unsigned &n_samples, unsigned long long &iters) {
+    broadcast_ray(s, owner);
+    __builtin_amdgcn_s_setprio(2);
+    unsigned steps = 0, samples = 0;
+    int code;
+    do {
+        code = ray_step<R, STORE, LAYOUT, MIXED, true>(A, P, rs, gr, gt, gp, s, samples);
+        ++steps;
+    } while (code < 0);
+    if (lane == owner) {
+        ray_store(A, s, code);
+        n_steps += steps;
+        n_samples += samples;
+    }
+    iters += steps;
 }
 
 __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
@@ -688,10 +781,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
                 idle = __ballot(!active);
             }
         }
-        const int live = __popcll(__ballot(active));
+        const unsigned long long live_mask = __ballot(active);
+        const int live = __popcll(live_mask);
         if (live == 0) {
             if (exhausted == all_q) break;
             continue;
+        }
+        if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
+            if (live == 1) {   // no refill happened above: the wave serves its last ray with all lanes
+                lone_ray<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
+                                                  n_steps, n_samples, iters);
+                active = false;
+                continue;
+            }
         }
         if (live <= 8) __builtin_amdgcn_s_setprio(2);   // a few long rays hold this wave: let them issue first
         else __builtin_amdgcn_s_setprio(0);
@@ -792,13 +894,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
             }
             if (!took) break;
         }
-        const int live = __popcll(__ballot(active));
-        if (live <= 8) __builtin_amdgcn_s_setprio(2);
-        else __builtin_amdgcn_s_setprio(0);
+        const unsigned long long live_mask = __ballot(active);
+        const int live = __popcll(live_mask);
         const R *gr = s_grid + A.grid_off;
         const R *gt = gr + A.nr;
         const R *gp = gt + A.nt;
         const HotParams<R> P = loop_params<R>(A);
+        if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
+            if (live == 1) {
+                lone_ray<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
+                                                  n_steps, n_samples, iters);
+                active = false;
+                continue;
+            }
+        }
+        if (live <= 8) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < B.steps_per_sync; ++k) {
             if (active) {
                 const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, n_samples);
@@ -970,10 +1081,10 @@ static int validate_scene(const bh_scene_desc &d) {
     return BH_OK;
 }
 
-static GridHint make_hint(const float *g, int n, bool periodic) {
+static GridHint make_hint(const float *g, int n, bool periodic, bool allow_log) {
     GridHint h{};
     const double g0 = g[0], gl = g[n - 1];
-    bool geometric = g0 > 0.0;
+    bool geometric = allow_log && g0 > 0.0;
     if (geometric && n > 2) {
         const double q = std::log2(double(g[1]) / g0);
         for (int i = 1; i < n && geometric; ++i) {
@@ -1250,9 +1361,9 @@ int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, i
     BH_HIP(hipMemcpy(b.coeffs, host.data(), host.size(), hipMemcpyHostToDevice));
     BH_HIP(hipMemcpy(b.grids, g.data(), sizeof(float) * g.size(), hipMemcpyHostToDevice));
     b.hgrids = g;
-    b.hr = make_hint(r_grid, nr, false);
-    b.ht = make_hint(theta_grid, nt, false);
-    b.hp = make_hint(phi_grid, np, true);
+    b.hr = make_hint(r_grid, nr, false, true);
+    b.ht = make_hint(theta_grid, nt, false, false);   // the kernels' theta and phi guesses are linear
+    b.hp = make_hint(phi_grid, np, true, false);
     b.live = true;
     ctx->bricks.push_back(b);
     *brick_id = int32_t(ctx->bricks.size() - 1);
