@@ -222,8 +222,10 @@ __global__ void adam_kernel(float *w, const float *g, float *m, float *v, size_t
 constexpr int kFRows = 128;             // rows per workgroup
 constexpr int kFWidth = 256;            // hidden width
 constexpr int kFActLd = kFWidth + 8;    // LDS row stride (bf16) of the activations
-constexpr int kFBk = 32;                // k per staged weight slice
-constexpr int kFBLd = kFBk + 8;         // LDS row stride (bf16) of a staged slice
+constexpr int kFK = 32;                 // k per MFMA step
+constexpr int kFWaveCols = 64;          // output columns per wave and pass (4 waves x 64 = 256)
+constexpr int kFRowTiles = kFRows / 16;
+constexpr int kFColTiles = kFWaveCols / 16;
 constexpr int kFMaxLayers = 10;
 
 struct FusedArgs {
@@ -236,40 +238,80 @@ struct FusedArgs {
     long long ld;
 };
 
-__global__ __launch_bounds__(256) void mlp_fused_kernel(FusedArgs F) {
+// B fragment of a 16x16x32 MFMA straight from global memory: lane l holds W^T[col][k0 + (l/16)*8 .. +8),
+// 16 contiguous bytes (the weights of a layer, 128 KB, stay in L2 and are shared by every workgroup).
+// Columns past n_cols read the last column (in bounds) and are zeroed without a branch.
+__device__ __forceinline__ bf16x8 load_b_frag(const __bf16 *Wt, int col, int n_cols, int k0) {
+    const bool ok = col < n_cols;
+    const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Wt + size_t(ok ? col : n_cols - 1) * kFWidth + k0);
+    return ok ? v : bf16x8{};
+}
+
+// Bias + SiLU of a hidden-layer tile computed as D = W^T act^T (lane l holds output columns
+// cb .. cb+3, cb = c0 + (l/16)*4, of row l%16), written to the activations as one 8-byte store.
+__device__ __forceinline__ void store_hidden(__bf16 (*act)[kFActLd], int row, int cb, f32x4 bias, f32x4 d) {
+    union {
+        __bf16 h[4];
+        uint2 u;
+    } p;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) p.h[q] = __bf16(silu(d[q] + bias[q]));
+    *reinterpret_cast<uint2 *>(&act[row][cb]) = p.u;
+}
+
+// The four biases of columns cb .. cb+3.
+__device__ __forceinline__ f32x4 bias4(const float *bias, int cb) {
+    return f32x4{bias[cb], bias[cb + 1], bias[cb + 2], bias[cb + 3]};
+}
+
+// Fused transposed inference of the width-256 network.  A workgroup carries 128 rows through every
+// layer with the activations resident in LDS (BF16).  Wave w computes all 128 rows x 64 output
+// columns of a pass: 8 x 4 MFMA 16x16x32 tiles, activation fragments read from LDS, weight fragments
+// read from global memory (L2), the next k-step's weight fragments in flight during the current
+// MFMAs.  Hidden layers compute each tile as D = W^T act^T, so a lane ends with four consecutive
+// columns of one row and writes them back with one 8-byte LDS store; two barriers per hidden layer
+// (all reads of the layer's input done; outputs written).  The output layer computes D = act W, so a
+// lane ends with four consecutive rows of one column: one 16-byte store into y[col][row].  Layer 0
+// (n_in <= 8) is one zero-padded MFMA k-step.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_fused_kernel(FusedArgs F) {
     __shared__ __align__(16) __bf16 act[kFRows][kFActLd];
-    __shared__ __align__(16) __bf16 bs[2][kFWidth][kFBLd];
+    __shared__ __align__(16) __bf16 xs[kFRows][8];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r0 = blockIdx.x * kFRows;
     const int rl = lane & 15, kq = (lane >> 4) * 8;
 
-    // layer 0 on the VALU: x staged as BF16-rounded floats in the (not yet used) slice buffer
-    float *xs = reinterpret_cast<float *>(&bs[0][0][0]);   // kFRows x 8 floats
+    // layer 0: A = W0^T (k 0..7, zero above), B = x^T (BF16-rounded inputs)
     for (int i = tid; i < kFRows * 8; i += 256) {
         const int r = i >> 3, k = i & 7;
-        const float v = (r0 + r < F.rows && k < F.n_in) ? F.x[size_t(r0 + r) * F.n_in + k] : 0.f;
-        xs[i] = float(__bf16(v));
+        xs[r][k] = __bf16((r0 + r < F.rows && k < F.n_in) ? F.x[size_t(r0 + r) * F.n_in + k] : 0.f);
     }
     __syncthreads();
     {
-        const float *W0 = F.w32 + F.w_off[0];   // [8][256]
+        const float *W0 = F.w32 + F.w_off[0];   // [8][256] FP32 master weights
         const float *b0 = F.w32 + F.b_off[0];
+        const int c0 = wave * kFWaveCols;
+        bf16x8 w[kFColTiles];
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-            const int c = lane + 64 * cc;
-            float w[8];
+        for (int j = 0; j < kFColTiles; ++j) {
+            w[j] = bf16x8{};
+            if (lane < 16) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) w[k] = k < F.n_in ? float(__bf16(W0[k * kFWidth + c])) : 0.f;
-            const float bias = b0[c];
-            for (int r = wave * 32; r < wave * 32 + 32; ++r) {
-                float v = 0.f;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) v += xs[r * 8 + k] * w[k];
-                act[r][c] = __bf16(silu(v + bias));
+                for (int k = 0; k < 8; ++k) w[j][k] = __bf16(k < F.n_in ? W0[k * kFWidth + c0 + j * 16 + rl] : 0.f);
             }
         }
+        f32x4 bv[kFColTiles];
+#pragma unroll
+        for (int j = 0; j < kFColTiles; ++j) bv[j] = bias4(b0, c0 + j * 16 + (lane >> 4) * 4);
+#pragma unroll
+        for (int i = 0; i < kFRowTiles; ++i) {
+            const bf16x8 xf = lane < 16 ? *reinterpret_cast<const bf16x8 *>(&xs[i * 16 + rl][0]) : bf16x8{};
+#pragma unroll
+            for (int j = 0; j < kFColTiles; ++j)
+                store_hidden(act, i * 16 + rl, c0 + j * 16 + (lane >> 4) * 4, bv[j],
+                             __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[j], xf, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0));
+        }
     }
-    __syncthreads();   // xs (in bs[0]) is dead from here on
+    __syncthreads();
 
     for (int l = 1; l <= F.depth; ++l) {
         const bool last = l == F.depth;
@@ -277,61 +319,59 @@ __global__ __launch_bounds__(256) void mlp_fused_kernel(FusedArgs F) {
         const __bf16 *Wt = F.wtb + F.w_off[l];   // [n_cols][256]
         const float *bias = F.w32 + F.b_off[l];
         for (int n0 = 0; n0 < n_cols; n0 += kFWidth) {
-            f32x4 acc[2][16];
+            const int c0 = n0 + wave * kFWaveCols;
+            f32x4 acc[kFRowTiles][kFColTiles];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < kFRowTiles; ++i)
 #pragma unroll
-                for (int j = 0; j < 16; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-            // thread t stages row n0 + t of the slice: 32 k-values = 4 x 16 bytes
-            const bool row_ok = n0 + tid < n_cols;
-            const uint4 *src = reinterpret_cast<const uint4 *>(Wt + size_t(n0 + tid) * kFWidth);
-            uint4 pre[4];
+                for (int j = 0; j < kFColTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (c0 < n_cols) {   // wave-uniform: the output layer's last pass may leave waves idle
+                bf16x8 w[kFColTiles];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) pre[q] = row_ok ? src[q] : uint4{0u, 0u, 0u, 0u};
-            for (int kt = 0; kt < kFWidth / kFBk; ++kt) {
-                const int buf = kt & 1;
-                uint4 *dst = reinterpret_cast<uint4 *>(&bs[buf][tid][0]);
+                for (int j = 0; j < kFColTiles; ++j) w[j] = load_b_frag(Wt, c0 + j * 16 + rl, n_cols, kq);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) dst[q] = pre[q];
-                __syncthreads();
-                if (kt + 1 < kFWidth / kFBk) {   // next slice in flight during this slice's MFMAs
+                for (int kt = 0; kt < kFWidth / kFK; ++kt) {
+                    bf16x8 wn[kFColTiles];
+                    if (kt + 1 < kFWidth / kFK) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) pre[q] = row_ok ? src[(kt + 1) * 4 + q] : uint4{0u, 0u, 0u, 0u};
-                }
-                bf16x8 a[2];
+                        for (int j = 0; j < kFColTiles; ++j)
+                            wn[j] = load_b_frag(Wt, c0 + j * 16 + rl, n_cols, (kt + 1) * kFK + kq);
+                    }
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
-                    a[i] = *reinterpret_cast<const bf16x8 *>(&act[wave * 32 + i * 16 + rl][kt * kFBk + kq]);
+                    for (int i = 0; i < kFRowTiles; ++i) {
+                        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&act[i * 16 + rl][kt * kFK + kq]);
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    if (n0 + j * 16 >= n_cols) break;   // wave-uniform: the last output pass is partial
-                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&bs[buf][j * 16 + rl][kq]);
+                        for (int j = 0; j < kFColTiles; ++j)
+                            acc[i][j] = last ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[j], acc[i][j], 0, 0, 0)
+                                             : __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[j], a, acc[i][j], 0, 0, 0);
+                    }
+                    if (kt + 1 < kFWidth / kFK) {
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b, acc[i][j], 0, 0, 0);
+                        for (int j = 0; j < kFColTiles; ++j) w[j] = wn[j];
+                    }
                 }
             }
-            // epilogue.  Lane l holds rows (l/16)*4 + q, column l%16 of each 16 x 16 tile.
             if (!last) {
-                // every wave has read its last A operand of this layer (program order within the wave)
+                f32x4 bv[kFColTiles];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < kFColTiles; ++j) bv[j] = bias4(bias, c0 + j * 16 + (lane >> 4) * 4);
+                __syncthreads();   // every wave has read this layer's input activations
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int c = j * 16 + rl;
-                        const float bv = bias[c];
+                for (int i = 0; i < kFRowTiles; ++i)
 #pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            act[wave * 32 + i * 16 + (lane >> 4) * 4 + q][c] = __bf16(silu(acc[i][j][q] + bv));
-                    }
-            } else {
+                    for (int j = 0; j < kFColTiles; ++j)
+                        store_hidden(act, i * 16 + rl, c0 + j * 16 + (lane >> 4) * 4, bv[j], acc[i][j]);
+                __syncthreads();   // the next layer reads all 256 columns
+            } else if (c0 < n_cols) {
+                // lane l holds rows (l/16)*4 + q and column l%16 of each 16 x 16 tile
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < kFColTiles; ++j) {
+                    const int c = c0 + j * 16 + rl;
+                    if (c >= F.n_out) continue;
+                    const float bv = bias[c];
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int c = n0 + j * 16 + rl;
-                        if (c >= F.n_out) continue;
-                        const float bv = bias[c];
-                        const int row = r0 + wave * 32 + i * 16 + (lane >> 4) * 4;
+                    for (int i = 0; i < kFRowTiles; ++i) {
+                        const int row = r0 + i * 16 + (lane >> 4) * 4;
                         float *o = F.y + size_t(c) * F.ld + row;
                         if (row + 3 < F.rows && (F.ld & 3) == 0) {
                             *reinterpret_cast<float4 *>(o) =
@@ -342,8 +382,8 @@ __global__ __launch_bounds__(256) void mlp_fused_kernel(FusedArgs F) {
                                 if (row + q < F.rows) o[q] = acc[i][j][q] + bv;
                         }
                     }
+                }
             }
-            __syncthreads();   // slice buffers and (for the next layer) activations are reused
         }
     }
 }
