@@ -14,6 +14,9 @@
 
 namespace bh {
 
+// dword 3 of a raw buffer resource on gfx950 (raw access, no format conversion); loads past
+// num_records return zero
+constexpr unsigned kRsrcWord3 = 0x00020000u;
 constexpr int kCoeffs = 11;
 constexpr int kStokes = 4;
 constexpr double kTwoPi = 2.0 * 3.14159265358979323846;

@@ -236,15 +236,22 @@ struct FusedArgs {
     long long w_off[kFMaxLayers], b_off[kFMaxLayers];
     float *y;                // [n_out][ld], rows r0 .. of this launch at column offset 0
     long long ld;
+    unsigned wt_bytes;       // bytes of the W^T buffer (buffer-resource bound)
 };
 
-// B fragment of a 16x16x32 MFMA straight from global memory: lane l holds W^T[col][k0 + (l/16)*8 .. +8),
-// 16 contiguous bytes (the weights of a layer, 128 KB, stay in L2 and are shared by every workgroup).
-// Columns past n_cols read the last column (in bounds) and are zeroed without a branch.
-__device__ __forceinline__ bf16x8 load_b_frag(const __bf16 *Wt, int col, int n_cols, int k0) {
-    const bool ok = col < n_cols;
-    const bf16x8 v = *reinterpret_cast<const bf16x8 *>(Wt + size_t(ok ? col : n_cols - 1) * kFWidth + k0);
-    return ok ? v : bf16x8{};
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Byte offset of lane l's weight fragment row: W^T[col][(l/16)*8 ..] of the layer at wbase; columns
+// past n_cols point beyond the buffer, so their loads return zeros.
+__device__ __forceinline__ unsigned w_frag_offset(unsigned wbase, int col, int n_cols, int kq) {
+    return col < n_cols ? wbase + unsigned(col) * unsigned(kFWidth * 2) + unsigned(kq * 2) : 0x80000000u;
+}
+
+// Weight (W^T) fragment of a 16x16x32 MFMA straight from global memory through a buffer resource:
+// 16 contiguous bytes per lane at k-step kt (the weights of a layer, 128 KB, stay in L2 and are shared
+// by every workgroup).
+__device__ __forceinline__ bf16x8 load_w_frag(__amdgpu_buffer_rsrc_t wrs, unsigned voff, int kt) {
+    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(wrs, int(voff), kt * kFK * 2, 0));
 }
 
 // Bias + SiLU of a hidden-layer tile computed as D = W^T act^T (lane l holds output columns
@@ -264,6 +271,66 @@ __device__ __forceinline__ f32x4 bias4(const float *bias, int cb) {
     return f32x4{bias[cb], bias[cb + 1], bias[cb + 2], bias[cb + 3]};
 }
 
+// acc[i][j] += the 16 x 16 tile (row tile i_lo + i, column tile j at column cb + 16 j) of act x W over
+// the full k = 256; TRANS computes each tile as W^T act^T instead (lane l: columns (l/16)*4 + q of
+// row l%16), else as act W (lane l: rows (l/16)*4 + q of column l%16).
+template <int NI, bool TRANS>
+__device__ __forceinline__ void gemm_tiles(const __bf16 (*act)[kFActLd], __amdgpu_buffer_rsrc_t wrs, unsigned wbase,
+                                           int n_cols, int cb, int i_lo, f32x4 (&acc)[NI][kFColTiles], int rl, int kq) {
+    unsigned voff[kFColTiles];
+    bf16x8 w[kFColTiles];
+#pragma unroll
+    for (int j = 0; j < kFColTiles; ++j) {
+        voff[j] = w_frag_offset(wbase, cb + j * 16 + rl, n_cols, kq);
+        w[j] = load_w_frag(wrs, voff[j], 0);
+    }
+#pragma unroll
+    for (int kt = 0; kt < kFWidth / kFK; ++kt) {
+        bf16x8 wn[kFColTiles];
+        if (kt + 1 < kFWidth / kFK) {
+#pragma unroll
+            for (int j = 0; j < kFColTiles; ++j) wn[j] = load_w_frag(wrs, voff[j], kt + 1);
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&act[(i_lo + i) * 16 + rl][kt * kFK + kq]);
+#pragma unroll
+            for (int j = 0; j < kFColTiles; ++j)
+                acc[i][j] = TRANS ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[j], a, acc[i][j], 0, 0, 0)
+                                  : __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < kFWidth / kFK) {
+#pragma unroll
+            for (int j = 0; j < kFColTiles; ++j) w[j] = wn[j];
+        }
+    }
+}
+
+// Output-layer tiles (act W form) + bias into y[col][row]: one 16-byte store of four rows per tile.
+template <int NI>
+__device__ __forceinline__ void store_output(const FusedArgs &F, const float *bias, int cb, int row0,
+                                             const f32x4 (&acc)[NI][kFColTiles], int lane) {
+#pragma unroll
+    for (int j = 0; j < kFColTiles; ++j) {
+        const int c = cb + j * 16 + (lane & 15);
+        if (c >= F.n_out) continue;
+        const float bv = bias[c];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int row = row0 + i * 16 + (lane >> 4) * 4;
+            float *o = F.y + size_t(c) * F.ld + row;
+            if (row + 3 < F.rows && (F.ld & 3) == 0) {
+                *reinterpret_cast<float4 *>(o) =
+                    make_float4(acc[i][j][0] + bv, acc[i][j][1] + bv, acc[i][j][2] + bv, acc[i][j][3] + bv);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (row + q < F.rows) o[q] = acc[i][j][q] + bv;
+            }
+        }
+    }
+}
+
 // Fused transposed inference of the width-256 network.  A workgroup carries 128 rows through every
 // layer with the activations resident in LDS (BF16).  Wave w computes all 128 rows x 64 output
 // columns of a pass: 8 x 4 MFMA 16x16x32 tiles, activation fragments read from LDS, weight fragments
@@ -279,6 +346,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r0 = blockIdx.x * kFRows;
     const int rl = lane & 15, kq = (lane >> 4) * 8;
+    const __amdgpu_buffer_rsrc_t wrs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16 *>(F.wtb), (short)0, int(F.wt_bytes), int(kRsrcWord3));
 
     // layer 0: A = W0^T (k 0..7, zero above), B = x^T (BF16-rounded inputs)
     for (int i = tid; i < kFRows * 8; i += 256) {
@@ -313,76 +382,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     }
     __syncthreads();
 
-    for (int l = 1; l <= F.depth; ++l) {
-        const bool last = l == F.depth;
-        const int n_cols = last ? F.n_out_pad : kFWidth;
-        const __bf16 *Wt = F.wtb + F.w_off[l];   // [n_cols][256]
+    for (int l = 1; l < F.depth; ++l) {   // hidden layers
+        const unsigned wbase = unsigned(F.w_off[l] * 2);   // W^T [256][256] of this layer, bytes
         const float *bias = F.w32 + F.b_off[l];
+        const int c0 = wave * kFWaveCols;
+        f32x4 acc[kFRowTiles][kFColTiles];
+#pragma unroll
+        for (int i = 0; i < kFRowTiles; ++i)
+#pragma unroll
+            for (int j = 0; j < kFColTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        gemm_tiles<kFRowTiles, true>(act, wrs, wbase, kFWidth, c0, 0, acc, rl, kq);
+        f32x4 bv[kFColTiles];
+#pragma unroll
+        for (int j = 0; j < kFColTiles; ++j) bv[j] = bias4(bias, c0 + j * 16 + (lane >> 4) * 4);
+        __syncthreads();   // every wave has read this layer's input activations
+#pragma unroll
+        for (int i = 0; i < kFRowTiles; ++i)
+#pragma unroll
+            for (int j = 0; j < kFColTiles; ++j) store_hidden(act, i * 16 + rl, c0 + j * 16 + (lane >> 4) * 4, bv[j], acc[i][j]);
+        __syncthreads();   // the next layer reads all 256 columns
+    }
+    {   // output layer, passes of 256 columns; a last pass of <= 64 columns is split over the waves by rows
+        const int n_cols = F.n_out_pad;
+        const unsigned wbase = unsigned(F.w_off[F.depth] * 2);   // W^T [n_cols][256], bytes
+        const float *bias = F.w32 + F.b_off[F.depth];
         for (int n0 = 0; n0 < n_cols; n0 += kFWidth) {
-            const int c0 = n0 + wave * kFWaveCols;
-            f32x4 acc[kFRowTiles][kFColTiles];
+            if (n_cols - n0 > kFWaveCols) {
+                const int c0 = n0 + wave * kFWaveCols;
+                if (c0 < n_cols) {   // wave-uniform
+                    f32x4 acc[kFRowTiles][kFColTiles];
 #pragma unroll
-            for (int i = 0; i < kFRowTiles; ++i)
+                    for (int i = 0; i < kFRowTiles; ++i)
 #pragma unroll
-                for (int j = 0; j < kFColTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-            if (c0 < n_cols) {   // wave-uniform: the output layer's last pass may leave waves idle
-                bf16x8 w[kFColTiles];
-#pragma unroll
-                for (int j = 0; j < kFColTiles; ++j) w[j] = load_b_frag(Wt, c0 + j * 16 + rl, n_cols, kq);
-#pragma unroll
-                for (int kt = 0; kt < kFWidth / kFK; ++kt) {
-                    bf16x8 wn[kFColTiles];
-                    if (kt + 1 < kFWidth / kFK) {
-#pragma unroll
-                        for (int j = 0; j < kFColTiles; ++j)
-                            wn[j] = load_b_frag(Wt, c0 + j * 16 + rl, n_cols, (kt + 1) * kFK + kq);
-                    }
-#pragma unroll
-                    for (int i = 0; i < kFRowTiles; ++i) {
-                        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&act[i * 16 + rl][kt * kFK + kq]);
-#pragma unroll
-                        for (int j = 0; j < kFColTiles; ++j)
-                            acc[i][j] = last ? __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[j], acc[i][j], 0, 0, 0)
-                                             : __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[j], a, acc[i][j], 0, 0, 0);
-                    }
-                    if (kt + 1 < kFWidth / kFK) {
-#pragma unroll
-                        for (int j = 0; j < kFColTiles; ++j) w[j] = wn[j];
-                    }
+                        for (int j = 0; j < kFColTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    gemm_tiles<kFRowTiles, false>(act, wrs, wbase, n_cols, c0, 0, acc, rl, kq);
+                    store_output<kFRowTiles>(F, bias, c0, r0, acc, lane);
                 }
-            }
-            if (!last) {
-                f32x4 bv[kFColTiles];
+            } else {
+                constexpr int kSplit = kFRowTiles / 4;
+                f32x4 acc[kSplit][kFColTiles];
 #pragma unroll
-                for (int j = 0; j < kFColTiles; ++j) bv[j] = bias4(bias, c0 + j * 16 + (lane >> 4) * 4);
-                __syncthreads();   // every wave has read this layer's input activations
+                for (int i = 0; i < kSplit; ++i)
 #pragma unroll
-                for (int i = 0; i < kFRowTiles; ++i)
-#pragma unroll
-                    for (int j = 0; j < kFColTiles; ++j)
-                        store_hidden(act, i * 16 + rl, c0 + j * 16 + (lane >> 4) * 4, bv[j], acc[i][j]);
-                __syncthreads();   // the next layer reads all 256 columns
-            } else if (c0 < n_cols) {
-                // lane l holds rows (l/16)*4 + q and column l%16 of each 16 x 16 tile
-#pragma unroll
-                for (int j = 0; j < kFColTiles; ++j) {
-                    const int c = c0 + j * 16 + rl;
-                    if (c >= F.n_out) continue;
-                    const float bv = bias[c];
-#pragma unroll
-                    for (int i = 0; i < kFRowTiles; ++i) {
-                        const int row = r0 + i * 16 + (lane >> 4) * 4;
-                        float *o = F.y + size_t(c) * F.ld + row;
-                        if (row + 3 < F.rows && (F.ld & 3) == 0) {
-                            *reinterpret_cast<float4 *>(o) =
-                                make_float4(acc[i][j][0] + bv, acc[i][j][1] + bv, acc[i][j][2] + bv, acc[i][j][3] + bv);
-                        } else {
-#pragma unroll
-                            for (int q = 0; q < 4; ++q)
-                                if (row + q < F.rows) o[q] = acc[i][j][q] + bv;
-                        }
-                    }
-                }
+                    for (int j = 0; j < kFColTiles; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                gemm_tiles<kSplit, false>(act, wrs, wbase, n_cols, n0, wave * kSplit, acc, rl, kq);
+                store_output<kSplit>(F, bias, n0, r0 + wave * kSplit * 16, acc, lane);
             }
         }
     }
@@ -600,7 +644,7 @@ int mlp_out_cols(const bh_mlp *m) { return m->n_out; }
 
 static bool fused_ok(const bh_mlp *m) {
     return m->fused && m->width == kFWidth && m->n_in_pad == 8 && m->depth + 1 <= kFMaxLayers &&
-           m->n_out_pad <= 4 * kFWidth;
+           m->n_out_pad <= 4 * kFWidth && m->n_params * 2 < (size_t(1) << 31);
 }
 
 // Fused inference of `rows` rows into y [n_out][ld] (transposed).
@@ -614,6 +658,7 @@ static int forward_fused(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, 
     F.n_out_pad = m->n_out_pad;
     F.w32 = m->w32;
     F.wtb = m->wtb;
+    F.wt_bytes = unsigned(m->n_params * 2);
     for (int l = 0; l <= m->depth; ++l) {
         F.w_off[l] = (long long)m->w_off[l];
         F.b_off[l] = (long long)m->b_off[l];

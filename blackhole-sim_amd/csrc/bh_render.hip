@@ -36,7 +36,6 @@
 namespace bh {
 
 constexpr int kMaxGrid = 2048;   // r + theta + phi grid values staged in LDS
-constexpr unsigned kRsrcWord3 = 0x00020000u;
 constexpr double kIslandTheta = 0.1;    // rad from a pole: FP64 geodesic steps in the mixed path
 constexpr double kIslandPtheta = 0.25;  // |p_theta| below this: near a theta turning point
 
