@@ -53,6 +53,18 @@ def rel_l1(a, b) -> float:
     return float(np.sum(np.abs(a - b)) / max(float(np.sum(np.abs(b))), 1e-300))
 
 
+def pixel_rel_I(I, I64, ok) -> dict:
+    """Per-pixel relative error on I over unflagged pixels whose FP64 I is not negligible
+    (|I64| > 1e-6 of the image's largest; pixels whose rays never sample the brick have I = 0)."""
+    a, b = np.asarray(I, np.float64)[ok], np.asarray(I64, np.float64)[ok]
+    keep = np.abs(b) > 1e-6 * max(float(np.abs(b).max()), 1e-300)
+    rel = np.abs(a[keep] - b[keep]) / np.abs(b[keep])
+    if rel.size == 0:
+        return {"pixels": 0}
+    return {"pixels": int(rel.size), "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+            "max": float(rel.max())}
+
+
 def cpu_baseline(scene, brick, seconds: float) -> dict:
     """Time the FP64 NumPy oracle on evenly spaced image rows using every host core."""
     from oracle import render as orender
@@ -180,7 +192,7 @@ def surrogate_path(rr, scene, brick, ref, I64, args) -> dict:
     out = {"mrays_s": n / dt / 1e6, "ms_per_render": dt * 1e3, "kernel_ms": float(np.mean(kms)),
            "rel_l1_I": rel_l1(iquv[..., 0][ok].astype(np.float64), I64[ok]),
            "max_abs_QUV": float(np.max(np.abs(iquv[..., 1:][ok].astype(np.float64) - ref["iquv"][..., 1:][ok]))),
-           "fallback_frac": fallback / n, "train_steps": info["steps"], "train_s": round(train_s, 2),
+           "pixel_rel_I": pixel_rel_I(iquv[..., 0], I64, ok), "fallback_frac": fallback / n, "train_steps": info["steps"], "train_s": round(train_s, 2),
            "train_loss": info["loss"], "train_rays": info["n_rays"], "flagged_train_frac": info["flagged"],
            "knots": cfg.n_knots, "width": cfg.width, "depth": cfg.depth}
     log(f"[bench] surrogate_mixed_bf16: {out}")
@@ -246,7 +258,9 @@ def main(argv=None) -> int:
         f"{value:.1f} Mrays/s, rel L1 I {err32:.2e}")
 
     paths = {"plain_fp32": {"mrays_s": value, "ms_per_render": plain["ms_per_step"], "kernel_ms": plain["kernel_ms"],
-                            "rel_l1_I": err32, "max_abs_QUV": quv32, "tflops": achieved, "layout": "corner"},
+                            "rel_l1_I": err32, "max_abs_QUV": quv32,
+                            "pixel_rel_I": pixel_rel_I(out32["iquv"][..., 0], I64, ok), "tflops": achieved,
+                            "layout": "corner"},
              "fp64_reference": {"kernel_ms": fp64_ms, "mrays_s": n / (fp64_ms * 1e-3) / 1e6}}
     log("[bench] plain FP32 path (SoA f32 brick)")
     soa = time_path(rr, scene, db32, PREC_FP32, bufs, args.steps, args.warmup)
@@ -264,6 +278,7 @@ def main(argv=None) -> int:
             "kernel_ms": mx["kernel_ms"],
             "rel_l1_I": rel_l1(outx["iquv"][..., 0][okx].astype(np.float64), I64[okx]),
             "max_abs_QUV": float(np.max(np.abs(outx["iquv"][..., 1:][okx].astype(np.float64) - ref["iquv"][..., 1:][okx]))),
+            "pixel_rel_I": pixel_rel_I(outx["iquv"][..., 0], I64, okx),
             "pole_flagged_frac": float(1 - okx.mean()), "rk2_steps": st, "brick_samples": sm}
         log(f"[bench] mixed {storage}: {paths[f'mixed_{storage}']}")
         rr.release(dbx)
