@@ -187,3 +187,53 @@ def test_invalid_arguments_are_errors(renderer):
         assert np.all(np.isfinite(out["iquv"]))
     finally:
         renderer.release(db)
+
+
+def test_step_cap_matches_oracle(renderer):
+    """max_steps = 1, 2, 7 with the camera inside the brick (r = 30), so the first steps already sample:
+    every ray stops at the cap unless an earlier rule ends it, exactly as in the oracle."""
+    br = scenes.scale_brick()
+    db = renderer.upload(br, "f32", "soa")
+    cam = scenes.Camera.spanning(r=30.0, inclination_degrees=60.0, half_width_m=12.0, width=24, height=16)
+    try:
+        for cap in (1, 2, 7):
+            sc = scenes.Scene(spin=0.94, camera=cam, width=24, height=16, step_mode=scenes.STEP_ADAPTIVE, step=0.05,
+                              max_steps=cap, escape_radius=220.0)
+            out = renderer.render(sc, db, PREC_FP64)
+            ref = orender.render_rays(sc, br)
+            np.testing.assert_array_equal(out["status"].ravel(), ref["status"])
+            np.testing.assert_array_equal(out["steps"].ravel(), ref["steps"])
+            assert out["steps"].max() <= cap
+            I, I0 = out["iquv"].reshape(-1, 4)[:, 0], ref["iquv"][:, 0]
+            assert np.allclose(I, I0, rtol=1e-12, atol=1e-300)
+    finally:
+        renderer.release(db)
+
+
+def test_large_image_fp32_against_fp64(renderer):
+    """2048 x 2048 (4 M rays, 16x the 512^2 grid of queue slots): FP32 against FP64 over the whole image."""
+    sc, br = scenes.scale_plain_scene(2048, 2048), scenes.scale_brick()
+    d32, dc = renderer.upload(br, "f32", "soa"), renderer.upload(br, "f32", "corner")
+    try:
+        o64 = renderer.render(sc, d32, PREC_FP64)
+        o32 = renderer.render(sc, dc, PREC_FP32)
+    finally:
+        renderer.release(d32)
+        renderer.release(dc)
+    ok = ((o64["status"] | o32["status"]) & orender.ST_POLE_CLAMPED) == 0
+    assert np.mean((o64["status"] == o32["status"])[ok]) >= 0.99
+    I32, I64 = o32["iquv"][..., 0][ok].astype(np.float64), o64["iquv"][..., 0][ok]
+    e = float(np.abs(I32 - I64).sum() / np.abs(I64).sum())
+    print(f"2048^2 fp32 vs fp64: rel L1 I {e:.2e}, flagged {1 - ok.mean():.4f}, longest ray {o32['steps'].max()}")
+    assert e <= 2e-3
+    assert o32["steps"].max() <= sc.max_steps and o32["steps"].min() >= 1
+
+
+def test_image_size_limit_is_an_error(renderer):
+    sc = scenes.scale_plain_scene(8, 8)
+    db = renderer.upload(scenes.scale_brick(), "f32", "corner")
+    try:
+        with pytest.raises(ValueError, match="image is too large"):
+            renderer.render(sc.with_size(1 << 16, 1 << 15), db, PREC_FP32)
+    finally:
+        renderer.release(db)
