@@ -28,7 +28,8 @@ def scene_desc(scene: Scene) -> _lib.SceneDesc:
         fov_y_degrees=float(cam.fov_y_degrees), roll_degrees=float(cam.roll_degrees),
         width=int(scene.width), height=int(scene.height), step_mode=int(scene.step_mode),
         max_steps=int(scene.max_steps), step=float(scene.step), escape_radius=float(scene.escape_radius),
-        affine_to_length_cm=float(scene.affine_to_length_cm), max_optical_depth=float(scene.max_optical_depth))
+        affine_to_length_cm=float(scene.affine_to_length_cm), max_optical_depth=float(scene.max_optical_depth),
+        transfer=_lib.TRANSFER_EXACT if scene.exact_step else _lib.TRANSFER_RK2)
 
 
 @dataclass

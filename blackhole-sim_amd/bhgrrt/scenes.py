@@ -92,6 +92,7 @@ class Scene:
     escape_radius: float = 220.0
     affine_to_length_cm: float = 1.0
     max_optical_depth: float = 30.0
+    exact_step: bool = False      # PolarizedTransferConfig.use_exact_matrix_step (else one RK2 step per segment)
 
     @property
     def aspect(self) -> float:

@@ -131,6 +131,7 @@ struct Mth<float> {
     static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float abs(float x) { return __builtin_fabsf(x); }
     static __device__ __forceinline__ float two_pi() { return 6.28318530717958648f; }
     static __device__ __forceinline__ float pi_minus_eps() { return float(3.14159265358979323846 - kEpsTheta); }
     static __device__ __forceinline__ float wrap(float phi, float base) {
@@ -148,6 +149,7 @@ struct Mth<double> {
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double div(double a, double b) { return a / b; }
     static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double abs(double x) { return __builtin_fabs(x); }
     static __device__ __forceinline__ double two_pi() { return kTwoPi; }
     static __device__ __forceinline__ double pi_minus_eps() { return 3.14159265358979323846 - kEpsTheta; }
     // Python float % (native_kernels.py:133-134): exact fmod, shifted when negative.
@@ -452,6 +454,130 @@ __device__ __forceinline__ void stokes_rk2(R s[4], const R c[kCoeffs], R ds) {
     for (int i = 0; i < 4; ++i) s[i] = s[i] + ds * k2[i];
 }
 
+// Exact constant-coefficient step of dS/ds = j - K S over ds (polarized_transfer.py:51-70, which takes
+// scipy's expm of the augmented matrix [[-K, j], [0, 0]] ds and applies it to [S; 1]).  With
+// A = -K h and b = j h for h = ds / 2^p (p halvings bring the infinity norm of A to <= 1/2),
+// exp([[A, b], [0, 0]]) = [[I + A F, F b], [0, 1]] where F = sum_k A^k / (k+1)!, truncated at order
+// TAYLOR (FP64 14, FP32 8: remainder below the rounding unit at norm 1/2) and evaluated by Horner.
+// Up to kExactSubsteps halvings the 2^p substeps S <- S + F (A S + b) are applied as matrix-vector
+// products (a 4x4 matrix and two 4-vectors live); beyond that (huge ds: optically thick cells,
+// pole-clamped rays) the block exponential (P, w) is formed once and squared p times,
+// (P, w) -> (P P, P w + w).  Only the kernels built for the exact rule contain this code.
+constexpr int kExactSubsteps = 3;
+
+template <typename R>
+__device__ __forceinline__ void exact_operator(const R c[kCoeffs], R ds, R A[4][4], R b[4], int &p) {
+    const R ai = c[4], aq = c[5], au = c[6], av = c[7], rv = c[8], rq = c[9], ru = c[10];
+    const R K[4][4] = {{ai, aq, au, av}, {aq, ai, rv, -ru}, {au, -rv, ai, rq}, {av, ru, -rq, ai}};
+    R norm = R(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const R row = Mth<R>::abs(K[i][0]) + Mth<R>::abs(K[i][1]) + Mth<R>::abs(K[i][2]) + Mth<R>::abs(K[i][3]);
+        norm = row > norm ? row : norm;
+    }
+    norm *= ds;
+    p = 0;
+    R h = ds;
+    while (norm > R(0.5) && p < 1100) {   // a NaN norm leaves p = 0; 2^-1100 underflows any finite norm
+        norm *= R(0.5);
+        h *= R(0.5);
+        ++p;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) A[i][j] = -K[i][j] * h;
+        b[i] = c[i] * h;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void stokes_exact_squaring(R s[4], const R c[kCoeffs], R ds) {
+    constexpr int TAYLOR = sizeof(R) == 8 ? 14 : 8;
+    R A[4][4], b[4];
+    int p;
+    exact_operator(c, ds, A, b, p);
+    R F[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) F[i][j] = R(i == j);
+    for (int k = TAYLOR + 1; k >= 2; --k) {   // F = I + A F / k
+        const R inv = R(1) / R(k);
+        R G[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                G[i][j] = R(i == j) + (A[i][0] * F[0][j] + A[i][1] * F[1][j] + A[i][2] * F[2][j] + A[i][3] * F[3][j]) * inv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) F[i][j] = G[i][j];
+    }
+    R P[4][4], w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            P[i][j] = R(i == j) + A[i][0] * F[0][j] + A[i][1] * F[1][j] + A[i][2] * F[2][j] + A[i][3] * F[3][j];
+        w[i] = F[i][0] * b[0] + F[i][1] * b[1] + F[i][2] * b[2] + F[i][3] * b[3];
+    }
+    for (int q = 0; q < p; ++q) {   // (P, w) <- (P P, P w + w)
+        R P2[4][4], w2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                P2[i][j] = P[i][0] * P[0][j] + P[i][1] * P[1][j] + P[i][2] * P[2][j] + P[i][3] * P[3][j];
+            w2[i] = P[i][0] * w[0] + P[i][1] * w[1] + P[i][2] * w[2] + P[i][3] * w[3] + w[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) P[i][j] = P2[i][j];
+            w[i] = w2[i];
+        }
+    }
+    R out[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[i] = P[i][0] * s[0] + P[i][1] * s[1] + P[i][2] * s[2] + P[i][3] * s[3] + w[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = out[i];
+}
+
+template <typename R>
+__device__ __forceinline__ void stokes_exact(R s[4], const R c[kCoeffs], R ds) {
+    if (ds <= R(0)) return;   // the reference returns S unchanged (a NaN ds falls through)
+    constexpr int TAYLOR = sizeof(R) == 8 ? 14 : 8;
+    R A[4][4], b[4];
+    int p;
+    exact_operator(c, ds, A, b, p);
+    if (p > kExactSubsteps) {
+        stokes_exact_squaring(s, c, ds);
+        return;
+    }
+    for (int n = 1 << p; n > 0; --n) {
+        R v[4], t[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            v[i] = b[i] + A[i][0] * s[0] + A[i][1] * s[1] + A[i][2] * s[2] + A[i][3] * s[3];
+            t[i] = v[i];
+        }
+#pragma unroll
+        for (int k = TAYLOR + 1; k >= 2; --k) {   // t = v + A t / k
+            const R inv = R(1) / R(k);
+            R u[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] = v[i] + (A[i][0] * t[0] + A[i][1] * t[1] + A[i][2] * t[2] + A[i][3] * t[3]) * inv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) t[i] = u[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s[i] += t[i];
+    }
+}
+
 // ---- camera (kerr.py:188-233), FP64 per ray --------------------------------
 
 __device__ __forceinline__ void camera_ray(const RenderArgs &A, int px, int py, double &pr, double &pth, double &pt,
@@ -564,7 +690,7 @@ __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr
 // One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
 // MIXED: the geodesic step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
 // |p_theta| < pth_island near a theta turning point) and the horizon/escape tests compare in FP64.
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool COOP = false>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool COOP = false>
 __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                         const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
     const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
@@ -603,7 +729,8 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> 
         const R dx = nr - s.r, dy = nt - s.th, dz = np - s.ph;
         const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
         const R ds = pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
-        stokes_rk2(s.S, c, ds);
+        if constexpr (EXACT) stokes_exact(s.S, c, ds);
+        else stokes_rk2(s.S, c, ds);
         s.tau = s.tau + pymax(c[4], R(0)) * ds;
         ++n_samples;
     }
@@ -685,7 +812,7 @@ __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R>
 // A wave left with one live ray (FP32 kernels, corner layout): every lane takes a copy of it, the brick sample of
 // each step is split over the lanes (COOP), and the ray runs to its end; then its owner lane writes
 // the result.  The copies stay identical, so the wave leaves the loop together.
-template <typename R, int STORE, int LAYOUT, bool MIXED>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
 __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                          const R *gr, const R *gt, const R *gp, RayState<R> &s, int lane, int owner,
                                          unsigned &n_steps, unsigned &n_samples, unsigned long long &iters) {
@@ -694,7 +821,7 @@ __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R>
     unsigned steps = 0, samples = 0;
     int code;
     do {
-        code = ray_step<R, STORE, LAYOUT, MIXED, true>(A, P, rs, gr, gt, gp, s, samples);
+        code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, true>(A, P, rs, gr, gt, gp, s, samples);
         ++steps;
     } while (code < 0);
     if (lane == owner) {
@@ -713,7 +840,7 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
 #define BH_RENDER_WAVES_PER_EU 2
 #endif
 
-template <typename R, int STORE, int LAYOUT, bool MIXED>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void render_kernel(
     RenderArgs A) {
     __shared__ R s_grid[kMaxGrid + 1];
@@ -788,7 +915,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         }
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {   // no refill happened above: the wave serves its last ray with all lanes
-                lone_ray<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
+                lone_ray<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
                                                   n_steps, n_samples, iters);
                 active = false;
                 continue;
@@ -798,7 +925,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < A.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
@@ -843,7 +970,7 @@ struct BatchArgs {
     int queues, q_tiles, steps_per_sync;
 };
 
-template <typename R, int STORE, int LAYOUT, bool MIXED>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void batch_kernel(
     BatchArgs B) {
     __shared__ R s_grid[kMaxGrid + 8];
@@ -901,7 +1028,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const HotParams<R> P = loop_params<R>(A);
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {
-                lone_ray<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
+                lone_ray<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
                                                   n_steps, n_samples, iters);
                 active = false;
                 continue;
@@ -911,7 +1038,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < B.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE, LAYOUT, MIXED>(A, P, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
@@ -1077,6 +1204,8 @@ static int validate_scene(const bh_scene_desc &d) {
         return set_error(BH_ERR_VALUE, "fov_y_degrees must lie in (0, 180)");
     if (std::isnan(d.escape_radius) || std::isnan(d.affine_to_length_cm) || std::isnan(d.max_optical_depth))
         return set_error(BH_ERR_VALUE, "escape_radius, affine_to_length_cm and max_optical_depth must not be NaN");
+    if (d.transfer != BH_TRANSFER_RK2 && d.transfer != BH_TRANSFER_EXACT)
+        return set_error(BH_ERR_VALUE, "transfer must be BH_TRANSFER_RK2 or BH_TRANSFER_EXACT");
     return BH_OK;
 }
 
@@ -1147,50 +1276,57 @@ static uint16_t cvt_f16(float x) {
 }
 
 typedef void (*RenderFn)(RenderArgs);
-
-template <typename R, int STORE, int LAYOUT, bool MIXED>
-static RenderFn kernel_ptr() { return render_kernel<R, STORE, LAYOUT, MIXED>; }
-
-template <int LAYOUT>
-static RenderFn pick_fp32(int storage, bool mixed) {
-    switch (storage) {
-        case BH_STORE_F32: return mixed ? kernel_ptr<float, BH_STORE_F32, LAYOUT, true>() : kernel_ptr<float, BH_STORE_F32, LAYOUT, false>();
-        case BH_STORE_BF16: return mixed ? kernel_ptr<float, BH_STORE_BF16, LAYOUT, true>() : kernel_ptr<float, BH_STORE_BF16, LAYOUT, false>();
-        case BH_STORE_F16: return mixed ? kernel_ptr<float, BH_STORE_F16, LAYOUT, true>() : kernel_ptr<float, BH_STORE_F16, LAYOUT, false>();
-    }
-    return nullptr;
-}
-
-static RenderFn pick_kernel(int precision, int storage, int layout) {
-    if (precision == BH_PREC_FP64)
-        return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA) ? kernel_ptr<double, BH_STORE_F32, BH_LAYOUT_SOA, false>()
-                                                                    : nullptr;
-    const bool mixed = precision == BH_PREC_MIXED;
-    return layout == BH_LAYOUT_CORNER ? pick_fp32<BH_LAYOUT_CORNER>(storage, mixed) : pick_fp32<BH_LAYOUT_SOA>(storage, mixed);
-}
-
 typedef void (*BatchFn)(BatchArgs);
 
-template <typename R, int STORE, int LAYOUT, bool MIXED>
-static BatchFn batch_ptr() { return batch_kernel<R, STORE, LAYOUT, MIXED>; }
+// Kernel table: precision x storage x layout x transfer rule.  The exact Stokes step is a template
+// parameter, so the RK2 kernels carry none of its code or registers.
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+struct KernelPair {
+    static RenderFn render() { return render_kernel<R, STORE, LAYOUT, MIXED, EXACT>; }
+    static BatchFn batch() { return batch_kernel<R, STORE, LAYOUT, MIXED, EXACT>; }
+};
 
-template <int LAYOUT>
-static BatchFn pick_batch_fp32(int storage, bool mixed) {
+template <bool BATCH, typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+static void *kernel_of() {
+    using K = KernelPair<R, STORE, LAYOUT, MIXED, EXACT>;
+    return BATCH ? reinterpret_cast<void *>(K::batch()) : reinterpret_cast<void *>(K::render());
+}
+
+template <bool BATCH, int LAYOUT, bool EXACT>
+static void *pick_fp32(int storage, bool mixed) {
     switch (storage) {
-        case BH_STORE_F32: return mixed ? batch_ptr<float, BH_STORE_F32, LAYOUT, true>() : batch_ptr<float, BH_STORE_F32, LAYOUT, false>();
-        case BH_STORE_BF16: return mixed ? batch_ptr<float, BH_STORE_BF16, LAYOUT, true>() : batch_ptr<float, BH_STORE_BF16, LAYOUT, false>();
-        case BH_STORE_F16: return mixed ? batch_ptr<float, BH_STORE_F16, LAYOUT, true>() : batch_ptr<float, BH_STORE_F16, LAYOUT, false>();
+        case BH_STORE_F32:
+            return mixed ? kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, true, EXACT>()
+                         : kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, false, EXACT>();
+        case BH_STORE_BF16:
+            return mixed ? kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, true, EXACT>()
+                         : kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, false, EXACT>();
+        case BH_STORE_F16:
+            return mixed ? kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, true, EXACT>()
+                         : kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, false, EXACT>();
     }
     return nullptr;
 }
 
-static BatchFn pick_batch_kernel(int precision, int storage, int layout) {
+template <bool BATCH, bool EXACT>
+static void *pick_any(int precision, int storage, int layout) {
     if (precision == BH_PREC_FP64)
-        return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA) ? batch_ptr<double, BH_STORE_F32, BH_LAYOUT_SOA, false>()
-                                                                    : nullptr;
+        return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA)
+                   ? kernel_of<BATCH, double, BH_STORE_F32, BH_LAYOUT_SOA, false, EXACT>()
+                   : nullptr;
     const bool mixed = precision == BH_PREC_MIXED;
-    return layout == BH_LAYOUT_CORNER ? pick_batch_fp32<BH_LAYOUT_CORNER>(storage, mixed)
-                                      : pick_batch_fp32<BH_LAYOUT_SOA>(storage, mixed);
+    return layout == BH_LAYOUT_CORNER ? pick_fp32<BATCH, BH_LAYOUT_CORNER, EXACT>(storage, mixed)
+                                      : pick_fp32<BATCH, BH_LAYOUT_SOA, EXACT>(storage, mixed);
+}
+
+static RenderFn pick_kernel(int precision, int storage, int layout, bool exact) {
+    return reinterpret_cast<RenderFn>(exact ? pick_any<false, true>(precision, storage, layout)
+                                            : pick_any<false, false>(precision, storage, layout));
+}
+
+static BatchFn pick_batch_kernel(int precision, int storage, int layout, bool exact) {
+    return reinterpret_cast<BatchFn>(exact ? pick_any<true, true>(precision, storage, layout)
+                                           : pick_any<true, false>(precision, storage, layout));
 }
 
 static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A) {
@@ -1245,7 +1381,7 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
     if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32 && precision != BH_PREC_MIXED)
         return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
     const BrickDev &b = ctx->bricks[brick_id];
-    RenderFn fn = pick_kernel(precision, b.storage, b.layout);
+    RenderFn fn = pick_kernel(precision, b.storage, b.layout, scene->transfer == BH_TRANSFER_EXACT);
     if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick in the SoA layout");
     RenderArgs A{};
     if ((st = setup_args(*scene, b, A))) return st;
@@ -1560,8 +1696,10 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         kmap[i] = k;
         const long long tiles = (long long)((items[i].scene.width + 7) / 8) * ((items[i].scene.height + 7) / 8);
         if (tiles >= (1 << 20)) return set_error(BH_ERR_VALUE, "render %d has too many 8x8 tiles", i);
+        if (items[i].scene.transfer != items[0].scene.transfer)
+            return set_error(BH_ERR_VALUE, "all renders of a batch must share the transfer rule");
     }
-    BatchFn fn = pick_batch_kernel(precision, storage, layout);
+    BatchFn fn = pick_batch_kernel(precision, storage, layout, items[0].scene.transfer == BH_TRANSFER_EXACT);
     if (!fn) return set_error(BH_ERR_VALUE, "the FP64 batch needs float32 bricks in the SoA layout");
     // merged coefficient buffer and grid table
     std::vector<size_t> coff(uniq.size());
@@ -2039,6 +2177,8 @@ static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id
     if (!scene || !sd || !d_work || !d_iquv || !sd->out_mul || !sd->out_add)
         return set_error(BH_ERR_VALUE, "NULL surrogate argument");
     if ((st = validate_scene(*scene))) return st;
+    if (scene->transfer != BH_TRANSFER_RK2)
+        return set_error(BH_ERR_VALUE, "the surrogate's transfer over knots takes the RK2 Stokes step only");
     if (brick_id < 0 || size_t(brick_id) >= ctx->bricks.size() || !ctx->bricks[brick_id].live)
         return set_error(BH_ERR_STATE, "unknown brick id %d", brick_id);
     const int K = sd->n_knots;

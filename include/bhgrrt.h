@@ -150,7 +150,14 @@ typedef struct bh_scene_desc {
     double escape_radius;     /* AcceleratedRenderConfig.escape_radius */
     double affine_to_length_cm;
     double max_optical_depth; /* PolarizedTransferConfig.max_optical_depth */
+    int32_t transfer;         /* BH_TRANSFER_RK2 or BH_TRANSFER_EXACT
+                                 (PolarizedTransferConfig.use_exact_matrix_step,
+                                 polarized_transfer.py:51-80) */
+    int32_t reserved;         /* 0 */
 } bh_scene_desc;
+
+#define BH_TRANSFER_RK2    0   /* one RK2 step of dS/ds = j - K S per segment (stokes_step_rk2) */
+#define BH_TRANSFER_EXACT  1   /* exact constant-coefficient step: exp of [[-K, j], [0, 0]] ds (stokes_step_exact) */
 
 #define BH_PREC_FP64   0   /* full FP64 render (reference-grade) */
 #define BH_PREC_FP32   1   /* plain FP32 geodesic + transfer */

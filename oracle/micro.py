@@ -185,6 +185,33 @@ def stokes_rhs(stokes: np.ndarray, c: np.ndarray) -> np.ndarray:
     return out
 
 
+def propagation_matrix(c: np.ndarray) -> np.ndarray:
+    """synchrotron.py:82-94: K of dS/ds = j - K S for (n, 11) coefficient rows, shape (n, 4, 4)."""
+    c = np.asarray(c, dtype=np.float64)
+    ai, aq, au, av, rv, rq, ru = (c[:, k] for k in range(4, 11))
+    return np.stack([np.stack([ai, aq, au, av], -1), np.stack([aq, ai, rv, -ru], -1),
+                     np.stack([au, -rv, ai, rq], -1), np.stack([av, ru, -rq, ai], -1)], -2)
+
+
+def stokes_exact_step(stokes: np.ndarray, c: np.ndarray, ds: np.ndarray) -> np.ndarray:
+    """polarized_transfer.py:51-70 on (n, 4) states with per-row ds: expm of the augmented matrix
+    [[-K, j], [0, 0]] ds (scipy.linalg.expm, the reference's routine) applied to [S; 1]; rows with
+    ds <= 0 keep S (a NaN ds goes through, as in the reference's ``if ds_cm <= 0.0``)."""
+    from scipy.linalg import expm
+
+    S = np.asarray(stokes, dtype=np.float64)
+    ds = np.asarray(ds, dtype=np.float64)
+    n = S.shape[0]
+    A = np.zeros((n, 5, 5))
+    A[:, :4, :4] = -propagation_matrix(c)
+    A[:, :4, 4] = np.asarray(c, dtype=np.float64)[:, :4]
+    with np.errstate(invalid="ignore", over="ignore"):
+        M = expm(A * ds[:, None, None])
+    y = np.concatenate([S, np.ones((n, 1))], axis=1)
+    out = np.einsum("nij,nj->ni", M, y)[:, :4]
+    return np.where((ds <= 0.0)[:, None], S, out)
+
+
 def sample_brick_trilinear(coeffs, r_grid, theta_grid, phi_grid, points) -> np.ndarray:
     """native_kernels.py:225-244 (per-point loop)."""
     c, rg, tg, pg, p = coerce_sampler_inputs(coeffs, r_grid, theta_grid, phi_grid, points)

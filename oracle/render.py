@@ -10,7 +10,8 @@ One ray per pixel.  Each ray follows, in order:
 * after each step the transfer segment of accelerated_renderer.py:112-125:
   sample the brick at the coordinate midpoint of (previous, current) state,
   skip the segment when r or theta is outside the brick, otherwise
-  ds = L * max(|delta(r, theta, phi)|, 1e-14), one RK2 Stokes step, and
+  ds = L * max(|delta(r, theta, phi)|, 1e-14), one RK2 Stokes step (or, with
+  ``scene.exact_step``, the exact step of polarized_transfer.py:51-70), and
   tau += max(alpha_I, 0) ds;
 * termination of kerr.py:357-377: a non-finite state ends the ray (the
   reference appends that state to the trace, so its transfer segment is still
@@ -90,6 +91,7 @@ def render_rays(scene, brick, pix: np.ndarray | None = None):
     L = float(scene.affine_to_length_cm)
     tau_max = float(scene.max_optical_depth)
     r_esc = float(scene.escape_radius)
+    exact = bool(getattr(scene, "exact_step", False))
 
     S_out = np.zeros((n, 4))
     steps_out = np.zeros(n, dtype=np.int32)
@@ -126,7 +128,10 @@ def render_rays(scene, brick, pix: np.ndarray | None = None):
             dlam = np.sqrt(dr * dr + dth * dth + dph * dph)
             ds = _pymax(L * _pymax(dlam, 1.0e-14), 0.0)
             if np.any(valid):
-                S[valid] = _rk2_stokes(S[valid], c[valid], ds[valid])
+                if exact:
+                    S[valid] = micro.stokes_exact_step(S[valid], c[valid], ds[valid])
+                else:
+                    S[valid] = _rk2_stokes(S[valid], c[valid], ds[valid])
                 tau[valid] = tau[valid] + _pymax(c[valid, 4], 0.0) * ds[valid]
 
         status = np.full(idx.size, -1, dtype=np.int8)

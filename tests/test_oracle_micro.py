@@ -217,3 +217,40 @@ def test_faraday_rotation_kat():
     assert abs(S[0]) < 1e-12
     assert abs(S[2]) > 0.05
     assert abs(np.hypot(S[1], S[2]) - 1.0) < 1e-9
+
+
+def test_exact_step_closed_forms_and_rk2_limit():
+    """oracle.micro.stokes_exact_step (polarized_transfer.py:51-70) against what the mathematics fixes:
+    pure emission + absorption, pure Faraday rotation, the limit of many RK2 steps, and composition."""
+    # emission + absorption: I(s) = I0 e^{-a s} + j/a (1 - e^{-a s})
+    c = np.zeros((3, 11)); c[:, 0] = [1.0, 0.3, 2.0]; c[:, 4] = [0.5, 2.0, 1e-3]
+    S0 = np.array([[0.2, 0, 0, 0], [0.0, 0, 0, 0], [5.0, 0, 0, 0]])
+    ds = np.array([0.7, 3.0, 10.0])
+    got = micro.stokes_exact_step(S0, c, ds)
+    a, j = c[:, 4], c[:, 0]
+    want = S0[:, 0] * np.exp(-a * ds) + j / a * (1 - np.exp(-a * ds))
+    np.testing.assert_allclose(got[:, 0], want, rtol=1e-13)
+    assert np.all(got[:, 1:] == 0.0)
+    # Faraday rotation by rho_V: (Q, U) turns by rho_V s, V and I untouched
+    c = np.zeros((1, 11)); c[0, 8] = 2.0
+    s = np.array([0.3])
+    got = micro.stokes_exact_step(np.array([[1.0, 1.0, 0.0, 0.25]]), c, s)[0]
+    np.testing.assert_allclose(got, [1.0, math.cos(0.6), math.sin(0.6), 0.25], rtol=1e-13, atol=1e-15)
+    # full coefficient sets: many RK2 steps converge to one exact step at second order
+    rng = np.random.default_rng(11)
+    c = np.concatenate([rng.uniform(0.1, 1.0, 4), rng.uniform(0.2, 0.6, 1), rng.uniform(-0.1, 0.1, 3),
+                        rng.uniform(-0.8, 0.8, 3)])[None, :]
+    S0 = rng.normal(size=(1, 4))
+    exact = micro.stokes_exact_step(S0, c, np.array([1.1]))
+    errs = []
+    for n in (64, 128):
+        S = S0.copy()
+        for _ in range(n):
+            S = micro.stokes_rk2_brick(c[0], 1.1 / n, S[0])[None, :]
+        errs.append(np.max(np.abs(S - exact)))
+    assert 3.5 < errs[0] / errs[1] < 4.5
+    # composition: two half steps equal one step; ds <= 0 keeps S
+    half = micro.stokes_exact_step(micro.stokes_exact_step(S0, c, np.array([0.55])), c, np.array([0.55]))
+    np.testing.assert_allclose(half, exact, rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(micro.stokes_exact_step(S0, c, np.array([0.0])), S0)
+    np.testing.assert_array_equal(micro.stokes_exact_step(S0, c, np.array([-1.0])), S0)
