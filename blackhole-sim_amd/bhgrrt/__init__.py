@@ -5,7 +5,10 @@ kernels for gfx950.  Submodules:
 
 * ``scenes``         -- camera / scene / brick descriptions and seeded generators
 * ``native_kernels`` -- mirror of the reference's native_kernels.py API (FP64 micro-kernels)
-* ``render``         -- fused render API (FP64, FP32, mixed precision)
+* ``render``         -- fused render API (FP64, FP32, mixed precision), batches
+* ``accelerated``    -- mirror of accelerated_renderer.py: config, image and progressive renders
+* ``surrogate``      -- learned surrogate (MFMA MLP, training, knot render)
+* ``autotune``       -- launch-shape autotuner for the persistent kernels
 """
 
 __version__ = "0.9.0"
