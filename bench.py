@@ -283,6 +283,22 @@ def main(argv=None) -> int:
         log(f"[bench] mixed {storage}: {paths[f'mixed_{storage}']}")
         rr.release(dbx)
 
+    # ---- exact Stokes step (use_exact_matrix_step) on the same scene ------------------------
+    log("[bench] exact Stokes step: FP32 corner render against the FP64 exact render")
+    scene_x = dataclasses.replace(scene, exact_step=True)
+    ref_x = rr.render(scene_x, db32, PREC_FP64)
+    fp64x_ms = rr.last_kernel_ms()
+    ex = time_path(rr, scene_x, dbc, PREC_FP32, bufs, args.steps, args.warmup)
+    out_x = rr.render(scene_x, dbc, PREC_FP32)
+    okx = ((ref_x["status"] | out_x["status"]) & 8) == 0
+    paths["plain_fp32_exact_step"] = {
+        "mrays_s": n / (ex["ms_per_step"] * 1e-3) / 1e6, "ms_per_render": ex["ms_per_step"], "kernel_ms": ex["kernel_ms"],
+        "rel_l1_I": rel_l1(out_x["iquv"][..., 0][okx].astype(np.float64), ref_x["iquv"][..., 0][okx]),
+        "pixel_rel_I": pixel_rel_I(out_x["iquv"][..., 0], ref_x["iquv"][..., 0], okx),
+        "fp64_exact_kernel_ms": fp64x_ms,
+        "rk2_vs_exact_rel_l1_I": rel_l1(ref["iquv"][..., 0][okx], ref_x["iquv"][..., 0][okx])}
+    log(f"[bench] exact step: {paths['plain_fp32_exact_step']}")
+
     # ---- batched paths -----------------------------------------------------------------
     from bhgrrt.autotune import autotune
     from bhgrrt.render import Batch

@@ -5,8 +5,8 @@
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
         python3 blackhole-sim_amd/bhgrrt/profile_workload.py
 
-Render paths are storage:layout:precision; each is rendered twice (the second launch is the
-measured one).  ``--surrogate`` adds the learned surrogate: FP64 geodesic knots for
+Render paths are storage:layout:precision[:exact] (``exact`` selects the exact Stokes step); each is
+rendered twice (the second launch is the measured one).  ``--surrogate`` adds the learned surrogate: FP64 geodesic knots for
 ``--surrogate-rays`` seeded rays, ``--train-steps`` Adam steps on the BF16 MFMA kernels and two
 1024^2 surrogate renders (bf16 corner brick, mixed-precision fallback).  ``--sampler N`` adds two
 launches of the FP64 trilinear sampler on N uniform points; ``--mlp-rows N`` times the surrogate
@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import dataclasses
 import math
 import sys
 from pathlib import Path
@@ -122,10 +123,11 @@ def main(argv=None) -> int:
     n = sc.rays
     bufs = (r.alloc(n * 32), r.alloc(n * 4), r.alloc(n))
     for spec in [p for p in args.paths.split(",") if p]:
-        storage, layout, prec = spec.split(":")
+        storage, layout, prec, *rule = spec.split(":")
+        scx = dataclasses.replace(sc, exact_step=rule == ["exact"])
         db = r.upload(br, storage, layout)
         for _ in range(2):
-            r.render_device(sc, db, PREC[prec], *bufs)
+            r.render_device(scx, db, PREC[prec], *bufs)
             r.synchronize()
         steps, samples = r.last_work()
         print(f"{spec}: {r.last_kernel_ms():.3f} ms, {steps} RK2 steps, {samples} brick samples", flush=True)
