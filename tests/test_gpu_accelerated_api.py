@@ -58,3 +58,24 @@ def test_progressive_frames_equal_single_renders(renderer):
             assert np.abs(f.stokes[..., 0]).sum() > 0
     finally:
         renderer.release(db)
+
+
+def test_float16_brick_file_renders_like_the_source_brick(renderer, tmp_path):
+    """A float16 brick file in the reference's schema, loaded and uploaded as FP16, gives the same image
+    as the source brick uploaded as FP16 (both round each value to the same half)."""
+    src = scenes.scale_brick(64, 32, 64, seed=5)
+    p = tmp_path / "half.npz"
+    np.savez_compressed(p, r=src.r, theta=src.theta, phi=src.phi, coeffs=src.coeffs.astype(np.float16),
+                        coeff_names=np.array(scenes.COEFF_NAMES), nu_hz=np.array(230e9), spin_a=np.array(0.94),
+                        dtype_name=np.array("float16"))
+    back = scenes.Brick.load_npz(p)
+    sc = scenes.scale_plain_scene(40, 40)
+    a, b = renderer.upload(src, "f16", "corner"), renderer.upload(back, "f16", "corner")
+    try:
+        oa, ob = renderer.render(sc, a, PREC_FP32), renderer.render(sc, b, PREC_FP32)
+    finally:
+        renderer.release(a)
+        renderer.release(b)
+    np.testing.assert_array_equal(oa["iquv"], ob["iquv"])
+    np.testing.assert_array_equal(oa["status"], ob["status"])
+    assert np.abs(oa["iquv"][..., 0]).sum() > 0
