@@ -2,7 +2,8 @@
 
 Same names and argument meaning as the reference: ``AcceleratedRenderConfig`` (:36-46, with the
 ``PolarizedTransferConfig`` fields the renderer reads, polarized_transfer.py:23-29),
-``render_stokes_image_bricks`` (:153-158), ``render_progressive_stokes_bricks`` (:161-166) and
+``render_tile_stokes_bricks`` (:129-150), ``render_stokes_image_bricks`` (:153-158),
+``render_progressive_stokes_bricks`` (:161-166) and
 ``ProgressiveFrame`` (:49-54), plus ``progressive_levels`` and ``generate_tiles``
 (tiles.py:30-50).  The image comes from the fused HIP render instead of the per-pixel CPU loop:
 
@@ -144,6 +145,15 @@ def render_stokes_image_bricks(cfg: AcceleratedRenderConfig, snapshot, bricks: B
             rr.close()
 
 
+def render_tile_stokes_bricks(cfg: AcceleratedRenderConfig, snapshot, bricks: Brick, tile: RenderTile,
+                              precision: int = PREC_FP64, renderer: Renderer | None = None) -> np.ndarray:
+    """accelerated_renderer.py:129-150: the (tile.height, tile.width, 4) block of the image.  The GPU traces
+    the whole image in one launch and the block is cut from it, so callers that loop over tiles should
+    call ``render_stokes_image_bricks`` once instead."""
+    img = render_stokes_image_bricks(cfg, snapshot, bricks, precision, renderer)
+    return img[tile.y0:tile.y1, tile.x0:tile.x1].copy()
+
+
 def render_progressive_stokes_bricks(cfg: AcceleratedRenderConfig, snapshot, bricks: Brick,
                                      precision: int = PREC_FP64, renderer: Renderer | None = None
                                      ) -> list[ProgressiveFrame]:
@@ -169,5 +179,6 @@ def render_progressive_stokes_bricks(cfg: AcceleratedRenderConfig, snapshot, bri
 
 
 __all__ = ["AcceleratedRenderConfig", "PolarizedTransferConfig", "ProgressiveFrame", "RenderTile", "generate_tiles",
-           "progressive_levels", "scene_from_config", "render_stokes_image_bricks", "render_progressive_stokes_bricks",
+           "progressive_levels", "scene_from_config", "render_stokes_image_bricks", "render_tile_stokes_bricks",
+           "render_progressive_stokes_bricks",
            "PREC_FP32", "PREC_FP64", "PREC_MIXED"]

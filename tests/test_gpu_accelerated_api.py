@@ -79,3 +79,13 @@ def test_float16_brick_file_renders_like_the_source_brick(renderer, tmp_path):
     np.testing.assert_array_equal(oa["iquv"], ob["iquv"])
     np.testing.assert_array_equal(oa["status"], ob["status"])
     assert np.abs(oa["iquv"][..., 0]).sum() > 0
+
+
+def test_tile_render_is_the_block_of_the_image(renderer):
+    cfg, br = _c0_config(False), scenes.small_brick()
+    img = acc.render_stokes_image_bricks(cfg, 0.9, br, PREC_FP64, renderer=renderer)
+    tiles = acc.generate_tiles(cfg.width, cfg.height, tile_size=20)
+    assert len(tiles) == 2
+    for t in tiles:
+        block = acc.render_tile_stokes_bricks(cfg, 0.9, br, t, PREC_FP64, renderer=renderer)
+        np.testing.assert_array_equal(block, img[t.y0:t.y1, t.x0:t.x1])
