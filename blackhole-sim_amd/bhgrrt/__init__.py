@@ -9,6 +9,8 @@ kernels for gfx950.  Submodules:
 * ``accelerated``    -- mirror of accelerated_renderer.py: config, image and progressive renders
 * ``surrogate``      -- learned surrogate (MFMA MLP, training, knot render)
 * ``autotune``       -- launch-shape autotuner for the persistent kernels
+* ``grmhd``          -- GRMHD snapshot fields and the analytic torus fixture
+* ``coefficient_bricks`` -- coefficient-brick precompute (synchrotron model on the GPU)
 """
 
 __version__ = "0.9.0"

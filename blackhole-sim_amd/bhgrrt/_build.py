@@ -20,7 +20,7 @@ BUILD = ROOT / "_build"
 LIB = PKG_DIR / "libbhgrrt.so"
 INCLUDE = ROOT.parent / "include"
 
-SOURCES = ["bh_common.hip", "bh_micro.hip", "bh_render.hip", "bh_mlp.hip"]
+SOURCES = ["bh_common.hip", "bh_micro.hip", "bh_render.hip", "bh_mlp.hip", "bh_synchro.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

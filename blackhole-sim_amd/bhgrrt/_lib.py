@@ -67,6 +67,20 @@ class SurrogateDesc(C.Structure):
                 ("flag_max", C.c_float), ("cap_lo", C.c_float), ("cap_hi", C.c_float)]
 
 
+class SynchrotronDesc(C.Structure):
+    """bh_synchrotron_desc."""
+    _fields_ = [("nu_hz", C.c_double), ("rho_cgs_per_code", C.c_double), ("b_gauss_per_code", C.c_double),
+                ("mean_molecular_weight_per_electron", C.c_double),
+                ("th_gamma_grid_size", C.c_int32), ("th_pitch_nodes", C.c_int32),
+                ("th_faraday_rotation_scale", C.c_double), ("th_faraday_conversion_scale", C.c_double),
+                ("nt_p", C.c_double), ("nt_gamma_min", C.c_double), ("nt_gamma_max", C.c_double),
+                ("nt_fraction", C.c_double), ("nt_gamma_grid_size", C.c_int32), ("nt_pitch_nodes", C.c_int32),
+                ("nt_absorption_scale", C.c_double), ("nt_faraday_scale", C.c_double)]
+
+
+# brick value types of the coefficient precompute (BH_STORE_F32, BH_STORE_F16, BH_STORE_F64)
+BRICK_DTYPE = {"float32": 0, "float16": 2, "float64": 3}
+
 # (name, restype, argtypes) for every symbol include/bhgrrt.h declares
 SIGNATURES = [
     ("bh_core_version", C.c_char_p, []),
@@ -129,6 +143,13 @@ SIGNATURES = [
       C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("bh_surrogate_last_fallback", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("bh_surrogate_work_bytes", C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]),
+    ("bh_precompute_coefficient_bricks", C.c_int,
+     [_dp, _dp, _dp, C.c_int64, C.POINTER(SynchrotronDesc), C.c_int32, C.c_void_p]),
+    ("bh_precompute_coefficient_bricks_device", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(SynchrotronDesc), C.c_int32,
+      C.c_void_p]),
+    ("bh_bessel_k", C.c_int, [C.c_double, C.c_double, _dp]),
+    ("bh_gauss_legendre", C.c_int, [C.c_int32, _dp, _dp]),
 ]
 
 _lock = threading.Lock()

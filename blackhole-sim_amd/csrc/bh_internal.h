@@ -81,6 +81,9 @@ struct bh_ctx {
     size_t order_cap = 0;
     std::vector<double> order_key;      // scene parameters the schedule was built for
     unsigned *sur_count = nullptr;      // fallback counter of the last surrogate render (in its work buffer)
+    void *d_synchro = nullptr;          // coefficient-precompute tables (grow-only)
+    size_t synchro_cap = 0;
+    std::vector<double> h_synchro;      // host copy the table upload reads from
 };
 
 struct bh_mlp;

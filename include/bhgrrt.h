@@ -307,6 +307,52 @@ int bh_surrogate_last_fallback(bh_ctx *ctx, int64_t *n_fallback);
 /* Bytes of d_work bh_surrogate_render_device needs for `rows` pixels. */
 int bh_surrogate_work_bytes(int64_t rows, int32_t n_knots, size_t *bytes);
 
+/* ---- coefficient-brick precompute (precompute_coefficient_bricks,
+ * coefficient_bricks.py:118-152, with HybridSynchrotronCoefficients,
+ * synchrotron.py:231-325) -------------------------------------------------------
+ * Cell frames as coefficient_bricks.py:110-116: n_e = rho rho_cgs_per_code /
+ * max(mu_e m_p, 1e-300), B = |b_con[1:4]| b_gauss_per_code, EVPA =
+ * atan2(b_con[3], b_con[2]), theta_e floored at 1e-8, pitch angle to the line of
+ * sight 90 deg.  Thermal (Maxwell-Juttner) plus power-law emission over the
+ * reference's Lorentz-factor grids and Gauss-Legendre pitch nodes, Kirchhoff /
+ * power-law absorption, Faraday terms, EVPA rotation; then the cast to the brick
+ * dtype and nan_to_num (NaN -> 0, +inf -> the dtype's max, -inf -> 0). */
+#define BH_STORE_F64   3   /* float64 brick values (precompute output only) */
+
+typedef struct bh_synchrotron_desc {
+    double nu_hz;                                  /* observing frequency */
+    double rho_cgs_per_code, b_gauss_per_code;     /* PhysicalScaling (calibration.py:28-56) */
+    double mean_molecular_weight_per_electron;
+    /* ThermalSynchrotronCoefficients (synchrotron.py:231-238) */
+    int32_t th_gamma_grid_size, th_pitch_nodes;    /* 192, 16 */
+    double th_faraday_rotation_scale, th_faraday_conversion_scale;   /* 1, 1 */
+    /* NonthermalPowerLawSynchrotronCoefficients (synchrotron.py:271-282) */
+    double nt_p, nt_gamma_min, nt_gamma_max, nt_fraction;            /* 3.5, 10, 1e5, 0.02 */
+    int32_t nt_gamma_grid_size, nt_pitch_nodes;                      /* 224, 16 */
+    double nt_absorption_scale, nt_faraday_scale;                    /* 1, 0.15 */
+} bh_synchrotron_desc;
+
+/* Replaces precompute_coefficient_bricks (coefficient_bricks.py:118-152) over
+ * n_cells cells given as flat field arrays (the caller takes the stride):
+ * rho[n], theta_e[n], b_con[n][4] -> out[n][11] in out_dtype (BH_STORE_F32,
+ * BH_STORE_F16 or BH_STORE_F64), channels in COEFF_NAMES order.  Grid sizes
+ * 2..1024, pitch nodes 1..16.  Host buffers, synchronous. */
+int bh_precompute_coefficient_bricks(const double *rho, const double *theta_e, const double *b_con,
+                                     int64_t n_cells, const bh_synchrotron_desc *desc, int32_t out_dtype,
+                                     void *out);
+/* The same with device buffers on the context stream (kernel time in bh_ctx_last_kernel_ms). */
+int bh_precompute_coefficient_bricks_device(bh_ctx *ctx, const double *d_rho, const double *d_theta_e,
+                                            const double *d_b_con, int64_t n_cells,
+                                            const bh_synchrotron_desc *desc, int32_t out_dtype, void *d_out);
+/* K_nu(x), the modified Bessel function of the second kind, as scipy.special.kv
+ * (synchrotron.py:148, 195), for nu >= 0 an integer or at least 1e-3 from one
+ * and x > 0 (Temme's series for x <= 2, Steed's continued fraction above).
+ * Host only. */
+int bh_bessel_k(double nu, double x, double *out);
+/* Gauss-Legendre nodes (ascending) and weights on [-1, 1], n in [1, 64], as
+ * numpy.polynomial.legendre.leggauss (synchrotron.py:219).  Host only. */
+int bh_gauss_legendre(int32_t n, double *x, double *w);
+
 #ifdef __cplusplus
 }
 #endif
