@@ -157,6 +157,68 @@ def sampler_path(rr, brick, n: int, reps: int) -> dict:
             "hbm_frac": gbs / PEAK_HBM_GBS, "bytes_per_point": 112, "max_rel_err_vs_oracle": err}
 
 
+def precompute_path(rr, dims, reps: int, cpu_cells: int) -> dict:
+    """Coefficient-brick precompute (precompute_coefficient_bricks) on the analytic torus: device-resident
+    fields, float32 brick out; 1024 sampled cells checked against the FP64 oracle from a float64 launch,
+    and the oracle's own rate on ``cpu_cells`` cells as the CPU figure."""
+    import ctypes as C
+    from bhgrrt import _lib
+    from bhgrrt import coefficient_bricks as cb
+    from bhgrrt.grmhd import generate_analytic_grmhd_torus
+    from oracle import synchro as osy
+
+    snap = generate_analytic_grmhd_torus(spin_a=0.85, nr=dims[0], ntheta=dims[1], nphi=dims[2])
+    sc = cb.PhysicalScaling.from_mdot(cb.BlackHoleSystem.m87_star(0.85), 1e-4)
+    n = snap.rho.size
+    fields = [np.ascontiguousarray(a) for a in (snap.rho, snap.theta_e, snap.b_con)]
+    bufs = [rr.alloc(a.nbytes) for a in fields]
+    for b, a in zip(bufs, fields):
+        b.from_host(a)
+    d_out = rr.alloc(n * 11 * 8)
+    desc = cb.synchrotron_desc(sc, 230e9)
+
+    def run(dtype):
+        _lib.check(rr._lib.bh_precompute_coefficient_bricks_device(rr.ctx, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                                                   C.c_int64(n), C.byref(desc), _lib.BRICK_DTYPE[dtype],
+                                                                   d_out.ptr))
+
+    run("float64")
+    rr.synchronize()
+    out64 = d_out.to_host(np.empty((n, 11)))
+    run("float32")
+    rr.synchronize()
+    kms = []
+    for _ in range(max(1, reps)):
+        run("float32")
+        kms.append(rr.last_kernel_ms())
+    rr.synchronize()
+    for b in bufs + [d_out]:
+        b.free()
+    rng = np.random.default_rng(11)
+    pick = rng.choice(n, 1024, replace=False)
+    take = lambda a: a.reshape(n, *a.shape[3:])[pick].reshape(1024, 1, 1, *a.shape[3:])
+    ref = osy.precompute_coefficient_bricks(take(snap.rho), take(snap.theta_e), take(snap.b_con), sc.rho_cgs_per_code,
+                                            sc.b_gauss_per_code, 230e9, "float64").reshape(1024, 11)
+    nz = ref != 0
+    err = float(np.max(np.abs(out64[pick][nz] - ref[nz]) / np.abs(ref[nz])))
+    cpu = {}
+    if cpu_cells > 0:
+        sub = rng.choice(n, cpu_cells, replace=False)
+        takes = lambda a: a.reshape(n, *a.shape[3:])[sub].reshape(cpu_cells, 1, 1, *a.shape[3:])
+        t0 = time.perf_counter()
+        osy.precompute_coefficient_bricks(takes(snap.rho), takes(snap.theta_e), takes(snap.b_con), sc.rho_cgs_per_code,
+                                          sc.b_gauss_per_code, 230e9, "float32")
+        dt = time.perf_counter() - t0
+        cpu = {"value": cpu_cells / dt / 1e6, "unit": "Mcells/s", "cores": 1, "kind": "port",
+               "sample": f"{cpu_cells} seeded cells of the same torus through the NumPy oracle"}
+    ms = float(np.mean(kms))
+    # per cell: 8 distinct pitch nodes x (192 thermal + 224 power-law Lorentz factors) table evaluations
+    evals = 8 * (192 + 224)
+    return {"grid": list(dims), "cells": n, "kernel_ms": ms, "mcells_s": n / (ms * 1e-3) / 1e6,
+            "gevals_s": n * evals / (ms * 1e-3) / 1e9, "evaluations_per_cell": evals, "dtype_out": "float32",
+            "max_rel_err_vs_oracle": err, "cpu_oracle": cpu}
+
+
 def surrogate_path(rr, scene, brick, ref, I64, args) -> dict:
     """Train the MLP surrogate on seeded RK2 geodesics, then time surrogate renders of the scene
     (bf16 corner brick, mixed-precision RK2 fallback) and measure their error against FP64."""
@@ -217,6 +279,8 @@ def main(argv=None) -> int:
     ap.add_argument("--surrogate-seconds", type=float, default=90.0, help="training time limit")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sampler-points", type=int, default=1 << 24)
+    ap.add_argument("--precompute-grid", default="128x64x128", help="analytic torus nr x ntheta x nphi for the "
+                    "coefficient precompute")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -374,6 +438,12 @@ def main(argv=None) -> int:
     log(f"[bench] FP64 sampler, {args.sampler_points} points")
     paths["sampler_fp64"] = sampler_path(rr, brick, args.sampler_points, args.steps)
     log(f"[bench] sampler: {paths['sampler_fp64']}")
+
+    # ---- coefficient-brick precompute (the input side of the path) ------------------------
+    dims = tuple(int(v) for v in args.precompute_grid.split("x"))
+    log(f"[bench] coefficient precompute, torus {args.precompute_grid}")
+    paths["coefficient_precompute"] = precompute_path(rr, dims, args.steps, 0 if args.no_cpu else 64)
+    log(f"[bench] precompute: {paths['coefficient_precompute']}")
 
     # ---- learned surrogate + mixed precision --------------------------------------------
     if not args.no_surrogate:

@@ -10,7 +10,8 @@ rendered twice (the second launch is the measured one).  ``--surrogate`` adds th
 ``--surrogate-rays`` seeded rays, ``--train-steps`` Adam steps on the BF16 MFMA kernels and two
 1024^2 surrogate renders (bf16 corner brick, mixed-precision fallback).  ``--sampler N`` adds two
 launches of the FP64 trilinear sampler on N uniform points; ``--mlp-rows N`` times the surrogate
-network's inference on N rows (fused kernel, then per-layer GEMMs).  A 256 MiB device-to-device copy comes
+network's inference on N rows (fused kernel, then per-layer GEMMs); ``--precompute NRxNTxNP`` runs the
+coefficient-brick precompute on the analytic torus.  A 256 MiB device-to-device copy comes
 last, so the FETCH_SIZE / WRITE_SIZE counters can be scaled by what they read for a copy of known
 size.
 """
@@ -55,6 +56,28 @@ def run_sampler(r: Renderer, brick, n: int) -> None:
                                                            C.c_int64(n), out.ptr))
         r.synchronize()
     print(f"sampler: {n} points, {r.last_kernel_ms():.3f} ms", flush=True)
+    for b in bufs + [out]:
+        b.free()
+
+
+def run_precompute(r: Renderer, dims: tuple[int, int, int]) -> None:
+    from bhgrrt import coefficient_bricks as cb
+    from bhgrrt.grmhd import generate_analytic_grmhd_torus
+
+    snap = generate_analytic_grmhd_torus(spin_a=0.85, nr=dims[0], ntheta=dims[1], nphi=dims[2])
+    sc = cb.PhysicalScaling.from_mdot(cb.BlackHoleSystem.m87_star(0.85), 1e-4)
+    fields = [np.ascontiguousarray(a) for a in (snap.rho, snap.theta_e, snap.b_con)]
+    bufs = [r.alloc(a.nbytes) for a in fields]
+    for b, a in zip(bufs, fields):
+        b.from_host(a)
+    n = snap.rho.size
+    out = r.alloc(n * 11 * 4)
+    desc = cb.synchrotron_desc(sc, 230e9)
+    for _ in range(2):
+        _lib.check(r._lib.bh_precompute_coefficient_bricks_device(r.ctx, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                                                                  C.c_int64(n), C.byref(desc), 0, out.ptr))
+        r.synchronize()
+    print(f"coefficient precompute: {n} cells, {r.last_kernel_ms():.3f} ms", flush=True)
     for b in bufs + [out]:
         b.free()
 
@@ -117,6 +140,8 @@ def main(argv=None) -> int:
                     "network on this many rows, fused kernel and per-layer GEMMs (0: skip)")
     ap.add_argument("--latency-sizes", default="", help="e.g. 16,64,256,1024: plain FP32 renders of the scene at "
                     "these sizes, printing kernel time, longest ray and time per step of the longest ray")
+    ap.add_argument("--precompute", default="", help="e.g. 128x64x128: two launches of the coefficient-brick "
+                    "precompute on the analytic torus of that grid (float32 out)")
     args = ap.parse_args(argv)
     sc, br = scenes.scale_plain_scene(args.size, args.size), scenes.scale_brick()
     r = Renderer(0)
@@ -148,6 +173,8 @@ def main(argv=None) -> int:
                   f"{1e3 * t / longest:.2f} us per step of the longest ray, {int(out['steps'].sum())} steps in all",
                   flush=True)
         r.release(db)
+    if args.precompute:
+        run_precompute(r, tuple(int(v) for v in args.precompute.split("x")))
     if args.mlp_rows:
         run_mlp(r, args.mlp_rows)
     if args.sampler:
