@@ -169,12 +169,13 @@ static void gauss_legendre(int n, double *x, double *w) {
 
 // ---- device tables -------------------------------------------------------------------------
 // One buffer of doubles, staged into LDS by every block:
-//   [0, 256) log x    [256, 512) log F    [512, 768) log G    [768, 1024) slope F    [1024, 1280) slope G
-//   [1280, 1296) thermal sin(alpha) of the unique nodes, [1296, 1312) thermal 0.5 w
-//   [1312, 1328) power-law sin(alpha),                   [1328, 1344) power-law 0.5 w
-//   [1344, 1344 + 3 G) power-law gamma, log gamma, normalised distribution (G = grid size)
-constexpr int kOffLx = 0, kOffLF = 256, kOffLG = 512, kOffSF = 768, kOffSG = 1024;
-constexpr int kOffThSin = 1280, kOffThHw = 1296, kOffNtSin = 1312, kOffNtHw = 1328, kOffNtGamma = 1344;
+//   [0, 1530)     255 knot segments of 6: log x_i, log x_{i+1}, log F_i, slope F_i, log G_i, slope G_i
+//   [1536, 1552)  thermal sin(alpha) of the distinct pitch nodes,  [1552, 1568) thermal 0.5 w
+//   [1568, 1584)  power-law sin(alpha),                            [1584, 1600) power-law 0.5 w
+//   [1600, 1600 + 2 G)  power-law log gamma, then w_k f_k (trapezoid weight times the normalised
+//                       distribution; G = grid size)
+constexpr int kSeg = 6;
+constexpr int kOffThSin = 1536, kOffThHw = 1552, kOffNtSin = 1568, kOffNtHw = 1584, kOffNtGamma = 1600;
 
 struct SynArgs {
     const double *rho, *theta_e, *b_con;
@@ -190,6 +191,7 @@ struct SynArgs {
     double nt_frac, nt_gmin, nt_abs, nt_pol, nt_fs, nt_log_ratio, nt_gmin2;
     double k_pref, k_mc2, k_nub, k_v, k_q, four_pi;
     double lx_first, lx_last, inv_dlx;
+    double lF_top, lG_top;                          // log F and log G at the last knot
 };
 
 template <typename T>
@@ -198,43 +200,44 @@ template <typename T>
 __device__ __forceinline__ T pymin(T a, T b) { return (b < a) ? b : a; }   // Python's min(a, b)
 
 // exp(np.interp(lx, log x, log F)) and the same for G, for lx already clipped to the knot range
-// (synchrotron.py:166-174).
+// (synchrotron.py:166-174).  The knots are evenly spaced in log x up to rounding, so the scaled
+// offset names the segment except within rounding of a knot, where a short search fixes it.
 __device__ __forceinline__ void kernel_interp(const double *t, const SynArgs &A, double lx, double &F, double &G) {
 #pragma clang fp contract(off)
-    if (lx >= A.lx_last) {   // x == xp[-1]: fp[-1]
-        F = exp(t[kOffLF + kTable - 1]);
-        G = exp(t[kOffLG + kTable - 1]);
-        return;
-    }
+    const bool top = lx >= A.lx_last;   // x == xp[-1]: fp[-1]
     int j = int((lx - A.lx_first) * A.inv_dlx);
     j = j < 0 ? 0 : (j > kTable - 2 ? kTable - 2 : j);
-    while (j > 0 && lx < t[kOffLx + j]) --j;
-    while (j < kTable - 2 && lx >= t[kOffLx + j + 1]) ++j;
-    const double dx = lx - t[kOffLx + j];
-    F = exp(t[kOffSF + j] * dx + t[kOffLF + j]);
-    G = exp(t[kOffSG + j] * dx + t[kOffLG + j]);
+    const double *e = t + kSeg * j;
+    if (!top && (lx < e[0] || lx >= e[1])) {
+        while (j > 0 && lx < t[kSeg * j]) --j;
+        while (j < kTable - 2 && lx >= t[kSeg * j + 1]) ++j;
+        e = t + kSeg * j;
+    }
+    const double dx = lx - e[0];
+    F = exp(top ? A.lF_top : e[3] * dx + e[2]);
+    G = exp(top ? A.lG_top : e[5] * dx + e[4]);
 }
 
 // The pitch-angle integral of synchrotron.py:210-228 for one cell and one electron population:
-// sum over nodes of 0.5 w * trapz(f * pref * F(x)), and the same with G.  DIST(k, gamma, log gamma, f).
+// sum over nodes of 0.5 w * trapz(f * pref * F(x)), and the same with G.  The trapezoid sums are
+// written as sum_k w_k y_k (w_k the trapezoid weight of gamma_k); DIST(k, log gamma_k, w_k f_k)
+// is called for k = 0, 1, ... in order.
 template <typename Dist>
 __device__ __forceinline__ void emissivities(const double *t, const SynArgs &A, int sin_off, int hw_off, int nodes,
-                                             int ng, double ne, double b, const Dist &dist, double &j_i, double &j_q) {
+                                             int ng, double ne, double b, Dist &dist, double &j_i, double &j_q) {
     const int nu_n = (nodes + 1) / 2;
     const double nu_b = (kE * b) / A.k_nub;
-    double cu[kMaxUnique], pref[kMaxUnique], ti[kMaxUnique], tq[kMaxUnique], yi_prev[kMaxUnique], yq_prev[kMaxUnique];
+    double cu[kMaxUnique], pref[kMaxUnique], ti[kMaxUnique], tq[kMaxUnique];
 #pragma unroll
     for (int u = 0; u < kMaxUnique; ++u) {
         const double s = t[sin_off + u];
         cu[u] = A.log_nu - log((1.5 * nu_b) * s);
         pref[u] = (A.k_pref * (b * s)) / A.k_mc2;
-        ti[u] = tq[u] = yi_prev[u] = yq_prev[u] = 0.0;
+        ti[u] = tq[u] = 0.0;
     }
-    double g_prev = 0.0;
     for (int k = 0; k < ng; ++k) {
-        double g, lg, f;
-        dist(k, g, lg, f);
-        const double d = g - g_prev;
+        double lg, wf;
+        dist(k, lg, wf);
 #pragma unroll
         for (int u = 0; u < kMaxUnique; ++u) {
             if (u < nu_n) {
@@ -242,16 +245,10 @@ __device__ __forceinline__ void emissivities(const double *t, const SynArgs &A, 
                 lx = lx < A.lx_first ? A.lx_first : (lx > A.lx_last ? A.lx_last : lx);
                 double F, G;
                 kernel_interp(t, A, lx, F, G);
-                const double yi = f * (pref[u] * F), yq = f * (pref[u] * G);
-                if (k > 0) {
-                    ti[u] += d * (yi + yi_prev[u]) / 2.0;
-                    tq[u] += d * (yq + yq_prev[u]) / 2.0;
-                }
-                yi_prev[u] = yi;
-                yq_prev[u] = yq;
+                ti[u] += wf * (pref[u] * F);
+                tq[u] += wf * (pref[u] * G);
             }
         }
-        g_prev = g;
     }
     // node order of the reference: the negative-mu half ascending, then its mirror
     double tot_i = 0.0, tot_q = 0.0;
@@ -307,7 +304,9 @@ __device__ __forceinline__ _Float16 to_half(double v) {
     return _Float16(f);
 }
 
-__global__ __launch_bounds__(256) void synchro_kernel(SynArgs A) {
+// three waves per SIMD (<= 168 VGPRs; a few values spill outside the evaluation loop) hide the LDS
+// and exp latencies better than two
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void synchro_kernel(SynArgs A) {
     extern __shared__ double lds[];
     for (int i = threadIdx.x; i < A.table_words; i += blockDim.x) lds[i] = A.tables[i];
     __syncthreads();
@@ -336,33 +335,42 @@ __global__ __launch_bounds__(256) void synchro_kernel(SynArgs A) {
         const double tq = pymax(th, 1.0e-12);
         const double norm = th * bessel_k(0.0, 2, kIntegerOrder, 1.0 / tq);
         const double norm_den = pymax(norm, 1.0e-300);
-        auto gamma_at = [&](int k) {
+        auto gamma_at = [&](int k) {   // 1 + geomspace(1e-5, gmax - 1, ng)[k], as 10^y = exp(y ln 10)
 #pragma clang fp contract(off)
             const double y = double(k) * step + l0;
-            const double gs = k == 0 ? 1.0e-5 : (k == ng - 1 ? gmax - 1.0 : pow(10.0, y));
+            const double gs = k == 0 ? 1.0e-5 : (k == ng - 1 ? gmax - 1.0 : exp(y * 2.302585092994045684));
             return 1.0 + gs;
         };
+        const double neg_inv_tq = -1.0 / tq, inv_norm = 1.0 / norm_den;
         auto mj = [&](double g) {   // unnormalised Maxwell-Juttner density (synchrotron.py:192-196)
             const double gm = g > 1.0 ? g : 1.0;
             const double beta = sqrt(fmax(1.0 - 1.0 / (gm * gm), 0.0));
-            return g * g * beta * exp(-g / tq) / norm_den;
+            return g * g * beta * exp(g * neg_inv_tq) * inv_norm;
         };
-        double integ = 0.0, g_prev = 0.0, f_prev = 0.0;
-        for (int k = 0; k < ng; ++k) {
-            const double g = gamma_at(k), f = mj(g);
-            if (k > 0) integ += (g - g_prev) * (f + f_prev) / 2.0;
-            g_prev = g;
-            f_prev = f;
+        // trapezoid weight of gamma_k: (gamma_{k+1} - gamma_{k-1}) / 2, one-sided at the ends
+        double integ = 0.0;
+        {
+            double gp = 0.0, g = gamma_at(0);
+            for (int k = 0; k < ng; ++k) {
+                const double gn = k + 1 < ng ? gamma_at(k + 1) : g;
+                const double w = ((k > 0 ? g - gp : 0.0) + (k + 1 < ng ? gn - g : 0.0)) / 2.0;
+                integ += w * mj(g);
+                gp = g;
+                g = gn;
+            }
         }
-        const double integ_den = pymax(integ, 1.0e-300);
+        const double inv_integ = 1.0 / pymax(integ, 1.0e-300);
+        double tg_prev = 0.0, tg = gamma_at(0);
+        auto thermal_dist = [&](int k, double &lg, double &wf) {
+            const double gn = k + 1 < ng ? gamma_at(k + 1) : tg;
+            const double w = ((k > 0 ? tg - tg_prev : 0.0) + (k + 1 < ng ? gn - tg : 0.0)) / 2.0;
+            lg = log(tg);
+            wf = w * (mj(tg) * inv_integ);
+            tg_prev = tg;
+            tg = gn;
+        };
         double j_i, j_q;
-        emissivities(t, A, kOffThSin, kOffThHw, A.th_nodes, ng, ne, b,
-                     [&](int k, double &g, double &lg, double &f) {
-                         g = gamma_at(k);
-                         lg = log(g);
-                         f = mj(g) / integ_den;
-                     },
-                     j_i, j_q);
+        emissivities(t, A, kOffThSin, kOffThHw, A.th_nodes, ng, ne, b, thermal_dist, j_i, j_q);
         const double bnu = planck_nu(A, th);
         const double a_i = j_i / pymax(bnu, 1.0e-300), a_q = j_q / pymax(bnu, 1.0e-300);
         const double rel = pymax(1.0 / (th * th + 1.0), 1.0e-8);
@@ -381,14 +389,11 @@ __global__ __launch_bounds__(256) void synchro_kernel(SynArgs A) {
             const double sne = ne * A.nt_frac;
             const int go = kOffNtGamma, gn = A.nt_ng;
             double p_i = 0.0, p_q = 0.0;
-            if (sne > 0.0)
-                emissivities(t, A, kOffNtSin, kOffNtHw, A.nt_nodes, gn, sne, b,
-                             [&](int k, double &g, double &lg, double &f) {
-                                 g = t[go + k];
-                                 lg = t[go + gn + k];
-                                 f = t[go + 2 * gn + k];
-                             },
-                             p_i, p_q);
+            auto powerlaw_dist = [&](int k, double &lg, double &wf) {
+                lg = t[go + k];
+                wf = t[go + gn + k];
+            };
+            if (sne > 0.0) emissivities(t, A, kOffNtSin, kOffNtHw, A.nt_nodes, gn, sne, b, powerlaw_dist, p_i, p_q);
             const double cg = pymax(A.nt_gmin, sqrt(pymax(A.nu / pymax(4.2e6 * b_perp, 1.0e-300), 1.0)));
             const double energy = cg * kMe * kC * kC;
             const double pa_i = A.nt_abs * p_i / pymax(energy * A.nu * A.nu / (kC * kC), 1.0e-300);
@@ -450,12 +455,13 @@ static int prepare(const bh_synchrotron_desc *d, int out_dtype, std::vector<doub
         return set_error(BH_ERR_VALUE, "gamma_min and gamma_max must be positive");
 
     const int ng = d->nt_gamma_grid_size;
-    tab.assign(size_t(kOffNtGamma + 3 * ng), 0.0);
+    tab.assign(size_t(kOffNtGamma + 2 * ng), 0.0);
     // kernel table (synchrotron.py:158-163): F by the closed form, G = x K_{2/3}(x)
     const double mu23 = 2.0 / 3.0 - 1.0;
     const GammaTerms g23 = gamma_terms(mu23);
     const double la = std::log10(1.0e-8), lb = std::log10(1.0e3);
     const double lstep = (lb - la) / double(kTable - 1);
+    double lxs[kTable], lfs[kTable], lgs[kTable];
     for (int i = 0; i < kTable; ++i) {
         double x = i == 0 ? 1.0e-8 : (i == kTable - 1 ? 1.0e3 : std::pow(10.0, double(i) * lstep + la));
         const double xx = std::fmin(std::fmax(x, 1.0e-12), 1.0e3);
@@ -463,14 +469,19 @@ static int prepare(const bh_synchrotron_desc *d, int out_dtype, std::vector<doub
         double G = xx * bessel_k(mu23, 1, g23, xx);
         if (!std::isfinite(F)) F = 0.0;
         if (!std::isfinite(G)) G = 0.0;
-        tab[kOffLx + i] = std::log(x);
-        tab[kOffLF + i] = std::log(std::fmax(F, 1.0e-300));
-        tab[kOffLG + i] = std::log(std::fmax(G, 1.0e-300));
+        lxs[i] = std::log(x);
+        lfs[i] = std::log(std::fmax(F, 1.0e-300));
+        lgs[i] = std::log(std::fmax(G, 1.0e-300));
     }
     for (int i = 0; i + 1 < kTable; ++i) {
-        const double dx = tab[kOffLx + i + 1] - tab[kOffLx + i];
-        tab[kOffSF + i] = (tab[kOffLF + i + 1] - tab[kOffLF + i]) / dx;
-        tab[kOffSG + i] = (tab[kOffLG + i + 1] - tab[kOffLG + i]) / dx;
+        const double dx = lxs[i + 1] - lxs[i];
+        double *e = tab.data() + kSeg * i;
+        e[0] = lxs[i];
+        e[1] = lxs[i + 1];
+        e[2] = lfs[i];
+        e[3] = (lfs[i + 1] - lfs[i]) / dx;
+        e[4] = lgs[i];
+        e[5] = (lgs[i + 1] - lgs[i]) / dx;
     }
     // Gauss-Legendre pitch nodes (synchrotron.py:219): sin(alpha) and 0.5 w of the negative-mu half
     double xs[kMaxNodes], ws[kMaxNodes];
@@ -484,18 +495,23 @@ static int prepare(const bh_synchrotron_desc *d, int out_dtype, std::vector<doub
         tab[kOffNtSin + u] = std::sqrt(std::fmax(0.0, 1.0 - xs[u] * xs[u]));
         tab[kOffNtHw + u] = 0.5 * ws[u];
     }
-    // power-law grid and its normalised distribution (synchrotron.py:202-207, 290-291)
-    double *gam = tab.data() + kOffNtGamma, *lgam = gam + ng, *fpl = gam + 2 * ng;
+    // power-law grid and its normalised distribution (synchrotron.py:202-207, 290-291), stored as
+    // log gamma_k and w_k f_k with w_k the trapezoid weight of gamma_k
+    std::vector<double> gam(ng), fpl(ng);
     const double ga = std::log10(d->nt_gamma_min), gb = std::log10(d->nt_gamma_max);
     const double gstep = (gb - ga) / double(ng - 1);
     for (int k = 0; k < ng; ++k) {
         gam[k] = k == 0 ? d->nt_gamma_min : (k == ng - 1 ? d->nt_gamma_max : std::pow(10.0, double(k) * gstep + ga));
-        lgam[k] = std::log(gam[k]);
         fpl[k] = (gam[k] >= d->nt_gamma_min && gam[k] <= d->nt_gamma_max) ? std::pow(gam[k], -d->nt_p) : 0.0;
     }
     double integ = 0.0;
     for (int k = 1; k < ng; ++k) integ += (gam[k] - gam[k - 1]) * (fpl[k] + fpl[k - 1]) / 2.0;
-    for (int k = 0; k < ng; ++k) fpl[k] /= std::fmax(integ, 1.0e-300);
+    double *lgam = tab.data() + kOffNtGamma, *wf = lgam + ng;
+    for (int k = 0; k < ng; ++k) {
+        const double w = ((k > 0 ? gam[k] - gam[k - 1] : 0.0) + (k + 1 < ng ? gam[k + 1] - gam[k] : 0.0)) / 2.0;
+        lgam[k] = std::log(gam[k]);
+        wf[k] = w * (fpl[k] / std::fmax(integ, 1.0e-300));
+    }
 
     std::memset(&A, 0, sizeof(A));
     A.out_dtype = out_dtype;
@@ -526,8 +542,10 @@ static int prepare(const bh_synchrotron_desc *d, int out_dtype, std::vector<doub
     A.k_v = std::pow(kE, 3.0) / (kPi * std::pow(kMe, 2.0) * std::pow(kC, 2.0));
     A.k_q = std::pow(kE, 4.0) / (4.0 * std::pow(kPi, 2.0) * std::pow(kMe, 3.0) * std::pow(kC, 3.0));
     A.four_pi = 4.0 * kPi;
-    A.lx_first = tab[kOffLx];
-    A.lx_last = tab[kOffLx + kTable - 1];
+    A.lx_first = lxs[0];
+    A.lx_last = lxs[kTable - 1];
+    A.lF_top = lfs[kTable - 1];
+    A.lG_top = lgs[kTable - 1];
     A.inv_dlx = double(kTable - 1) / (A.lx_last - A.lx_first);
     return BH_OK;
 }
