@@ -45,7 +45,7 @@ TRANSFER_RK2, TRANSFER_EXACT = 0, 1
 class Tuning(C.Structure):
     """bh_tuning."""
     _fields_ = [("block_threads", C.c_int32), ("blocks_per_cu", C.c_int32), ("steps_per_sync", C.c_int32),
-                ("refill_min", C.c_int32), ("queues", C.c_int32), ("order", C.c_int32)]
+                ("refill_min", C.c_int32), ("queues", C.c_int32), ("order", C.c_int32), ("coop_steps", C.c_int32)]
 
 
 class BatchItem(C.Structure):
@@ -113,6 +113,7 @@ SIGNATURES = [
     ("bh_ctx_set_tuning", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),
     ("bh_ctx_get_tuning", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Tuning)]),
     ("bh_ctx_last_work", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("bh_ctx_last_coop", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("bh_batch_create", C.c_int,
      [C.c_void_p, C.POINTER(BatchItem), C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_void_p)]),
     ("bh_batch_launch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

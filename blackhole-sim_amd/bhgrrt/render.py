@@ -91,7 +91,7 @@ class Renderer:
     def set_tuning(self, precision: int, **kw) -> None:
         t = _lib.Tuning(block_threads=kw.get("block_threads", 0), blocks_per_cu=kw.get("blocks_per_cu", -1),
                         steps_per_sync=kw.get("steps_per_sync", 0), refill_min=kw.get("refill_min", 0),
-                        queues=kw.get("queues", 0), order=kw.get("order", -1))
+                        queues=kw.get("queues", 0), order=kw.get("order", -1), coop_steps=kw.get("coop_steps", -1))
         _lib.check(self._lib.bh_ctx_set_tuning(self.ctx, int(precision), C.byref(t)))
 
     def get_tuning(self, precision: int) -> dict:
@@ -131,6 +131,12 @@ class Renderer:
         """(RK2 steps, in-brick samples) of the last render launch."""
         a, b = C.c_uint64(0), C.c_uint64(0)
         _lib.check(self._lib.bh_ctx_last_work(self.ctx, C.byref(a), C.byref(b), None))
+        return int(a.value), int(b.value)
+
+    def last_coop(self) -> tuple[int, int]:
+        """(lone rays split over two waves, hand-overs that timed out) of the last render."""
+        a, b = C.c_uint64(), C.c_uint64()
+        _lib.check(self._lib.bh_ctx_last_coop(self.ctx, C.byref(a), C.byref(b)))
         return int(a.value), int(b.value)
 
     def last_lane_utilisation(self) -> float:

@@ -64,6 +64,7 @@ struct Tuning {
     int refill_min = 64;
     int queues = 8;
     int order = 1;        // 1: tiles nearest the critical curve (longest rays) first
+    int coop_steps = 32;  // lone steps before a lone ray asks an idle wave for help (0: never)
 };
 
 }  // namespace bh

@@ -72,12 +72,13 @@ struct RenderArgs {
     int *steps;
     uint8_t *status;
     unsigned *counters;
-    unsigned long long *work;   // [0] RK2 steps, [1] in-brick samples of this launch
+    unsigned long long *work;   // [0] RK2 steps, [1] in-brick samples, [2] lane slots, [3] split lone rays, [4] time-outs
     const int *tile_order;      // queue slot -> tile, or null for row-major
     const int *ray_list;        // list mode: queue slot -> pixel index (tile_order unused)
     const unsigned *n_list;     // list mode: number of listed pixels (device, read at kernel start)
     int tiles_x, n_ids, queues, q_len;
     int steps_per_sync, refill_min;
+    int coop_steps;             // lone steps before a lone ray asks an idle wave for help (0: never)
 };
 
 template <typename R>
@@ -163,6 +164,15 @@ struct Mth<double> {
 // Python max(a, b): b if b > a else a (NaN in a survives).
 template <typename R>
 __device__ __forceinline__ R pymax(R a, R b) { return (b > a) ? b : a; }
+
+// Midpoint of a segment, rounded on its own: contracted, the halving could fuse into whichever
+// subtraction consumes it (the phi wrap), and whether it does depends on the code around it, so two
+// paths sampling the same segment could round it differently.
+template <typename R>
+__device__ __forceinline__ R midpoint(R a, R b) {
+#pragma clang fp contract(off)
+    return R(0.5) * (a + b);
+}
 
 // np.clip to [eps, pi - eps], NaN-propagating (kerr.py:35-36).
 template <typename R>
@@ -305,6 +315,15 @@ __device__ __forceinline__ bool locate(const RenderArgs &A, const R *gr, const R
     return true;
 }
 
+// 1 - w for a trilinear weight, rounded on its own in FP32: contracted with the division that made w
+// (which happens only when both sit in one block) it would round differently in the paths whose w
+// comes from a register or from LDS, so every path sees the same weights.
+template <typename R>
+__device__ __forceinline__ R one_minus(R w) {
+    if constexpr (sizeof(R) == 4) return __ocml_sub_rte_f32(1.0f, w);
+    else return R(1) - w;
+}
+
 // Trilinear sample of all 11 channels at (r, theta, phi); false when outside r/theta.
 // FP64 keeps the reference's nested (1-w)a + wb lerps (native_kernels.py:180-186);
 // FP32 uses the equivalent eight-weight sum.
@@ -340,7 +359,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, base + (unsigned(q * kCoeffs) + k) * esz, 0);
         __builtin_amdgcn_sched_barrier(0);
-        const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
+        const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
         R acc = w8[0] * R(v[0]);
@@ -368,7 +387,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         }
         // all loads of the record leave before the first use, so a lone ray waits for one round trip
         __builtin_amdgcn_sched_barrier(0);
-        const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
+        const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
 #pragma unroll
@@ -420,7 +439,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
             for (int q = 0; q < 8; ++q) raw[k][q] = load_coeff<STORE>(rs, o[q], so);
         }
         __builtin_amdgcn_sched_barrier(0);
-        const R ar = R(1) - wr, at = R(1) - wt, ap = R(1) - wp;
+        const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
 #pragma unroll
@@ -687,25 +706,33 @@ __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr
     npt = pth + h * k2pt;
 }
 
-// One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
-// MIXED: the geodesic step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
-// |p_theta| < pth_island near a theta turning point) and the horizon/escape tests compare in FP64.
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool COOP = false>
-__device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
-                                        const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
+// The end point of one geodesic step: the state after it, whether it met the pole pathology (the
+// theta clamp fired, or the step swung phi by more than 1 rad) and whether all five values are finite.
+template <typename R>
+struct GeoPoint {
+    R r, th, ph, pr, pth;
+    bool hit, fin;
+};
+
+// One RK2 geodesic step of the ray from (r, th, ph, pr, pth); the ray's constants of motion come from s.
+// MIXED: the step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
+// |p_theta| < pth_island near a theta turning point).
+template <typename R, bool MIXED>
+__device__ __forceinline__ GeoPoint<R> geo_next(const RenderArgs &A, const HotParams<R> &P, const RayState<R> &s, R r,
+                                                R th, R ph, R pr, R pth) {
     const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
     R nr, nt_raw, np, npr, npt, mt;
     bool island = false;
     if constexpr (MIXED) {
-        island = (double(s.r) < A.r_island) || (double(s.th) < A.th_island) ||
-                 (double(s.th) > 3.14159265358979323846 - A.th_island) || (fabs(double(s.pth)) < A.pth_island);
+        island = (double(r) < A.r_island) || (double(th) < A.th_island) ||
+                 (double(th) > 3.14159265358979323846 - A.th_island) || (fabs(double(pth)) < A.pth_island);
     }
     if (MIXED && island) {
-        const double hd = A.adaptive ? A.step * (double(s.r) - A.r_plus) : A.step;
+        const double hd = A.adaptive ? A.step * (double(r) - A.r_plus) : A.step;
         const RayK<double> kd = make_k<double>(A.a, s.Ed, s.Ld);
         double dr_, dt_, dp_, dpr_, dpt_, dm_;
-        geo_step<double>(kd, double(s.r), double(s.th), double(s.ph), double(s.pr), double(s.pth), hd, dr_, dt_, dp_,
-                         dpr_, dpt_, dm_);
+        geo_step<double>(kd, double(r), double(th), double(ph), double(pr), double(pth), hd, dr_, dt_, dp_, dpr_, dpt_,
+                         dm_);
         nr = R(dr_);
         nt_raw = R(dt_);
         np = R(dp_);
@@ -713,49 +740,84 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> 
         npt = R(dpt_);
         mt = R(dm_);
     } else {
-        const R h = P.adaptive ? P.step * (s.r - P.r_plus) : P.step;
-        geo_step<R>(s.k, s.r, s.th, s.ph, s.pr, s.pth, h, nr, nt_raw, np, npr, npt, mt);
+        const R h = P.adaptive ? P.step * (r - P.r_plus) : P.step;
+        geo_step<R>(s.k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
     }
-    const R nt = clamp_theta(nt_raw);
-    // pole pathology: the theta clamp fired, or one step swung phi by more than 1 rad
-    s.clamped = s.clamped || (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi) || (np - s.ph > R(1)) ||
-                (s.ph - np > R(1));
-    const bool fin = finite_(nr) && finite_(nt) && finite_(np) && finite_(npr) && finite_(npt);
+    GeoPoint<R> b;
+    b.r = nr;
+    b.th = clamp_theta(nt_raw);
+    b.ph = np;
+    b.pr = npr;
+    b.pth = npt;
+    b.hit = (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi) || (np - ph > R(1)) || (ph - np > R(1));
+    b.fin = finite_(nr) && finite_(b.th) && finite_(np) && finite_(npr) && finite_(npt);
+    return b;
+}
 
-    // transfer segment (accelerated_renderer.py:113-125)
-    R c[kCoeffs];
-    if (sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, R(0.5) * (s.r + nr), R(0.5) * (s.th + nt),
-                                             R(0.5) * (s.ph + np), c)) {
-        const R dx = nr - s.r, dy = nt - s.th, dz = np - s.ph;
-        const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
-        const R ds = pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
-        if constexpr (EXACT) stokes_exact(s.S, c, ds);
-        else stokes_rk2(s.S, c, ds);
-        s.tau = s.tau + pymax(c[4], R(0)) * ds;
-        ++n_samples;
-    }
-    const R r_prev = s.r;
-    s.r = nr;
-    s.th = nt;
-    s.ph = np;
-    s.pr = npr;
-    s.pth = npt;
-    s.steps += 1;
-    // first match wins: invalid, captured, escaped, opaque, out of steps (written as selects, no branches)
-    bool captured, escaped;
+// ds of the segment from the ray's position to b (accelerated_renderer.py:119-121).
+template <typename R>
+__device__ __forceinline__ R segment_ds(const HotParams<R> &P, const RayState<R> &s, const GeoPoint<R> &b) {
+    const R dx = b.r - s.r, dy = b.th - s.th, dz = b.ph - s.ph;
+    const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
+    return pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
+}
+
+// Transfer over a segment of length ds with the coefficients c sampled at its midpoint
+// (accelerated_renderer.py:113-125).
+template <typename R, bool EXACT>
+__device__ __forceinline__ void segment_transfer(RayState<R> &s, R ds, const R c[kCoeffs]) {
+    if constexpr (EXACT) stokes_exact(s.S, c, ds);
+    else stokes_rk2(s.S, c, ds);
+    s.tau = s.tau + pymax(c[4], R(0)) * ds;
+}
+
+// The geodesic tests of a step that ends at b from radius r_prev: captured, escaped (MIXED: in FP64).
+template <typename R, bool MIXED>
+__device__ __forceinline__ void end_tests(const RenderArgs &A, const HotParams<R> &P, R r_prev, R r, bool &captured,
+                                          bool &escaped) {
     if constexpr (MIXED) {
-        captured = double(nr) <= A.r_cap;
-        escaped = (double(nr) > A.r_esc) && (nr > r_prev);
+        captured = double(r) <= A.r_cap;
+        escaped = (double(r) > A.r_esc) && (r > r_prev);
     } else {
-        captured = nr <= P.r_cap;
-        escaped = (nr > P.r_esc) && (nr > r_prev);
+        captured = r <= P.r_cap;
+        escaped = (r > P.r_esc) && (r > r_prev);
     }
+}
+
+// First match wins: invalid, captured, escaped, opaque, out of steps (selects, no branches).
+template <typename R>
+__device__ __forceinline__ int end_code(const HotParams<R> &P, const RayState<R> &s, bool captured, bool escaped,
+                                        bool fin) {
     int code = (s.steps >= P.max_steps) ? 0 : -1;
     code = (s.tau >= P.tau_max) ? 4 : code;
     code = escaped ? 2 : code;
     code = captured ? 1 : code;
     code = fin ? code : 3;
     return code;
+}
+
+// One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool COOP = false>
+__device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
+                                        const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
+    const GeoPoint<R> b = geo_next<R, MIXED>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
+    s.clamped = s.clamped || b.hit;
+    R c[kCoeffs];
+    if (sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
+                                             midpoint(s.ph, b.ph), c)) {
+        segment_transfer<R, EXACT>(s, segment_ds(P, s, b), c);
+        ++n_samples;
+    }
+    const R r_prev = s.r;
+    s.r = b.r;
+    s.th = b.th;
+    s.ph = b.ph;
+    s.pr = b.pr;
+    s.pth = b.pth;
+    s.steps += 1;
+    bool captured, escaped;
+    end_tests<R, MIXED>(A, P, r_prev, b.r, captured, escaped);
+    return end_code(P, s, captured, escaped, b.fin);
 }
 
 template <typename T>
@@ -809,27 +871,305 @@ __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R>
     if (A.status) A.status[s.pix] = uint8_t(code | (s.clamped ? 8 : 0));
 }
 
+// ---- a lone ray split over two waves -----------------------------------------------------
+// A ray that stays alone in its wave for coop_steps steps asks for help: a wave of the same
+// workgroup that has no rays of its own takes the geodesic (RK2 step, cell lookup, weights, ds and
+// the end tests) and hands each finished step to the lone wave through a ring in LDS; the lone
+// wave keeps the brick sample, the Stokes update and the optical depth, and decides where the ray
+// ends.  Each wave then issues about half of a step's instructions, and the two halves overlap.
+// Every wait is bounded: a helper that stops answering makes the lone wave finish the ray alone
+// from the last step it received (counted in work[4]).
+constexpr int kCoopRing = 32;
+constexpr unsigned kCoopSpin = 1u << 21;   // ~0.1 s of s_sleep(1) polls
+// The hand-over is one word, phase | requesting wave << 8, so every transition is a single
+// compare-and-swap on the exact value read: Idle -> Asked (lone wave), Asked -> Accepted (helper),
+// Accepted -> Go (lone wave, after writing the ray) or Accepted -> Cancel (the ray ended first),
+// Go / Cancel -> Free (helper, leaving), Free or Asked -> Idle (lone wave).
+enum : int { kCoopIdle = 0, kCoopAsked = 1, kCoopAccepted = 2, kCoopGo = 3, kCoopCancel = 4, kCoopFree = 5 };
+__device__ __forceinline__ int coop_word(int phase, int wave) { return phase | (wave << 8); }
+
+struct CoopSlot {
+    float r, th, ph, pr, pth;   // the ray after the step (restart point if the helper stops answering)
+    float wr, wt, wp, ds;
+    unsigned cell;              // byte offset of the midpoint's cell record, ~0u outside the brick
+    unsigned flags;             // bit 0 captured, 1 escaped, 2 finite, 3 pole pathology
+    unsigned pad;
+};
+
+struct CoopLds {
+    int state;       // coop_word(phase, requesting wave)
+    int tracing;     // waves of the workgroup that may still trace rays
+    unsigned head;   // steps written by the helper
+    unsigned tail;   // steps taken by the lone wave
+    int stop;        // the lone wave is done: the helper stops
+    RayState<float> ray;
+    CoopSlot ring[kCoopRing];
+};
+
+// Byte offset of a cell's record in the corner layout (8 corners x 11 channels, corner-major).
+template <int STORE>
+__device__ __forceinline__ unsigned corner_record(const RenderArgs &A, int r0, int t0, int p0) {
+    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+    const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
+    return A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+}
+
+// Ring indices: a wave's LDS operations execute in order, so a relaxed store of the index after the
+// slot's stores (kept in order by a compiler fence) publishes the slot without waiting for the
+// stores to complete, and a reader that sees the index sees the slot.
+template <typename T>
+__device__ __forceinline__ T lds_peek(T *p) {
+    const T v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ void lds_publish(T *p, T v) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T, typename F>
+__device__ __forceinline__ T lds_wait_relaxed(T *p, F pred, bool &timed_out) {
+    T v = lds_peek(p);
+    unsigned spins = 0;
+    while (!pred(v)) {
+        if (++spins > kCoopSpin) {
+            timed_out = true;
+            return v;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = lds_peek(p);
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T lds_load(T *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void lds_store(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ bool lds_cas(int *p, int expect, int v) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// one lane does it, the wave learns the result
+__device__ __forceinline__ bool wave_cas(int *p, int expect, int v, int lane) {
+    int ok = 0;
+    if (lane == 0) ok = lds_cas(p, expect, v) ? 1 : 0;
+    return __builtin_amdgcn_readfirstlane(ok) != 0;
+}
+// poll *p until pred(value) holds or the spin bound passes; returns the last value read
+template <typename T, typename F>
+__device__ __forceinline__ T lds_wait(T *p, F pred, bool &timed_out) {
+    T v = lds_load(p);
+    unsigned spins = 0;
+    while (!pred(v)) {
+        if (++spins > kCoopSpin) {
+            timed_out = true;
+            return v;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = lds_load(p);
+    }
+    return v;
+}
+
+// Helper side: the geodesic of the handed-over ray, one ring slot per step, until the geodesic ends the
+// ray, the lone wave says stop, or the ring stays full past the spin bound.
+template <int STORE, bool MIXED>
+__device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotParams<float> &P, const float *gr,
+                                                 const float *gt, const float *gp, CoopLds *c, int lane) {
+    RayState<float> s = c->ray;
+    __builtin_amdgcn_s_setprio(2);   // the lone wave waits on every step of this wave
+    unsigned head = 0, tail = 0;
+    for (;;) {
+        if (head - tail >= unsigned(kCoopRing)) {   // ring full: wait for the lone wave
+            bool timed_out = false;
+            tail = lds_wait_relaxed(&c->tail, [&](unsigned v) { return head - v < unsigned(kCoopRing) || lds_peek(&c->stop); },
+                                    timed_out);
+            if (timed_out || head - tail >= unsigned(kCoopRing)) break;
+        }
+        if ((head & 7u) == 0 && lds_peek(&c->stop)) break;
+        const GeoPoint<float> b = geo_next<float, MIXED>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
+        CellPos<float> cp;
+        const bool in = locate(A, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th), midpoint(s.ph, b.ph), cp);
+        const float ds = segment_ds(P, s, b);
+        bool captured, escaped;
+        end_tests<float, MIXED>(A, P, s.r, b.r, captured, escaped);
+        s.r = b.r;
+        s.th = b.th;
+        s.ph = b.ph;
+        s.pr = b.pr;
+        s.pth = b.pth;
+        s.steps += 1;
+        if (lane == 0) {
+            CoopSlot &sl = c->ring[head & unsigned(kCoopRing - 1)];
+            sl.r = b.r;
+            sl.th = b.th;
+            sl.ph = b.ph;
+            sl.pr = b.pr;
+            sl.pth = b.pth;
+            sl.wr = cp.wr;
+            sl.wt = cp.wt;
+            sl.wp = cp.wp;
+            sl.ds = ds;
+            sl.cell = in ? corner_record<STORE>(A, cp.r0, cp.t0, cp.p0) : ~0u;
+            sl.flags = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u);
+            lds_publish(&c->head, head + 1);
+        }
+        ++head;
+        if (captured || escaped || !b.fin || s.steps >= P.max_steps) break;   // the geodesic ends the ray here
+    }
+    return head;
+}
+
+// Lone-wave side: take the helper's steps until the ray ends.  Returns the end code, or -1 after a
+// time-out (s then holds the ray after the last step taken, and the lone loop continues alone).
+template <int STORE, bool EXACT>
+__device__ __forceinline__ int coop_consume(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
+                                            RayState<float> &s, CoopLds *c, int lane, unsigned &steps,
+                                            unsigned &samples) {
+    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+    const unsigned k = min(unsigned(lane), unsigned(kCoeffs - 1));
+    unsigned tail = 0;
+    for (;;) {
+        bool timed_out = false;
+        lds_wait_relaxed(&c->head, [&](unsigned h) { return h != tail; }, timed_out);
+        if (timed_out) {
+            if (lane == 0) lds_store(&c->stop, 1);
+            return -1;
+        }
+        const CoopSlot sl = c->ring[tail & unsigned(kCoopRing - 1)];
+        s.clamped = s.clamped || (sl.flags & 8u);
+        if (sl.cell != ~0u) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, sl.cell + (unsigned(q * kCoeffs) + k) * esz, 0);
+            float w8[8];
+            {
+                const float ar = one_minus(sl.wr), at = one_minus(sl.wt), ap = one_minus(sl.wp);
+                const float q00 = ar * at, q01 = ar * sl.wt, q10 = sl.wr * at, q11 = sl.wr * sl.wt;
+                w8[0] = q00 * ap;
+                w8[1] = q00 * sl.wp;
+                w8[2] = q01 * ap;
+                w8[3] = q01 * sl.wp;
+                w8[4] = q10 * ap;
+                w8[5] = q10 * sl.wp;
+                w8[6] = q11 * ap;
+                w8[7] = q11 * sl.wp;
+            }
+            float acc = w8[0] * v[0];
+#pragma unroll
+            for (int q = 1; q < 8; ++q) acc = acc + w8[q] * v[q];
+            float cf[kCoeffs];
+#pragma unroll
+            for (int j = 0; j < kCoeffs; ++j)
+                cf[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, acc), j));
+            segment_transfer<float, EXACT>(s, sl.ds, cf);
+            ++samples;
+        }
+        s.r = sl.r;
+        s.th = sl.th;
+        s.ph = sl.ph;
+        s.pr = sl.pr;
+        s.pth = sl.pth;
+        s.steps += 1;
+        ++steps;
+        ++tail;
+        if (lane == 0) lds_publish(&c->tail, tail);
+        const int code = end_code(P, s, (sl.flags & 1u) != 0, (sl.flags & 2u) != 0, (sl.flags & 4u) != 0);
+        if (code >= 0) {
+            if (lane == 0) lds_store(&c->stop, 1);
+            return code;
+        }
+    }
+}
+
 // A wave left with one live ray (FP32 kernels, corner layout): every lane takes a copy of it, the brick sample of
 // each step is split over the lanes (COOP), and the ray runs to its end; then its owner lane writes
-// the result.  The copies stay identical, so the wave leaves the loop together.
+// the result.  The copies stay identical, so the wave leaves the loop together.  With coop != null and
+// coop_steps > 0, a ray still alone after coop_steps steps asks an idle wave of the workgroup to take its
+// geodesic (see above).
 template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
 __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                          const R *gr, const R *gt, const R *gp, RayState<R> &s, int lane, int owner,
-                                         unsigned &n_steps, unsigned &n_samples, unsigned long long &iters) {
+                                         unsigned &n_steps, unsigned &n_samples, unsigned long long &iters,
+                                         CoopLds *coop, int wave) {
     broadcast_ray(s, owner);
     __builtin_amdgcn_s_setprio(2);
     unsigned steps = 0, samples = 0;
     int code;
-    do {
-        code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, true>(A, P, rs, gr, gt, gp, s, samples);
-        ++steps;
-    } while (code < 0);
+    bool asked = false, helped = false, timed_out = false;
+    bool coop_on = false;
+    if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) coop_on = coop != nullptr && A.coop_steps > 0;
+    for (;;) {
+        // the lone loop proper; with cooperation on, it pauses every few steps for the bookkeeping below
+        const unsigned pause = coop_on ? (asked ? steps + 4u : max(steps + 8u, unsigned(A.coop_steps))) : ~0u;
+        do {
+            code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, true>(A, P, rs, gr, gt, gp, s, samples);
+            ++steps;
+        } while (code < 0 && steps < pause);
+        if (code >= 0) break;
+        if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
+            if (!asked) {
+                asked = wave_cas(&coop->state, kCoopIdle, coop_word(kCoopAsked, wave), lane);
+            } else if (lds_load(&coop->state) == coop_word(kCoopAccepted, wave)) {
+                if (lane == 0) {
+                    coop->ray = s;
+                    coop->head = 0u;
+                    coop->tail = 0u;
+                    coop->stop = 0;
+                    lds_store(&coop->state, coop_word(kCoopGo, wave));
+                }
+                helped = true;
+                coop_on = false;
+                code = coop_consume<STORE, EXACT>(A, P, rs, s, coop, lane, steps, samples);
+                if (code >= 0) break;
+                timed_out = true;   // finish alone from the last step received
+            }
+        }
+    }
+    if (asked && lane == 0) {
+        // withdraw the request (nobody accepted it), or cancel it / wait for the helper to leave
+        if (!lds_cas(&coop->state, coop_word(kCoopAsked, wave), kCoopIdle)) {
+            lds_cas(&coop->state, coop_word(kCoopAccepted, wave), coop_word(kCoopCancel, wave));
+            bool t2 = false;
+            const int free_word = coop_word(kCoopFree, wave);
+            lds_wait(&coop->state, [free_word](int v) { return v == free_word; }, t2);
+            timed_out = timed_out || t2;
+            lds_cas(&coop->state, free_word, kCoopIdle);   // on a time-out the word stays taken: no more hand-overs here
+        }
+    }
     if (lane == owner) {
         ray_store(A, s, code);
         n_steps += steps;
         n_samples += samples;
+        if (helped) atomicAdd(&A.work[3], 1ull);
+        if (timed_out) atomicAdd(&A.work[4], 1ull);
     }
     iters += steps;
+}
+
+// Helper side of the handshake, for a wave with no rays of its own.  Returns true when it helped (or
+// took a request that was then withdrawn).
+template <int STORE, bool MIXED>
+__device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<float> &P, const float *gr,
+                                          const float *gt, const float *gp, CoopLds *c, int lane, int wave,
+                                          unsigned long long &iters) {
+    const int word = lds_load(&c->state);
+    const int requester = word >> 8;
+    if ((word & 0xff) != kCoopAsked || requester == wave) return false;
+    if (!wave_cas(&c->state, word, coop_word(kCoopAccepted, requester), lane)) return false;
+    bool timed_out = false;
+    const int go = coop_word(kCoopGo, requester), cancel = coop_word(kCoopCancel, requester);
+    const int st = lds_wait(&c->state, [go, cancel](int v) { return v == go || v == cancel; }, timed_out);
+    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED>(A, P, gr, gt, gp, c, lane);
+    // leave: Go / Cancel -> Free (after a time-out the lone wave sees no Free and stops using the slot)
+    if (lane == 0 && !timed_out) lds_store(&c->state, coop_word(kCoopFree, requester));
+    return true;
 }
 
 __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
@@ -847,6 +1187,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     const int ng = A.nr + A.nt + A.np;
     for (int i = threadIdx.x; i < ng; i += blockDim.x) s_grid[i] = R(A.grids[i]);
     if (threadIdx.x == 0) s_grid[ng] = R(__builtin_huge_valf());   // sentinel g[np] of the phi grid
+    __shared__ CoopLds coop;
+    if (threadIdx.x == 0) {
+        coop.state = kCoopIdle;
+        coop.tracing = int((blockDim.x + 63) / 64);
+    }
     __syncthreads();
     const R *gr = s_grid;
     const R *gt = s_grid + A.nr;
@@ -869,9 +1214,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         q_len = (n_ids + A.queues - 1) / A.queues;
     }
 
+    const int wave = int(threadIdx.x >> 6);
     for (;;) {
         unsigned long long idle = __ballot(!active);
         const int n_idle = __popcll(idle);
+        if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
+            // a wave with no rays first serves a lone ray of its workgroup that asks for help
+            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+        }
         if (exhausted != all_q && (n_idle >= A.refill_min || n_idle == 64)) {
             for (int t = 0; t < A.queues && idle; ++t) {
                 const int q = (q_own + t) % A.queues;
@@ -910,13 +1260,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const unsigned long long live_mask = __ballot(active);
         const int live = __popcll(live_mask);
         if (live == 0) {
-            if (exhausted == all_q) break;
+            if (exhausted == all_q) {
+                if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
+                    // out of rays for good: stay while another wave of the workgroup still traces, since its
+                    // last ray may ask for help (bounded, like every wait of the hand-over)
+                    if (lane == 0) __hip_atomic_fetch_add(&coop.tracing, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (A.coop_steps > 0) {
+                        unsigned spins = 0;
+                        while (lds_peek(&coop.tracing) > 0 && spins < kCoopSpin) {
+                            if (coop_help<STORE, MIXED>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+                            __builtin_amdgcn_s_sleep(2);
+                            ++spins;
+                        }
+                    }
+                }
+                break;
+            }
             continue;
         }
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {   // no refill happened above: the wave serves its last ray with all lanes
                 lone_ray<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
-                                                  n_steps, n_samples, iters);
+                                                  n_steps, n_samples, iters, &coop, wave);
                 active = false;
                 continue;
             }
@@ -1029,7 +1394,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {
                 lone_ray<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
-                                                  n_steps, n_samples, iters);
+                                                  n_steps, n_samples, iters, nullptr, 0);
                 active = false;
                 continue;
             }
@@ -1413,6 +1778,7 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
     }
     A.steps_per_sync = u.steps_per_sync;
     A.refill_min = u.refill_min;
+    A.coop_steps = u.coop_steps;
     int per_cu = u.blocks_per_cu;
     if (per_cu <= 0) {
         int nb = 0;
@@ -1530,6 +1896,17 @@ int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples, uint64_t *
     return BH_OK;
 }
 
+int bh_ctx_last_coop(bh_ctx *ctx, uint64_t *split_rays, uint64_t *timeouts) {
+    int st = ctx_check(ctx);
+    if (st) return st;
+    unsigned long long w[2] = {0, 0};
+    BH_HIP(hipMemcpyAsync(w, ctx->d_counters + kWorkOffset + 6, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    if (split_rays) *split_rays = w[0];
+    if (timeouts) *timeouts = w[1];
+    return BH_OK;
+}
+
 int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t) {
     int st = ctx_check(ctx);
     if (st) return st;
@@ -1541,6 +1918,7 @@ int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t) {
     if (t->refill_min) u.refill_min = t->refill_min;
     if (t->queues) u.queues = t->queues;
     if (t->order >= 0) u.order = t->order;
+    if (t->coop_steps >= 0) u.coop_steps = t->coop_steps;
     if (u.block_threads < 64 || u.block_threads > 256 || u.block_threads % 64)
         return set_error(BH_ERR_VALUE, "block_threads must be 64, 128, 192 or 256");
     if (u.steps_per_sync < 1 || u.steps_per_sync > 4096) return set_error(BH_ERR_VALUE, "steps_per_sync out of range");
@@ -1548,6 +1926,7 @@ int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t) {
     if (u.queues < 1 || u.queues > kMaxQueues) return set_error(BH_ERR_VALUE, "queues must lie in 1..%d", kMaxQueues);
     if (u.blocks_per_cu > 64) return set_error(BH_ERR_VALUE, "blocks_per_cu out of range");
     if (u.order != 0 && u.order != 1) return set_error(BH_ERR_VALUE, "order must be 0 or 1");
+    if (u.coop_steps > (1 << 20)) return set_error(BH_ERR_VALUE, "coop_steps out of range");
     ctx->tuning[precision] = u;
     return BH_OK;
 }
@@ -1563,6 +1942,7 @@ int bh_ctx_get_tuning(bh_ctx *ctx, int32_t precision, bh_tuning *t) {
     t->refill_min = u.refill_min;
     t->queues = u.queues;
     t->order = u.order;
+    t->coop_steps = u.coop_steps;
     return BH_OK;
 }
 

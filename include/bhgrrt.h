@@ -175,6 +175,8 @@ typedef struct bh_tuning {
     int32_t refill_min;       /* refill when at least this many lanes are idle (1..64) */
     int32_t queues;           /* 1, or 8 for one ray queue per XCD */
     int32_t order;            /* 0: row-major tiles; 1: tiles nearest the critical curve first */
+    int32_t coop_steps;       /* render kernels: a ray alone in its wave for this many steps asks an idle
+                                 wave of its workgroup to take its geodesic (0: never; -1 in set: keep) */
 } bh_tuning;
 
 int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t);
@@ -185,6 +187,9 @@ int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t);
  * after ray termination and compaction.  Any pointer may be NULL. */
 int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples, uint64_t *lane_slots);
 int bh_ctx_get_tuning(bh_ctx *ctx, int32_t precision, bh_tuning *t);
+/* Lone rays of the last render whose geodesic an idle wave took over, and hand-overs that timed out
+ * (the lone wave then finished the ray alone). */
+int bh_ctx_last_coop(bh_ctx *ctx, uint64_t *split_rays, uint64_t *timeouts);
 
 /* Host outputs: iquv is H*W*4 doubles for BH_PREC_FP64, floats otherwise;
  * steps H*W int32; status H*W bytes (either may be NULL). */

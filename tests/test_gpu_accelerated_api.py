@@ -50,6 +50,9 @@ def test_progressive_frames_equal_single_renders(renderer):
     frames = acc.render_progressive_stokes_bricks(cfg, 0.94, br, PREC_FP32, renderer=renderer)
     assert [(f.level, f.width, f.height) for f in frames] == [(0, 24, 13), (1, 48, 27), (2, 96, 54)]
     db = renderer.upload(br, "f32", "corner")
+    saved = renderer.get_tuning(PREC_FP32)
+    # the batched launch has no two-wave split of lone rays: compare with single renders without it
+    renderer.set_tuning(PREC_FP32, coop_steps=0)
     try:
         for f in frames:
             single = renderer.render(acc.scene_from_config(cfg, 0.94, f.width, f.height), db, PREC_FP32)
@@ -57,6 +60,7 @@ def test_progressive_frames_equal_single_renders(renderer):
             np.testing.assert_array_equal(f.stokes, single["iquv"].astype(np.float64))
             assert np.abs(f.stokes[..., 0]).sum() > 0
     finally:
+        renderer.set_tuning(PREC_FP32, **saved)
         renderer.release(db)
 
 

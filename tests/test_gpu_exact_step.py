@@ -86,6 +86,9 @@ def test_exact_batch_equals_single_renders(renderer):
     jobs = [(replace(sc.with_size(w, w), exact_step=True), dbs[k % 2]) for k, (sc, _), w in
             zip(range(3), scenes.batch_scenes(3, seed=5), (24, 40, 17))]
     batch = Batch(renderer, jobs, PREC_FP32)
+    saved = renderer.get_tuning(PREC_FP32)
+    # the batched launch has no two-wave split of lone rays: compare with single renders without it
+    renderer.set_tuning(PREC_FP32, coop_steps=0)
     try:
         batch.launch(graph=False)
         for (sc, db), g in zip(jobs, batch.results()):
@@ -94,6 +97,7 @@ def test_exact_batch_equals_single_renders(renderer):
             np.testing.assert_array_equal(g["steps"], ref["steps"])
             np.testing.assert_array_equal(g["iquv"], ref["iquv"])
     finally:
+        renderer.set_tuning(PREC_FP32, **saved)
         batch.close()
         for d in dbs:
             renderer.release(d)

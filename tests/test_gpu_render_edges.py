@@ -27,10 +27,12 @@ from oracle import render as orender
 
 pytestmark = pytest.mark.gpu
 
+# coop_steps=0: the lone-ray path without the two-wave split (the split is checked against it below,
+# to rounding, because its brick weights reach the sample through LDS)
 TUNINGS = [
-    dict(block_threads=64, blocks_per_cu=1, steps_per_sync=1, refill_min=1, queues=1, order=0),
-    dict(block_threads=256, blocks_per_cu=0, steps_per_sync=32, refill_min=64, queues=8, order=1),
-    dict(block_threads=192, blocks_per_cu=2, steps_per_sync=7, refill_min=17, queues=3, order=1),
+    dict(block_threads=64, blocks_per_cu=1, steps_per_sync=1, refill_min=1, queues=1, order=0, coop_steps=0),
+    dict(block_threads=256, blocks_per_cu=0, steps_per_sync=32, refill_min=64, queues=8, order=1, coop_steps=0),
+    dict(block_threads=192, blocks_per_cu=2, steps_per_sync=7, refill_min=17, queues=3, order=1, coop_steps=0),
 ]
 
 
@@ -47,10 +49,38 @@ def test_tuning_does_not_change_the_image(renderer, storage, layout, precision):
     db = renderer.upload(br, storage, layout)
     saved = renderer.get_tuning(precision)
     try:
+        renderer.set_tuning(precision, coop_steps=0)
         base = renderer.render(sc, db, precision)
         for t in TUNINGS:
             renderer.set_tuning(precision, **t)
             _same(renderer.render(sc, db, precision), base)
+    finally:
+        renderer.set_tuning(precision, **saved)
+        renderer.release(db)
+
+
+@pytest.mark.parametrize("storage,precision,size", [("f32", PREC_FP32, 1024), ("f32", PREC_FP32, 512),
+                                                   ("bf16", PREC_MIXED, 1024)])
+def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size):
+    """A lone ray whose geodesic an idle wave takes over (coop_steps > 0) ends on the same step with the same
+    status as the lone-wave path, and its Stokes vector agrees to rounding."""
+    sc, br = scenes.scale_plain_scene(size, size), scenes.scale_brick()
+    db = renderer.upload(br, storage, "corner")
+    saved = renderer.get_tuning(precision)
+    try:
+        renderer.set_tuning(precision, coop_steps=0)
+        base = renderer.render(sc, db, precision)
+        assert renderer.last_coop() == (0, 0)
+        for cs in (8, 32):
+            renderer.set_tuning(precision, coop_steps=cs)
+            out = renderer.render(sc, db, precision)
+            split, timeouts = renderer.last_coop()
+            d = np.abs(out["iquv"].astype(np.float64) - base["iquv"]) / np.abs(base["iquv"][..., 0]).max()
+            print(f"{storage} {size}: coop_steps {cs}: {split} split rays, {timeouts} time-outs, max |dIQUV| / max I {d.max():.1e}")
+            assert split > 0 and timeouts == 0
+            np.testing.assert_array_equal(out["status"], base["status"])
+            np.testing.assert_array_equal(out["steps"], base["steps"])
+            assert d.max() <= 1e-6
     finally:
         renderer.set_tuning(precision, **saved)
         renderer.release(db)
