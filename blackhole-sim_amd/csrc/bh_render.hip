@@ -273,16 +273,34 @@ struct CellPos {
     R wr, wt, wp;
 };
 
+// The first half of locate, which reads no grid: the r/theta range test, the wrapped phi and the
+// three index guesses.
 template <typename R>
-__device__ __forceinline__ bool locate(const RenderArgs &A, const R *gr, const R *gt, const R *gp, R x, R y, R z,
-                                       CellPos<R> &c) {
+struct CellGuess {
+    R p;
+    int ir, it, ip;
+};
+
+template <typename R>
+__device__ __forceinline__ bool locate_guess(const RenderArgs &A, R x, R y, R z, CellGuess<R> &g) {
     const R r_first = R(A.hr.first), r_last = R(A.hr.last), t_first = R(A.ht.first), t_last = R(A.ht.last);
     if (x < r_first || x > r_last || y < t_first || y > t_last) return false;   // NaN passes, as in the reference
+    g.p = Mth<R>::wrap(z, R(A.hp.first));
+    g.ir = guess_index<true>(float(x), A.hr, A.nr - 2);
+    g.it = guess_index<false>(float(y), A.ht, A.nt - 2);
+    g.ip = guess_index<false>(float(g.p), A.hp, A.np - 1);
+    return true;
+}
+
+// The second half: check the guesses against the grids (LDS), search where they miss, the seam cell
+// and the weights.
+template <typename R>
+__device__ __forceinline__ void locate_finish(const RenderArgs &A, const R *gr, const R *gt, const R *gp, R x, R y,
+                                              const CellGuess<R> &g, CellPos<R> &c) {
+    const R r_last = R(A.hr.last), t_last = R(A.ht.last);
     const R p_first = R(A.hp.first);
-    const R p = Mth<R>::wrap(z, p_first);
-    int ir = guess_index<true>(float(x), A.hr, A.nr - 2);
-    int it = guess_index<false>(float(y), A.ht, A.nt - 2);
-    int ip = guess_index<false>(float(p), A.hp, A.np - 1);
+    const R p = g.p;
+    int ir = g.ir, it = g.it, ip = g.ip;
     R rl = gr[ir], rh = gr[ir + 1], tl = gt[it], th = gt[it + 1], pl = gp[ip], ph = gp[ip + 1];   // g[np] = +inf
     // bitwise, not short-circuit: all six reads are needed, so no branch may split them
     const bool bad = ((ir > 0) & (x < rl)) | ((ir < A.nr - 2) & (x >= rh)) | ((it > 0) & (y < tl)) |
@@ -312,6 +330,14 @@ __device__ __forceinline__ bool locate(const RenderArgs &A, const R *gr, const R
     c.wr = (x == r_last) ? R(1) : Mth<R>::div(x - rl, rh - rl);
     c.wt = (y == t_last) ? R(1) : Mth<R>::div(y - tl, th - tl);
     c.wp = Mth<R>::div(pp - pl, p_hi - pl);
+}
+
+template <typename R>
+__device__ __forceinline__ bool locate(const RenderArgs &A, const R *gr, const R *gt, const R *gp, R x, R y, R z,
+                                       CellPos<R> &c) {
+    CellGuess<R> g;
+    if (!locate_guess(A, x, y, z, g)) return false;
+    locate_finish(A, gr, gt, gp, x, y, g, c);
     return true;
 }
 
@@ -873,12 +899,14 @@ __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R>
 
 // ---- a lone ray split over two waves -----------------------------------------------------
 // A ray that stays alone in its wave for coop_steps steps asks for help: a wave of the same
-// workgroup that has no rays of its own takes the geodesic (RK2 step, cell lookup, weights, ds and
-// the end tests) and hands each finished step to the lone wave through a ring in LDS; the lone
-// wave keeps the brick sample, the Stokes update and the optical depth, and decides where the ray
-// ends.  Each wave then issues about half of a step's instructions, and the two halves overlap.
-// Every wait is bounded: a helper that stops answering makes the lone wave finish the ray alone
-// from the last step it received (counted in work[4]).
+// workgroup that has no rays of its own takes the geodesic (the RK2 step, the end tests, ds, and the
+// midpoint with its wrapped phi and index guesses) and hands each step to the lone wave through a
+// ring in LDS; the lone wave does the rest: it checks the guesses against the grids, forms the
+// weights, samples the brick, updates Stokes and the optical depth, and decides where the ray ends.
+// It starts the next step's brick loads before it integrates the current step, so the loads are in
+// flight while it works.  A lone step is bound by the issue of its dependent instructions, and the
+// two waves each issue about half of them.  Every wait is bounded: a helper that stops answering
+// makes the lone wave finish the ray alone from the last step it took (counted in work[4]).
 constexpr int kCoopRing = 32;
 constexpr unsigned kCoopSpin = 1u << 21;   // ~0.1 s of s_sleep(1) polls
 // The hand-over is one word, phase | requesting wave << 8, so every transition is a single
@@ -888,12 +916,17 @@ constexpr unsigned kCoopSpin = 1u << 21;   // ~0.1 s of s_sleep(1) polls
 enum : int { kCoopIdle = 0, kCoopAsked = 1, kCoopAccepted = 2, kCoopGo = 3, kCoopCancel = 4, kCoopFree = 5 };
 __device__ __forceinline__ int coop_word(int phase, int wave) { return phase | (wave << 8); }
 
-struct CoopSlot {
+struct alignas(16) CoopSlot {
     float r, th, ph, pr, pth;   // the ray after the step (restart point if the helper stops answering)
-    float wr, wt, wp, ds;
+    unsigned flags;             // bit 0 captured, 1 escaped, 2 finite, 3 pole pathology, 4 midpoint in r/theta range
+    float ds;
+    // RK2 transfer: the midpoint's r and theta, its wrapped phi and index guesses (with bit 4)
+    float x, y;
+    CellGuess<float> g;
+    // exact transfer, whose Stokes step costs the lone wave more: the whole lookup, done by the helper
     unsigned cell;              // byte offset of the midpoint's cell record, ~0u outside the brick
-    unsigned flags;             // bit 0 captured, 1 escaped, 2 finite, 3 pole pathology
-    unsigned pad;
+    float wr, wt, wp;
+    unsigned pad[3];
 };
 
 struct CoopLds {
@@ -979,7 +1012,7 @@ __device__ __forceinline__ T lds_wait(T *p, F pred, bool &timed_out) {
 
 // Helper side: the geodesic of the handed-over ray, one ring slot per step, until the geodesic ends the
 // ray, the lone wave says stop, or the ring stays full past the spin bound.
-template <int STORE, bool MIXED>
+template <int STORE, bool MIXED, bool EXACT>
 __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotParams<float> &P, const float *gr,
                                                  const float *gt, const float *gp, CoopLds *c, int lane) {
     RayState<float> s = c->ray;
@@ -994,11 +1027,15 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
         }
         if ((head & 7u) == 0 && lds_peek(&c->stop)) break;
         const GeoPoint<float> b = geo_next<float, MIXED>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
-        CellPos<float> cp;
-        const bool in = locate(A, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th), midpoint(s.ph, b.ph), cp);
-        const float ds = segment_ds(P, s, b);
         bool captured, escaped;
         end_tests<float, MIXED>(A, P, s.r, b.r, captured, escaped);
+        const float x = midpoint(s.r, b.r), y = midpoint(s.th, b.th), z = midpoint(s.ph, b.ph);
+        CellGuess<float> g;
+        CellPos<float> cp;
+        bool in;
+        if constexpr (EXACT) in = locate(A, gr, gt, gp, x, y, z, cp);
+        else in = locate_guess(A, x, y, z, g);
+        const float ds = segment_ds(P, s, b);
         s.r = b.r;
         s.th = b.th;
         s.ph = b.ph;
@@ -1012,12 +1049,18 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
             sl.ph = b.ph;
             sl.pr = b.pr;
             sl.pth = b.pth;
-            sl.wr = cp.wr;
-            sl.wt = cp.wt;
-            sl.wp = cp.wp;
+            sl.flags = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u) | (in ? 16u : 0u);
             sl.ds = ds;
-            sl.cell = in ? corner_record<STORE>(A, cp.r0, cp.t0, cp.p0) : ~0u;
-            sl.flags = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u);
+            if constexpr (EXACT) {
+                sl.cell = in ? corner_record<STORE>(A, cp.r0, cp.t0, cp.p0) : ~0u;
+                sl.wr = cp.wr;
+                sl.wt = cp.wt;
+                sl.wp = cp.wp;
+            } else {
+                sl.x = x;
+                sl.y = y;
+                sl.g = g;
+            }
             lds_publish(&c->head, head + 1);
         }
         ++head;
@@ -1026,65 +1069,146 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
     return head;
 }
 
+// One step taken from the ring by the lone wave: its end point and flags, the midpoint's cell record
+// (~0u outside the brick) and weights, ds, and channel k's 8 corner values.
+struct CoopStep {
+    float r, th, ph, pr, pth;
+    unsigned flags, cell;
+    float wr, wt, wp, ds;
+    float v[8];
+};
+
+// Waits for ring slot `next`, then finishes the lookup of the step's midpoint cell.  False after a
+// time-out.
+template <int STORE, bool EXACT>
+__device__ __forceinline__ bool coop_locate(const RenderArgs &A, const float *gr, const float *gt, const float *gp,
+                                            CoopLds *c, unsigned next, CoopStep &st) {
+    // the slot is read with the index, not after it: a wave's LDS reads execute in order and the helper
+    // writes the slot before the index, so a ready index means the slot read saw the step
+    CoopSlot sl;
+    const unsigned h = lds_peek(&c->head);
+    sl = c->ring[next & unsigned(kCoopRing - 1)];
+    // keeps the compiler from sinking the slot read below the test of the index
+    asm volatile("" ::"v"(sl.r), "v"(sl.th), "v"(sl.ph), "v"(sl.pr), "v"(sl.pth), "v"(sl.flags), "v"(sl.ds));
+    if constexpr (EXACT) asm volatile("" ::"v"(sl.cell), "v"(sl.wr), "v"(sl.wt), "v"(sl.wp));
+    else asm volatile("" ::"v"(sl.x), "v"(sl.y), "v"(sl.g.p), "v"(sl.g.ir), "v"(sl.g.it), "v"(sl.g.ip));
+    if (h == next) {
+        bool timed_out = false;
+        lds_wait_relaxed(&c->head, [next](unsigned v) { return v != next; }, timed_out);
+        if (timed_out) return false;
+        sl = c->ring[next & unsigned(kCoopRing - 1)];
+    }
+    st.r = sl.r;
+    st.th = sl.th;
+    st.ph = sl.ph;
+    st.pr = sl.pr;
+    st.pth = sl.pth;
+    st.flags = sl.flags;
+    st.ds = sl.ds;
+    if constexpr (EXACT) {
+        st.cell = sl.cell;
+        st.wr = sl.wr;
+        st.wt = sl.wt;
+        st.wp = sl.wp;
+        return true;
+    }
+    st.cell = ~0u;
+    if (sl.flags & 16u) {
+        CellPos<float> cp;
+        locate_finish(A, gr, gt, gp, sl.x, sl.y, sl.g, cp);
+        st.cell = corner_record<STORE>(A, cp.r0, cp.t0, cp.p0);
+        st.wr = cp.wr;
+        st.wt = cp.wt;
+        st.wp = cp.wp;
+    }
+    return true;
+}
+
+// Channel k's 8 corner values of the record at byte offset rec.  Every step issues these 8 loads, from
+// the brick's first record when it has no cell to read, so the count of loads in flight is the same on
+// every path and the wait before a step's arithmetic leaves the next step's loads outstanding.
+template <int STORE>
+__device__ __forceinline__ void coop_loads(__amdgpu_buffer_rsrc_t rs, unsigned rec, unsigned k, CoopStep &st) {
+    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) st.v[q] = load_coeff<STORE>(rs, rec + (unsigned(q * kCoeffs) + k) * esz, 0);
+}
+
+// One step of the lone wave: start the next step (slot `next`: lookup and loads), then integrate the
+// current one.  True when the ray ended (code >= 0) or the helper stopped answering (code -1).
+template <int STORE, bool EXACT>
+__device__ __forceinline__ bool coop_step(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
+                                          const float *gr, const float *gt, const float *gp, RayState<float> &s,
+                                          CoopLds *c, int lane, unsigned k, unsigned &next, const CoopStep &cur,
+                                          CoopStep &nxt, unsigned &steps, unsigned &samples, int &code) {
+    if (lane == 0) lds_publish(&c->tail, next);   // slot contents are in registers: the helper may reuse them
+    // the helper writes no slot past the step that ends the geodesic
+    const bool geo_end = (cur.flags & 3u) != 0 || (cur.flags & 4u) == 0 || s.steps + 1u >= P.max_steps;
+    bool have_next = false;
+    if (!geo_end) have_next = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, nxt);
+    coop_loads<STORE>(rs, (have_next && nxt.cell != ~0u) ? nxt.cell : A.brick_base, k, nxt);
+    s.clamped = s.clamped || (cur.flags & 8u);
+    if (cur.cell != ~0u) {
+        float w8[8];
+        {
+            const float ar = one_minus(cur.wr), at = one_minus(cur.wt), ap = one_minus(cur.wp);
+            const float q00 = ar * at, q01 = ar * cur.wt, q10 = cur.wr * at, q11 = cur.wr * cur.wt;
+            w8[0] = q00 * ap;
+            w8[1] = q00 * cur.wp;
+            w8[2] = q01 * ap;
+            w8[3] = q01 * cur.wp;
+            w8[4] = q10 * ap;
+            w8[5] = q10 * cur.wp;
+            w8[6] = q11 * ap;
+            w8[7] = q11 * cur.wp;
+        }
+        float acc = w8[0] * cur.v[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) acc = acc + w8[q] * cur.v[q];
+        float cf[kCoeffs];
+#pragma unroll
+        for (int j = 0; j < kCoeffs; ++j)
+            cf[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, acc), j));
+        segment_transfer<float, EXACT>(s, cur.ds, cf);
+        ++samples;
+    }
+    s.r = cur.r;
+    s.th = cur.th;
+    s.ph = cur.ph;
+    s.pr = cur.pr;
+    s.pth = cur.pth;
+    s.steps += 1;
+    ++steps;
+    code = end_code(P, s, (cur.flags & 1u) != 0, (cur.flags & 2u) != 0, (cur.flags & 4u) != 0);
+    if (code >= 0 || !have_next) {   // the ray ended, or the helper stopped answering
+        if (lane == 0) lds_store(&c->stop, 1);
+        return true;
+    }
+    ++next;
+    return false;
+}
+
 // Lone-wave side: take the helper's steps until the ray ends.  Returns the end code, or -1 after a
-// time-out (s then holds the ray after the last step taken, and the lone loop continues alone).
+// time-out (s then holds the ray after the last step taken, and the lone loop continues alone).  Two
+// step buffers take turns as current and next, so no step's values are copied.
 template <int STORE, bool EXACT>
 __device__ __forceinline__ int coop_consume(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
-                                            RayState<float> &s, CoopLds *c, int lane, unsigned &steps,
-                                            unsigned &samples) {
-    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+                                            const float *gr, const float *gt, const float *gp, RayState<float> &s,
+                                            CoopLds *c, int lane, unsigned &steps, unsigned &samples) {
     const unsigned k = min(unsigned(lane), unsigned(kCoeffs - 1));
-    unsigned tail = 0;
+    unsigned next = 0;   // ring slots taken
+    CoopStep a, b;
+    const bool ok = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, a);
+    coop_loads<STORE>(rs, (ok && a.cell != ~0u) ? a.cell : A.brick_base, k, a);
+    if (!ok) {
+        if (lane == 0) lds_store(&c->stop, 1);
+        return -1;
+    }
+    ++next;
+    int code = -1;
     for (;;) {
-        bool timed_out = false;
-        lds_wait_relaxed(&c->head, [&](unsigned h) { return h != tail; }, timed_out);
-        if (timed_out) {
-            if (lane == 0) lds_store(&c->stop, 1);
-            return -1;
-        }
-        const CoopSlot sl = c->ring[tail & unsigned(kCoopRing - 1)];
-        s.clamped = s.clamped || (sl.flags & 8u);
-        if (sl.cell != ~0u) {
-            float v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, sl.cell + (unsigned(q * kCoeffs) + k) * esz, 0);
-            float w8[8];
-            {
-                const float ar = one_minus(sl.wr), at = one_minus(sl.wt), ap = one_minus(sl.wp);
-                const float q00 = ar * at, q01 = ar * sl.wt, q10 = sl.wr * at, q11 = sl.wr * sl.wt;
-                w8[0] = q00 * ap;
-                w8[1] = q00 * sl.wp;
-                w8[2] = q01 * ap;
-                w8[3] = q01 * sl.wp;
-                w8[4] = q10 * ap;
-                w8[5] = q10 * sl.wp;
-                w8[6] = q11 * ap;
-                w8[7] = q11 * sl.wp;
-            }
-            float acc = w8[0] * v[0];
-#pragma unroll
-            for (int q = 1; q < 8; ++q) acc = acc + w8[q] * v[q];
-            float cf[kCoeffs];
-#pragma unroll
-            for (int j = 0; j < kCoeffs; ++j)
-                cf[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, acc), j));
-            segment_transfer<float, EXACT>(s, sl.ds, cf);
-            ++samples;
-        }
-        s.r = sl.r;
-        s.th = sl.th;
-        s.ph = sl.ph;
-        s.pr = sl.pr;
-        s.pth = sl.pth;
-        s.steps += 1;
-        ++steps;
-        ++tail;
-        if (lane == 0) lds_publish(&c->tail, tail);
-        const int code = end_code(P, s, (sl.flags & 1u) != 0, (sl.flags & 2u) != 0, (sl.flags & 4u) != 0);
-        if (code >= 0) {
-            if (lane == 0) lds_store(&c->stop, 1);
-            return code;
-        }
+        if (coop_step<STORE, EXACT>(A, P, rs, gr, gt, gp, s, c, lane, k, next, a, b, steps, samples, code)) return code;
+        if (coop_step<STORE, EXACT>(A, P, rs, gr, gt, gp, s, c, lane, k, next, b, a, steps, samples, code)) return code;
     }
 }
 
@@ -1126,7 +1250,7 @@ __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R>
                 }
                 helped = true;
                 coop_on = false;
-                code = coop_consume<STORE, EXACT>(A, P, rs, s, coop, lane, steps, samples);
+                code = coop_consume<STORE, EXACT>(A, P, rs, gr, gt, gp, s, coop, lane, steps, samples);
                 if (code >= 0) break;
                 timed_out = true;   // finish alone from the last step received
             }
@@ -1155,7 +1279,7 @@ __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R>
 
 // Helper side of the handshake, for a wave with no rays of its own.  Returns true when it helped (or
 // took a request that was then withdrawn).
-template <int STORE, bool MIXED>
+template <int STORE, bool MIXED, bool EXACT>
 __device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<float> &P, const float *gr,
                                           const float *gt, const float *gp, CoopLds *c, int lane, int wave,
                                           unsigned long long &iters) {
@@ -1166,7 +1290,7 @@ __device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<f
     bool timed_out = false;
     const int go = coop_word(kCoopGo, requester), cancel = coop_word(kCoopCancel, requester);
     const int st = lds_wait(&c->state, [go, cancel](int v) { return v == go || v == cancel; }, timed_out);
-    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED>(A, P, gr, gt, gp, c, lane);
+    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED, EXACT>(A, P, gr, gt, gp, c, lane);
     // leave: Go / Cancel -> Free (after a time-out the lone wave sees no Free and stops using the slot)
     if (lane == 0 && !timed_out) lds_store(&c->state, coop_word(kCoopFree, requester));
     return true;
@@ -1220,7 +1344,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const int n_idle = __popcll(idle);
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             // a wave with no rays first serves a lone ray of its workgroup that asks for help
-            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED, EXACT>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
         }
         if (exhausted != all_q && (n_idle >= A.refill_min || n_idle == 64)) {
             for (int t = 0; t < A.queues && idle; ++t) {
@@ -1268,7 +1392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
                     if (A.coop_steps > 0) {
                         unsigned spins = 0;
                         while (lds_peek(&coop.tracing) > 0 && spins < kCoopSpin) {
-                            if (coop_help<STORE, MIXED>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+                            if (coop_help<STORE, MIXED, EXACT>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
                             __builtin_amdgcn_s_sleep(2);
                             ++spins;
                         }
