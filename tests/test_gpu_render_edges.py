@@ -59,12 +59,15 @@ def test_tuning_does_not_change_the_image(renderer, storage, layout, precision):
         renderer.release(db)
 
 
-@pytest.mark.parametrize("storage,precision,size", [("f32", PREC_FP32, 1024), ("f32", PREC_FP32, 512),
-                                                   ("bf16", PREC_MIXED, 1024)])
-def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size):
+@pytest.mark.parametrize("storage,precision,size,exact", [("f32", PREC_FP32, 1024, False), ("f32", PREC_FP32, 512, False),
+                                                         ("bf16", PREC_MIXED, 1024, False), ("f32", PREC_FP32, 1024, True),
+                                                         ("f16", PREC_MIXED, 512, True)])
+def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size, exact):
     """A lone ray whose geodesic an idle wave takes over (coop_steps > 0) ends on the same step with the same
-    status as the lone-wave path, and its Stokes vector agrees to rounding."""
+    status as the lone-wave path, and its Stokes vector agrees to rounding.  With the exact Stokes step the
+    helper also does the whole cell lookup."""
     sc, br = scenes.scale_plain_scene(size, size), scenes.scale_brick()
+    sc = replace(sc, exact_step=exact)
     db = renderer.upload(br, storage, "corner")
     saved = renderer.get_tuning(precision)
     try:
@@ -76,7 +79,7 @@ def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size):
             out = renderer.render(sc, db, precision)
             split, timeouts = renderer.last_coop()
             d = np.abs(out["iquv"].astype(np.float64) - base["iquv"]) / np.abs(base["iquv"][..., 0]).max()
-            print(f"{storage} {size}: coop_steps {cs}: {split} split rays, {timeouts} time-outs, max |dIQUV| / max I {d.max():.1e}")
+            print(f"{storage} {size} exact={exact}: coop_steps {cs}: {split} split rays, {timeouts} time-outs, max |dIQUV| / max I {d.max():.1e}")
             assert split > 0 and timeouts == 0
             np.testing.assert_array_equal(out["status"], base["status"])
             np.testing.assert_array_equal(out["steps"], base["steps"])
