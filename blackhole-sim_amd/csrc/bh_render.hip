@@ -186,13 +186,56 @@ struct RayK {
     R a, a2, E, L, L2, a2E2, twoaEL, aE, aL, fourE;
 };
 
-// d/dlambda (r, theta, phi, p_r, p_theta) of the Kerr null Hamiltonian.
-template <typename R>
+// FP64 math for the islands of the mixed kernels, cheaper than the library's and still within a few ulp
+// (the FP64 reference kernels keep the library's).
+// sin and cos of x in [0, pi]: x = k pi/2 + t with k = rint(2x/pi) in {0, 1, 2}, pi/2 taken as a 33-bit head
+// (so k * head is exact) plus the double nearest the rest, |t| <= pi/4 + 2^-50; then the Taylor series of
+// sin t to t^17 and of cos t to t^18 (truncation below 1e-19 for |t| <= pi/4), coefficients +-1/n!.
+__device__ __forceinline__ void island_sincos(double x, double &s, double &c) {
+    const double k = __builtin_rint(x * 0.63661977236758134);
+    double t = __builtin_fma(-k, 0x1.921fb544p+0, x);
+    t = __builtin_fma(-k, 0x1.0b4611a626331p-34, t);
+    const double t2 = t * t;
+    double p = 2.8114572543455206e-15;           // 1/17!
+    p = __builtin_fma(p, t2, -7.647163731819816e-13);   // -1/15!
+    p = __builtin_fma(p, t2, 1.6059043836821613e-10);   // 1/13!
+    p = __builtin_fma(p, t2, -2.505210838544172e-08);   // -1/11!
+    p = __builtin_fma(p, t2, 2.7557319223985893e-06);   // 1/9!
+    p = __builtin_fma(p, t2, -0.0001984126984126984);   // -1/7!
+    p = __builtin_fma(p, t2, 0.008333333333333333);     // 1/5!
+    p = __builtin_fma(p, t2, -0.16666666666666666);     // -1/3!
+    const double st = __builtin_fma(t * t2, p, t);
+    double q = -1.5619206968586225e-16;          // -1/18!
+    q = __builtin_fma(q, t2, 4.779477332387385e-14);    // 1/16!
+    q = __builtin_fma(q, t2, -1.1470745597729725e-11);  // -1/14!
+    q = __builtin_fma(q, t2, 2.08767569878681e-09);     // 1/12!
+    q = __builtin_fma(q, t2, -2.755731922398589e-07);   // -1/10!
+    q = __builtin_fma(q, t2, 2.48015873015873e-05);     // 1/8!
+    q = __builtin_fma(q, t2, -0.001388888888888889);    // -1/6!
+    q = __builtin_fma(q, t2, 0.041666666666666664);     // 1/4!
+    q = __builtin_fma(q, t2, -0.5);                     // -1/2!
+    const double ct = __builtin_fma(t2, q, 1.0);
+    // quadrant k: (sin, cos) = (st, ct), (ct, -st), (-st, -ct)
+    s = k == 0.0 ? st : (k == 1.0 ? ct : -st);
+    c = k == 0.0 ? ct : (k == 1.0 ? -st : -ct);
+}
+
+// 1/x from the hardware estimate and two Newton steps (each squares the relative error).
+__device__ __forceinline__ double island_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
+// d/dlambda (r, theta, phi, p_r, p_theta) of the Kerr null Hamiltonian.  ISLAND: FP64 inside the islands
+// of a mixed kernel, with the island math above.
+template <typename R, bool ISLAND = false>
 __device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
                                          R &dpth) {
     const R th = clamp_theta(th_in);
     R s, c;
-    Mth<R>::sincos(th, s, c);
+    if constexpr (ISLAND) island_sincos(th, s, c);
+    else Mth<R>::sincos(th, s, c);
     R s2 = s * s;
     s2 = (s2 < R(kEpsTheta * kEpsTheta)) ? R(kEpsTheta * kEpsTheta) : s2;   // kerr.py:93 floor
     const R r2 = r * r;
@@ -201,10 +244,12 @@ __device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R
     const R dlt = r2a2 - R(2) * r;
     const R P = r2a2 * k.E - k.aL;
     R inv_s2, inv_sig, inv_dlt;
-    if constexpr (sizeof(R) == 4) {
+    if constexpr (sizeof(R) == 4 || ISLAND) {
         // one reciprocal of Sigma * Delta * sin^2 serves all three (|product| stays well inside FP32 range)
         const R sd = sig * dlt;
-        const R inv = Mth<R>::rcp(sd * s2);
+        R inv;
+        if constexpr (ISLAND) inv = island_rcp(sd * s2);
+        else inv = Mth<R>::rcp(sd * s2);
         inv_s2 = sd * inv;
         inv_sig = (dlt * s2) * inv;
         inv_dlt = (sig * s2) * inv;
@@ -716,15 +761,15 @@ __device__ __forceinline__ RayK<G> make_k(double a, double E, double L) {
 
 // RK2 midpoint step of the geodesic in precision G: k1 = f(y), k2 = f(y + h/2 k1), y + h k2.
 // Returns the raw (unclamped) end theta in nt_raw and the midpoint theta in mt.
-template <typename G>
+template <typename G, bool ISLAND = false>
 __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
                                          G &npr, G &npt, G &mt) {
     G k1r, k1t, k1p, k1pr, k1pt;
-    kerr_rhs(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    kerr_rhs<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
     const G hh = G(0.5) * h;
     mt = th + hh * k1t;
     G k2r, k2t, k2p, k2pr, k2pt;
-    kerr_rhs(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    kerr_rhs<G, ISLAND>(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
     nr = r + h * k2r;
     nt_raw = th + h * k2t;
     np = ph + h * k2p;
@@ -757,7 +802,7 @@ __device__ __forceinline__ GeoPoint<R> geo_next(const RenderArgs &A, const HotPa
         const double hd = A.adaptive ? A.step * (double(r) - A.r_plus) : A.step;
         const RayK<double> kd = make_k<double>(A.a, s.Ed, s.Ld);
         double dr_, dt_, dp_, dpr_, dpt_, dm_;
-        geo_step<double>(kd, double(r), double(th), double(ph), double(pr), double(pth), hd, dr_, dt_, dp_, dpr_, dpt_,
+        geo_step<double, true>(kd, double(r), double(th), double(ph), double(pr), double(pth), hd, dr_, dt_, dp_, dpr_, dpt_,
                          dm_);
         nr = R(dr_);
         nt_raw = R(dt_);
