@@ -89,6 +89,31 @@ def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size, exa
         renderer.release(db)
 
 
+@pytest.mark.parametrize("block_threads", [64, 128])
+def test_split_with_small_workgroups(renderer, block_threads):
+    """With one wave per workgroup no helper exists: lone rays ask, nobody accepts, the request is withdrawn
+    and the image is the lone-ray image; with two waves the split still happens."""
+    sc, br = scenes.scale_plain_scene(512, 512), scenes.scale_brick()
+    db = renderer.upload(br, "f32", "corner")
+    saved = renderer.get_tuning(PREC_FP32)
+    try:
+        renderer.set_tuning(PREC_FP32, coop_steps=0)
+        base = renderer.render(sc, db, PREC_FP32)
+        renderer.set_tuning(PREC_FP32, block_threads=block_threads, coop_steps=8)
+        out = renderer.render(sc, db, PREC_FP32)
+        split, timeouts = renderer.last_coop()
+        print(f"block_threads {block_threads}: {split} split rays, {timeouts} time-outs")
+        assert timeouts == 0
+        assert split == 0 if block_threads == 64 else split > 0
+        np.testing.assert_array_equal(out["status"], base["status"])
+        np.testing.assert_array_equal(out["steps"], base["steps"])
+        d = np.abs(out["iquv"].astype(np.float64) - base["iquv"]) / np.abs(base["iquv"][..., 0]).max()
+        assert d.max() <= 1e-6
+    finally:
+        renderer.set_tuning(PREC_FP32, **saved)
+        renderer.release(db)
+
+
 def test_autotuned_batch_gives_the_same_images(renderer):
     from bhgrrt import autotune
 
