@@ -950,8 +950,10 @@ __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R>
 // weights, samples the brick, updates Stokes and the optical depth, and decides where the ray ends.
 // It starts the next step's brick loads before it integrates the current step, so the loads are in
 // flight while it works.  A lone step is bound by the issue of its dependent instructions, and the
-// two waves each issue about half of them.  Every wait is bounded: a helper that stops answering
-// makes the lone wave finish the ray alone from the last step it took (counted in work[4]).
+// two waves each issue about half of them.  With the exact Stokes step, whose matrix exponential
+// lengthens the lone wave's share, the helper does the whole cell lookup instead.  Every wait is
+// bounded: a helper that stops answering makes the lone wave finish the ray alone from the last step
+// it took (counted in work[4]).
 constexpr int kCoopRing = 32;
 constexpr unsigned kCoopSpin = 1u << 21;   // ~0.1 s of s_sleep(1) polls
 // The hand-over is one word, phase | requesting wave << 8, so every transition is a single
