@@ -330,6 +330,18 @@ def main(argv=None) -> int:
     soa = time_path(rr, scene, db32, PREC_FP32, bufs, args.steps, args.warmup)
     paths["plain_fp32_soa"] = {"mrays_s": n / (soa["ms_per_step"] * 1e-3) / 1e6, "ms_per_render": soa["ms_per_step"],
                                "kernel_ms": soa["kernel_ms"], "layout": "soa"}
+    # the small plain scene (576 rays, 64 fixed steps): a latency figure, one launch per render
+    log("[bench] small plain scene (32x18, 64 fixed RK2 steps, 8x6x8 brick)")
+    s_scene, s_brick = scenes.small_plain_scene(), scenes.small_brick()
+    dbs32, dbs64 = rr.upload(s_brick, "f32", "corner"), rr.upload(s_brick, "f32")
+    sm32 = time_path(rr, s_scene, dbs32, PREC_FP32, bufs, args.steps, args.warmup)
+    sm64 = time_path(rr, s_scene, dbs64, PREC_FP64, bufs, args.steps, args.warmup)
+    paths["small_plain"] = {"rays": s_scene.rays, "fp32_kernel_ms": sm32["kernel_ms"],
+                            "fp32_ms_per_render": sm32["ms_per_step"], "fp64_kernel_ms": sm64["kernel_ms"],
+                            "fp64_ms_per_render": sm64["ms_per_step"]}
+    log(f"[bench] small plain: {paths['small_plain']}")
+    rr.release(dbs32)
+    rr.release(dbs64)
     for storage in ("bf16", "f16"):
         log(f"[bench] mixed precision path, {storage} brick")
         dbx = rr.upload(brick, storage, "corner")
