@@ -457,6 +457,27 @@ def main(argv=None) -> int:
     paths["coefficient_precompute"] = precompute_path(rr, dims, args.steps, 0 if args.no_cpu else 64)
     log(f"[bench] precompute: {paths['coefficient_precompute']}")
 
+    # ---- brick files: the reference's npz schema, written, read back and uploaded ------------
+    import tempfile
+
+    log("[bench] brick file round trip (npz, 128x64x128x11 float32)")
+    with tempfile.TemporaryDirectory() as tmp:
+        fpath = Path(tmp) / "brick.npz"
+        t0 = time.perf_counter()
+        brick.save_npz(fpath)
+        t1 = time.perf_counter()
+        back = scenes.Brick.load_npz(fpath)
+        t2 = time.perf_counter()
+        dbf = rr.upload(back, "f32", "corner")
+        rr.synchronize()
+        t3 = time.perf_counter()
+        rr.release(dbf)
+        mb = brick.coeffs.nbytes / 1e6
+        paths["brick_files"] = {"coeff_MB": mb, "file_MB": fpath.stat().st_size / 1e6,
+                                "save_s": t1 - t0, "load_s": t2 - t1, "upload_corner_s": t3 - t2,
+                                "round_trip_equal": bool(np.array_equal(back.coeffs, brick.coeffs))}
+    log(f"[bench] brick files: {paths['brick_files']}")
+
     # ---- learned surrogate + mixed precision --------------------------------------------
     if not args.no_surrogate:
         paths["surrogate_mixed_bf16"] = surrogate_path(rr, scene, brick, ref, I64, args)
