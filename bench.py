@@ -418,7 +418,7 @@ def main(argv=None) -> int:
     I32b = np.concatenate([o["iquv"][..., 0].ravel() for o in batch32]).astype(np.float64)
     paths["batched_fp32"]["rel_l1_I"] = rel_l1(I32b[ok], I64b[ok])
 
-    log("[bench] configs[3]-style batch: 64 renders, 128^2..1024^2, random spin and inclination, 4 bf16 bricks")
+    log("[bench] 64-render batch: 128^2..1024^2, random spin and inclination, 4 bf16 bricks")
     bricks = [rr.upload(b, "bf16", "corner") for b in scenes.batch_bricks()]
     jobs = [(sc, bricks[k]) for sc, k in scenes.batch_scenes(64)]
     bt = Batch(rr, jobs, PREC_MIXED)
