@@ -96,20 +96,25 @@ def cpu_baseline(scene, brick, seconds: float) -> dict:
 
 
 def time_path(rr, scene, db, precision, bufs, steps: int, warmup: int) -> dict:
+    """Wall time per render over `steps` renders enqueued back to back (one synchronisation at the end),
+    then the kernel's own time (HIP events) and work counters over as many renders, read after each."""
     d_iquv, d_steps, d_status = bufs
     for _ in range(warmup):
         rr.render_device(scene, db, precision, d_iquv, d_steps, d_status)
         rr.synchronize()
-    kms = []
-    work = []
     rr.synchronize()
     t0 = time.perf_counter()
+    for _ in range(steps):
+        rr.render_device(scene, db, precision, d_iquv, d_steps, d_status)
+    rr.synchronize()
+    t1 = time.perf_counter()
+    kms = []
+    work = []
     for _ in range(steps):
         rr.render_device(scene, db, precision, d_iquv, d_steps, d_status)
         kms.append(rr.last_kernel_ms())
         work.append(rr.last_work())
     rr.synchronize()
-    t1 = time.perf_counter()
     return {"ms_per_step": (t1 - t0) * 1e3 / steps, "kernel_ms": float(np.mean(kms)), "work": work[-1]}
 
 
