@@ -17,8 +17,8 @@
 //  * The Lorentz-factor loop is the outer loop: each gamma's distribution value is formed once and
 //    the trapezoid sums of all nodes advance together (the thermal part takes two passes: its
 //    normalisation integral, then the emissivities).
-//  * K_nu (Temme's series for x <= 2, Steed's continued fraction above) gives the x K_{2/3}(x)
-//    table on the host and the K_2 normalisation of the Maxwell-Juttner distribution per cell.
+//  * K_nu by the trapezoidal rule on its cosh integral gives the x K_{2/3}(x) table on the host and
+//    the K_2 normalisation of the Maxwell-Juttner distribution per cell.
 
 #include "bh_internal.h"
 
@@ -31,7 +31,6 @@ namespace bh {
 namespace syn {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr double kEuler = 0.57721566490153286061;
 // calibration.py:18-26 and synchrotron.py:200 (cgs)
 constexpr double kC = 2.99792458e10;
 constexpr double kMp = 1.67262192369e-24;
@@ -45,125 +44,85 @@ constexpr int kMaxNodes = 16;       // Gauss-Legendre pitch nodes per model
 constexpr int kMaxUnique = (kMaxNodes + 1) / 2;
 constexpr int kMaxGamma = 1024;     // Lorentz-factor grid points per model
 
-// Temme's combinations of Gamma functions at mu (|mu| <= 1/2):
-// gam1 = (1/G(1-mu) - 1/G(1+mu)) / (2 mu), gam2 = (1/G(1-mu) + 1/G(1+mu)) / 2, gampl = 1/G(1+mu), gammi = 1/G(1-mu).
-struct GammaTerms {
-    double gam1, gam2, gampl, gammi;
-};
-
-// K_mu(x) and K_{mu+1}(x), |mu| <= 1/2, x > 0: Temme's series (x <= 2) or Steed's continued fraction
-// for the ratio K_{mu+1}/K_mu with Temme's normalisation sum (x > 2) -- N. M. Temme, J. Comput. Phys. 19
-// (1975) 324; I. J. Thompson and A. R. Barnett, J. Comput. Phys. 64 (1986) 490.
-__host__ __device__ inline void bessel_k_pair(double mu, const GammaTerms &g, double x, double &kmu, double &kmu1) {
-    constexpr double kEps = 1.0e-16;
-    constexpr int kMaxIt = 20000;
-    if (x <= 2.0) {
-        const double x2 = 0.5 * x;
-        const double pimu = kPi * mu;
-        const double fact = fabs(pimu) < kEps ? 1.0 : pimu / sin(pimu);
-        double d = -log(x2);
-        double e = mu * d;
-        const double fact2 = fabs(e) < kEps ? 1.0 : sinh(e) / e;
-        double ff = fact * (g.gam1 * cosh(e) + g.gam2 * fact2 * d);
-        double sum = ff;
-        e = exp(e);
-        double p = 0.5 * e / g.gampl;
-        double q = 0.5 / (e * g.gammi);
-        double c = 1.0;
-        d = x2 * x2;
-        double sum1 = p;
-        for (int i = 1; i <= kMaxIt; ++i) {
-            const double di = double(i);
-            ff = (di * ff + p + q) / (di * di - mu * mu);
-            c *= d / di;
-            p /= di - mu;
-            q /= di + mu;
-            const double del = c * ff;
-            sum += del;
-            sum1 += c * (p - di * ff);
-            if (fabs(del) < fabs(sum) * kEps) break;
-        }
-        kmu = sum;
-        kmu1 = sum1 * (2.0 / x);
-    } else {
-        double b = 2.0 * (1.0 + x);
-        double d = 1.0 / b;
-        double h = d, delh = d;
-        double q1 = 0.0, q2 = 1.0;
-        const double a1 = 0.25 - mu * mu;
-        double q = a1, c = a1, a = -a1;
-        double s = 1.0 + q * delh;
-        for (int i = 2; i <= kMaxIt; ++i) {
-            a -= 2.0 * double(i - 1);
-            c = -a * c / double(i);
-            const double qnew = (q1 - b * q2) / a;
-            q1 = q2;
-            q2 = qnew;
-            q += c * qnew;
-            b += 2.0;
-            d = 1.0 / (b + a * d);
-            delh = (b * d - 1.0) * delh;
-            h += delh;
-            const double dels = q * delh;
-            s += dels;
-            if (fabs(dels / s) < kEps) break;
-        }
-        h = a1 * h;
-        kmu = sqrt(kPi / (2.0 * x)) * exp(-x) / s;
-        kmu1 = kmu * (mu + x + 0.5 - h) / x;
+// K_nu(x) for real nu >= 0 and x > 0 from the integral representation (DLMF 10.32.9)
+//     K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt,
+// summed by the trapezoidal rule.  The integrand is even in t and analytic in the strip
+// |Im t| < pi/2, so the rule's error falls like exp(-2 pi d / h) for any d inside the strip
+// (Trefethen & Weideman, SIAM Review 56 (2014) 385, section 5); for large x the integrand is close
+// to a Gaussian of width 1/sqrt(x), whose trapezoid error is exp(-2 pi^2 / (x h^2)).  The step
+// h = min(1/8, 1/(2 sqrt x)) puts both exponents beyond 60.  The factor exp(-x) is taken out of the
+// sum and cosh t - 1 is formed as 2 sinh(t/2)^2 with expm1, so no term loses digits to cancellation.
+// The integrand rises to one peak and then falls (its log-derivative -x sinh t + nu tanh(nu t) changes
+// sign once), so the sum stops on the falling side once a term is below 1e-18 of it.  exp(-x)
+// underflows for x > 745 and K_nu is then 0, as scipy.special.kv returns.
+__host__ __device__ inline double bessel_k(double nu, double x) {
+    const double scale = exp(-x);
+    if (scale == 0.0) return 0.0;
+    const double h = fmin(0.125, 0.5 / sqrt(x));
+    double sum = 0.5;   // the t = 0 node, half weight: exp(0) cosh(0)
+    double carry = 0.0; // compensated summation
+    double last = 1.0;
+    for (int k = 1; k < 16384; ++k) {
+        const double t = double(k) * h;
+        const double em = expm1(0.5 * t);              // e^{t/2} - 1
+        const double sh = 0.5 * (em + em / (1.0 + em)); // sinh(t/2)
+        const double en = exp(nu * t);
+        const double term = exp(-2.0 * x * sh * sh) * 0.5 * (en + 1.0 / en);
+        const double y = term - carry;
+        const double next = sum + y;
+        carry = (next - sum) - y;
+        sum = next;
+        if (term <= last && term < 1.0e-18 * sum) break;
+        last = term;
     }
+    return scale * h * sum;
 }
 
-// K_nu(x) for nu = nl + mu (nl >= 0 an integer) by upward recurrence from K_mu, K_{mu+1}.
-__host__ __device__ inline double bessel_k(double mu, int nl, const GammaTerms &g, double x) {
-    double k0, k1;
-    bessel_k_pair(mu, g, x, k0, k1);
-    for (int i = 1; i <= nl; ++i) {
-        const double kn = (mu + double(i)) * (2.0 / x) * k1 + k0;
-        k0 = k1;
-        k1 = kn;
+// Legendre polynomial P_n(x) and its derivative P_n'(x) in x87 extended precision, from the
+// three-term recurrence (k+1) P_{k+1} = (2k+1) x P_k - k P_{k-1} and the derivative identity
+// P'_{k+1} = P'_{k-1} + (2k+1) P_k (DLMF 18.9.1 and 18.9.19), with no division by 1 - x^2.
+static void legendre_eval(int n, long double x, long double &pn, long double &dpn) {
+    long double p_prev = 1.0L, p = x;     // P_0, P_1
+    long double d_prev = 0.0L, d = 1.0L;  // P_0', P_1'
+    for (int k = 1; k < n; ++k) {
+        const long double p_next = ((2 * k + 1) * x * p - k * p_prev) / (k + 1);
+        const long double d_next = d_prev + (2 * k + 1) * p;
+        p_prev = p;
+        p = p_next;
+        d_prev = d;
+        d = d_next;
     }
-    return k0;
+    pn = p;
+    dpn = d;
 }
 
-constexpr GammaTerms kIntegerOrder{-kEuler, 1.0, 1.0, 1.0};   // mu = 0
-
-static GammaTerms gamma_terms(double mu) {
-    if (mu == 0.0) return kIntegerOrder;
-    const double gampl = 1.0 / std::tgamma(1.0 + mu), gammi = 1.0 / std::tgamma(1.0 - mu);
-    return {(gammi - gampl) / (2.0 * mu), 0.5 * (gammi + gampl), gampl, gammi};
-}
-
-// Gauss-Legendre nodes (ascending) and weights on [-1, 1]: Newton's method on P_n from the
-// asymptotic root estimates, mirrored so that x[i] = -x[n-1-i] and w[i] = w[n-1-i] exactly.
+// Gauss-Legendre nodes (ascending) and weights on [-1, 1].  The positive roots of P_n start from
+// Tricomi's asymptotic estimate x_k = (1 - (n - 1) / (8 n^3)) cos(pi (4k - 1) / (4n + 2))
+// (F. G. Tricomi, 1950; see N. Hale & A. Townsend, SIAM J. Sci. Comput. 35 (2013) A652) and are
+// polished by Newton steps on P_n / P_n' until the update stops shrinking; the weight is
+// w = 2 / ((1 - x^2) P_n'(x)^2) (Abramowitz & Stegun 25.4.29).  Both are formed in extended precision
+// (64-bit significand), so the doubles they round to are within an ulp of the exact values: in double
+// precision the weights of the outer nodes would inherit the node's rounding error amplified by
+// 2 x / (1 - x^2).  Nodes are mirrored so that x[i] = -x[n-1-i] and w[i] = w[n-1-i] exactly.
 static void gauss_legendre(int n, double *x, double *w) {
-    const int m = (n + 1) / 2;
-    for (int i = 0; i < m; ++i) {
-        double z = std::cos(kPi * (double(i) + 0.75) / (double(n) + 0.5));
-        double pp = 1.0;
-        for (int it = 0; it < 100; ++it) {
-            double p1 = 1.0, p2 = 0.0;
-            for (int j = 1; j <= n; ++j) {
-                const double p3 = p2;
-                p2 = p1;
-                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
-            }
-            pp = n * (z * p1 - p2) / ((z - 1.0) * (z + 1.0));
-            const double z1 = z;
-            z = z1 - p1 / pp;
-            if (std::fabs(z - z1) <= 1.0e-16 * std::fabs(z) || std::fabs(z - z1) < 1.0e-300) break;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const long double shrink = 1.0L - (long double)(n - 1) / (8.0L * n * n * n);
+    for (int k = 1; k <= (n + 1) / 2; ++k) {   // k-th largest root (the middle one of odd n is 0)
+        long double z = (2 * k - 1 == n) ? 0.0L : shrink * cosl(pi * (4 * k - 1) / (4 * n + 2));
+        long double pn, dpn, prev_step = 1.0L;
+        for (int it = 0; it < 64 && z != 0.0L; ++it) {
+            legendre_eval(n, z, pn, dpn);
+            const long double step = pn / dpn;
+            z -= step;
+            const long double mag = fabsl(step);
+            if (mag >= prev_step || mag < 1.0e-21L) break;   // rounding-bound, or converged
+            prev_step = mag;
         }
-        if ((n & 1) && i == m - 1) z = 0.0;
-        double p1 = 1.0, p2 = 0.0;   // derivative at the final root
-        for (int j = 1; j <= n; ++j) {
-            const double p3 = p2;
-            p2 = p1;
-            p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
-        }
-        pp = n * (z * p1 - p2) / ((z - 1.0) * (z + 1.0));
-        x[i] = -z;
-        x[n - 1 - i] = z;
-        w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z) * (1.0 + z) * pp * pp);
+        legendre_eval(n, z, pn, dpn);
+        const long double wk = 2.0L / (((1.0L - z) * (1.0L + z)) * dpn * dpn);
+        x[k - 1] = -double(z);
+        x[n - k] = double(z);   // after the mirror, so the middle node of odd n is +0
+        w[n - k] = w[k - 1] = double(wk);
     }
 }
 
@@ -333,7 +292,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
         const int ng = A.th_ng;
         const double step = (l1 - l0) / double(ng - 1);
         const double tq = pymax(th, 1.0e-12);
-        const double norm = th * bessel_k(0.0, 2, kIntegerOrder, 1.0 / tq);
+        const double norm = th * bessel_k(2.0, 1.0 / tq);
         const double norm_den = pymax(norm, 1.0e-300);
         auto gamma_at = [&](int k) {   // 1 + geomspace(1e-5, gmax - 1, ng)[k], as 10^y = exp(y ln 10)
 #pragma clang fp contract(off)
@@ -457,8 +416,6 @@ static int prepare(const bh_synchrotron_desc *d, int out_dtype, std::vector<doub
     const int ng = d->nt_gamma_grid_size;
     tab.assign(size_t(kOffNtGamma + 2 * ng), 0.0);
     // kernel table (synchrotron.py:158-163): F by the closed form, G = x K_{2/3}(x)
-    const double mu23 = 2.0 / 3.0 - 1.0;
-    const GammaTerms g23 = gamma_terms(mu23);
     const double la = std::log10(1.0e-8), lb = std::log10(1.0e3);
     const double lstep = (lb - la) / double(kTable - 1);
     double lxs[kTable], lfs[kTable], lgs[kTable];
@@ -466,7 +423,7 @@ static int prepare(const bh_synchrotron_desc *d, int out_dtype, std::vector<doub
         double x = i == 0 ? 1.0e-8 : (i == kTable - 1 ? 1.0e3 : std::pow(10.0, double(i) * lstep + la));
         const double xx = std::fmin(std::fmax(x, 1.0e-12), 1.0e3);
         double F = 1.25 * std::pow(xx, 1.0 / 3.0) * std::exp(-xx) * std::pow(648.0 + xx * xx, 1.0 / 12.0);
-        double G = xx * bessel_k(mu23, 1, g23, xx);
+        double G = xx * bessel_k(2.0 / 3.0, xx);
         if (!std::isfinite(F)) F = 0.0;
         if (!std::isfinite(G)) G = 0.0;
         lxs[i] = std::log(x);
@@ -585,11 +542,7 @@ int bh_bessel_k(double nu, double x, double *out) {
     if (!out) return set_error(BH_ERR_VALUE, "out is NULL");
     if (!(nu >= 0.0) || !std::isfinite(nu)) return set_error(BH_ERR_VALUE, "nu must be finite and >= 0");
     if (!(x > 0.0) || !std::isfinite(x)) return set_error(BH_ERR_VALUE, "x must be finite and > 0");
-    const int nl = int(nu + 0.5);
-    const double mu = nu - nl;
-    if (mu != 0.0 && std::fabs(mu) < 1.0e-3)
-        return set_error(BH_ERR_VALUE, "nu must be an integer or at least 1e-3 away from one");
-    *out = syn::bessel_k(mu, nl, syn::gamma_terms(mu), x);
+    *out = syn::bessel_k(nu, x);
     return BH_OK;
 }
 

@@ -350,9 +350,8 @@ int bh_precompute_coefficient_bricks_device(bh_ctx *ctx, const double *d_rho, co
                                             const double *d_b_con, int64_t n_cells,
                                             const bh_synchrotron_desc *desc, int32_t out_dtype, void *d_out);
 /* K_nu(x), the modified Bessel function of the second kind, as scipy.special.kv
- * (synchrotron.py:148, 195), for nu >= 0 an integer or at least 1e-3 from one
- * and x > 0 (Temme's series for x <= 2, Steed's continued fraction above).
- * Host only. */
+ * (synchrotron.py:148, 195), for finite nu >= 0 and finite x > 0 (trapezoidal
+ * rule on K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt).  Host only. */
 int bh_bessel_k(double nu, double x, double *out);
 /* Gauss-Legendre nodes (ascending) and weights on [-1, 1], n in [1, 64], as
  * numpy.polynomial.legendre.leggauss (synchrotron.py:219).  Host only. */

@@ -1,6 +1,6 @@
 """Host-side pieces of the coefficient precompute, without a GPU.
 
-* ``bh_bessel_k`` (Temme's series / Steed's continued fraction) against scipy.special.kv, which the
+* ``bh_bessel_k`` (trapezoidal rule on the cosh integral) against scipy.special.kv, which the
   reference calls for the x K_{2/3}(x) table (synchrotron.py:148) and the Maxwell-Juttner
   normalisation K_2 (synchrotron.py:195), over the argument ranges those calls see.
 * ``bh_gauss_legendre`` against numpy.polynomial.legendre.leggauss (synchrotron.py:219).
@@ -28,7 +28,7 @@ def _kv(nu, x):
     return out.value
 
 
-@pytest.mark.parametrize("nu", [2.0 / 3.0, 2.0, 0.0, 1.0, 5.0 / 3.0, 0.25])
+@pytest.mark.parametrize("nu", [2.0 / 3.0, 2.0, 0.0, 1.0, 5.0 / 3.0, 0.25, 2.0005, 3.5])
 def test_bessel_k_matches_mpmath_and_scipy(nu):
     import mpmath
 
@@ -51,7 +51,7 @@ def test_bessel_k_underflows_to_zero_like_scipy():
 
 
 def test_bessel_k_rejects_bad_arguments():
-    for nu, x in ((-1.0, 1.0), (2.0, 0.0), (2.0, -1.0), (2.0005, 1.0), (float("nan"), 1.0), (1.0, float("inf"))):
+    for nu, x in ((-1.0, 1.0), (2.0, 0.0), (2.0, -1.0), (float("inf"), 1.0), (float("nan"), 1.0), (1.0, float("inf"))):
         with pytest.raises(ValueError):
             _kv(nu, x)
 
