@@ -172,3 +172,80 @@ def test_schwarzschild_critical_impact_parameter():
 def test_critical_curve_schwarzschild_limit():
     al, be = kerr.critical_curve(1e-6, math.radians(60.0))
     np.testing.assert_allclose(np.hypot(al, be), math.sqrt(27.0), rtol=1e-3)
+
+
+def test_camera_orientation_kat():
+    """kerr.py:200-201: x positive is local +phi, y positive is local -theta on the image plane."""
+    cam = Camera.from_degrees(r=60.0, inclination_degrees=70.0, fov_y_degrees=40.0)
+    for a in (0.0, 0.9, -0.6):
+        for ndc_y in (0.4, -0.7):
+            y0, pt, pph = kerr.camera_ray_initial_state(cam, a, 0.0, ndc_y, 1.5)
+            dth = kerr.geodesic_rhs(y0, a, pt, pph)[2]
+            assert np.sign(y0[5]) == -np.sign(ndc_y) and np.sign(dth) == -np.sign(ndc_y)
+        if a == 0.0:   # no frame dragging: d phi / d lambda has the sign of ndc_x
+            for ndc_x in (0.3, -0.8):
+                y0, pt, pph = kerr.camera_ray_initial_state(cam, a, ndc_x, 0.1, 1.5)
+                assert np.sign(pph) == np.sign(ndc_x)
+                assert np.sign(kerr.geodesic_rhs(y0, a, pt, pph)[3]) == np.sign(ndc_x)
+
+
+def _bardeen(pt, pph, th, pth, a, inc):
+    """Bardeen's image coordinates of a ray from its constants: alpha = -xi / sin i,
+    beta = sqrt(eta + a^2 cos^2 i - xi^2 cot^2 i) (xi = L / E, eta = Q / E^2)."""
+    E, L = -pt, pph
+    Q = kerr.carter_constant_vec(th, pth, pt, pph, a)
+    xi, eta = L / E, Q / (E * E)
+    b2 = eta + a * a * math.cos(inc) ** 2 - xi * xi / math.tan(inc) ** 2
+    return -xi / math.sin(inc), math.sqrt(max(b2, 0.0))
+
+
+def test_pixel_offsets_are_bardeen_image_coordinates():
+    """A camera at r = 1000 sees a ray at image-plane offsets (r ndc_x tan_x, r ndc_y tan_y) M; these equal
+    Bardeen's (-alpha, |beta|) from the ray's constants of motion to O(M / r): the image's +x is local
+    +phi (xi = L / E > 0, alpha = -xi / sin i < 0), and beta's sign is opposite to p_theta's (the
+    image's +y is -theta)."""
+    r0 = 1000.0
+    for a, inc_deg in ((0.94, 17.0), (0.998, 60.0), (-0.5, 80.0)):
+        cam = Camera.spanning(r=r0, inclination_degrees=inc_deg, half_width_m=20.0, width=64, height=48)
+        tan_y = math.tan(math.radians(cam.fov_y_degrees) / 2)
+        tan_x = tan_y * 64 / 48
+        rng = np.random.default_rng(5)
+        for nx, ny in zip(rng.uniform(-1, 1, 25), rng.uniform(-1, 1, 25)):
+            y0, pt, pph = kerr.camera_ray_initial_state(cam, a, nx, ny, 64 / 48)
+            al, be = _bardeen(pt, pph, y0[2], y0[5], a, cam.theta)
+            a_pix, b_pix = r0 * nx * tan_x, r0 * ny * tan_y
+            assert abs(-al - a_pix) < 2e-3 * max(1.0, abs(a_pix)), (a, inc_deg, al, a_pix)
+            assert abs(be - abs(b_pix)) < 2e-3 * max(1.0, abs(b_pix)) + 2e-2, (a, inc_deg, be, b_pix)
+            if abs(b_pix) > 0.5:
+                assert np.sign(b_pix) == -np.sign(y0[5])
+
+
+def test_kerr_critical_curve_separates_capture_from_escape():
+    """At a = 0.998, rays launched 0.05 M inside the closed-form critical curve are captured and rays 0.05 M
+    outside escape, by the radial potential of each ray's own constants of motion (textbook turning-point
+    criterion): the curve's position, including the flat prograde side, is pinned, not only its a -> 0 circle."""
+    a, inc, r0 = 0.998, math.radians(60.0), 1000.0
+    al, be = kerr.critical_curve(a, inc, n=4000)
+    cam = Camera.spanning(r=r0, inclination_degrees=60.0, half_width_m=20.0, width=64, height=64)
+    tan = math.tan(math.radians(cam.fov_y_degrees) / 2)
+    centre = np.array([np.mean(al), np.mean(be)])
+    picks = np.linspace(0, al.size - 1, 24).astype(int)
+    checked = 0
+    for k in picks:
+        p = np.array([al[k], be[k]])
+        if abs(p[1]) < 0.3:   # the curve's two halves meet at beta = 0, where the normal is least defined
+            continue
+        k0, k1 = max(k - 3, 0), min(k + 3, al.size - 1)
+        t = np.array([al[k1] - al[k0], be[k1] - be[k0]])
+        nrm = np.array([t[1], -t[0]]) / np.hypot(*t)
+        if np.dot(nrm, p - centre) < 0:
+            nrm = -nrm   # outward
+        for sgn, expect in ((-1.0, True), (1.0, False)):
+            q = p + sgn * 0.05 * nrm
+            # Bardeen (alpha, beta) -> pixel offsets (-alpha, beta)
+            y0, pt, pph = kerr.camera_ray_initial_state(cam, a, -q[0] / (r0 * tan), q[1] / (r0 * tan), 1.0)
+            Q = kerr.carter_constant_vec(y0[2], y0[5], pt, pph, a)
+            cap, _ = _radial_potential_captures(-pt, pph, Q, a, r0)
+            assert cap == expect, (k, p, sgn)
+        checked += 1
+    assert checked >= 16

@@ -127,6 +127,28 @@ def test_status_codes_and_steps_are_consistent():
     code = out["status"] & 7
     assert set(np.unique(code)) <= {1, 2, 4}
     assert np.all(out["steps"] >= 1) and np.all(out["steps"] <= sc.max_steps)
-    # the shadow (captured rays) sits inside the Bardeen critical curve
-    al, be = kerr.critical_curve(sc.spin, sc.camera.theta)
     assert 0 < np.sum(code == 1) < code.size
+    # the shadow (captured rays) is the inside of the Bardeen critical curve: pixels more than 1.5 pixel
+    # widths inside it are captured and pixels that far outside are not (the image's x is -alpha)
+    al, be = kerr.critical_curve(sc.spin, sc.camera.theta, n=4000)
+    px, py = np.meshgrid(np.arange(sc.width), np.arange(sc.height))
+    ndc_x, ndc_y = scenes.pixel_ndc(sc, px, py)
+    tan_y = math.tan(math.radians(sc.camera.fov_y_degrees) / 2)
+    x_m, y_m = -sc.camera.r * ndc_x * tan_y * sc.aspect, sc.camera.r * ndc_y * tan_y
+    inside = _inside_polygon(x_m, y_m, al, be)
+    dist = np.min(np.hypot(x_m[..., None] - al, y_m[..., None] - be), axis=-1)
+    far = dist > 1.5 * 2.0 * sc.camera.r * tan_y * sc.aspect / sc.width
+    ok = (out["status"] & render.ST_POLE_CLAMPED) == 0
+    assert np.sum(far & inside & ok) >= 10 and np.sum(far & ~inside & ok) > 200
+    assert np.all((code == 1)[far & inside & ok]) and not np.any((code == 1)[far & ~inside & ok])
+
+
+def _inside_polygon(x, y, px, py):
+    """Even-odd ray casting of points (x, y) against the closed polygon (px, py)."""
+    inside = np.zeros(np.shape(x), dtype=bool)
+    qx, qy = np.roll(px, 1), np.roll(py, 1)
+    for x0, y0, x1, y1 in zip(px, py, qx, qy):
+        crosses = (y0 > y) != (y1 > y)
+        xc = x0 + (y - y0) * (x1 - x0) / np.where(y1 != y0, y1 - y0, 1.0)
+        inside ^= crosses & (x < xc)
+    return inside
