@@ -122,6 +122,8 @@ SIGNATURES = [
     ("bh_render", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, _i32p, _u8p]),
     ("bh_render_device", C.c_int,
      [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("bh_render_device_noalloc", C.c_int,
+     [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("bh_mlp_create", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     ("bh_mlp_destroy", C.c_int, [C.c_void_p]),
     ("bh_mlp_shape", C.c_int, [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_int64)]),

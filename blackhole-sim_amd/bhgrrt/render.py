@@ -119,6 +119,14 @@ class Renderer:
         _lib.check(self._lib.bh_render_device(self.ctx, C.byref(desc), dbrick.brick_id, int(precision),
                                               _ptr(d_iquv), _ptr(d_steps), _ptr(d_status)))
 
+    def render_device_noalloc(self, scene: Scene, dbrick: DeviceBrick, precision: int, d_iquv, d_steps=None,
+                              d_status=None) -> None:
+        """``render_device`` without device allocation: counters and tile schedule in the library's static
+        per-GPU array (images up to 2048 x 2048); calls overlapping on one GPU race."""
+        desc = scene_desc(scene)
+        _lib.check(self._lib.bh_render_device_noalloc(self.ctx, C.byref(desc), dbrick.brick_id, int(precision),
+                                                      _ptr(d_iquv), _ptr(d_steps), _ptr(d_status)))
+
     def synchronize(self) -> None:
         _lib.check(self._lib.bh_ctx_synchronize(self.ctx))
 

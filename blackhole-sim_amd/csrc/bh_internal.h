@@ -76,6 +76,7 @@ struct bh_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int timed_launches = 0;
     unsigned int *d_counters = nullptr;   // ray-queue counters, kCounterWords words
+    const unsigned int *last_counters = nullptr;   // counter block of the last render launch (null: d_counters)
     std::vector<bh::BrickDev> bricks;
     bh::Tuning tuning[3];
     int *d_order = nullptr;             // tile schedule of the last scene (device)

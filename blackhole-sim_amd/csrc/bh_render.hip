@@ -1692,12 +1692,10 @@ static std::vector<float> tile_distances(const bh_scene_desc &d, const CamConst 
     return dist;
 }
 
-static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y,
-                          int queues, int q_tiles) {
-    std::vector<double> key = {d.spin, d.cam_r, d.cam_theta, d.cam_phi, d.fov_y_degrees, d.roll_degrees,
-                               double(d.width), double(d.height), double(queues)};
+// Queue slots -> tiles: queue q holds slots [q * q_tiles, (q + 1) * q_tiles).
+static std::vector<int> schedule_of(const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y, int queues,
+                                    int q_tiles) {
     const int n_tiles = tiles_x * tiles_y;
-    if (ctx->d_order && ctx->order_key == key) return BH_OK;
     const std::vector<float> dist = tile_distances(d, C, tiles_x, tiles_y);
     std::vector<int> idx(n_tiles);
     for (int t = 0; t < n_tiles; ++t) idx[t] = t;
@@ -1711,6 +1709,15 @@ static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C
     // queue slots past the dealt tiles repeat a tile index that lies outside the image
     for (int q = 0; q < queues; ++q)
         for (int j = fill[q]; j < q_tiles; ++j) order[size_t(q) * q_tiles + j] = n_tiles;
+    return order;
+}
+
+static int build_schedule(bh_ctx *ctx, const bh_scene_desc &d, const CamConst &C, int tiles_x, int tiles_y,
+                          int queues, int q_tiles) {
+    std::vector<double> key = {d.spin, d.cam_r, d.cam_theta, d.cam_phi, d.fov_y_degrees, d.roll_degrees,
+                               double(d.width), double(d.height), double(queues)};
+    if (ctx->d_order && ctx->order_key == key) return BH_OK;
+    const std::vector<int> order = schedule_of(d, C, tiles_x, tiles_y, queues, q_tiles);
     const size_t bytes = order.size() * sizeof(int);
     if (bytes > ctx->order_cap) {
         if (ctx->d_order) (void)hipFree(ctx->d_order);
@@ -1904,11 +1911,43 @@ static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A)
     return BH_OK;
 }
 
+// ---- memory of the no-allocation render ------------------------------------------------------
+// bh_render_device_noalloc keeps a launch's ray-queue counters (kCounterWords words) and its tile schedule
+// (at most kScratchTiles tiles, padded to a multiple of the queue count) in this array instead of memory
+// the context allocates.  Each GPU holds its own copy for the life of the process (257 KiB).  In host
+// code `scratch` names a host-side stand-in, never the device memory: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is handed to kernels and copies.
+constexpr int kScratchTiles = 65536;   // 8x8 tiles: 2048 x 2048 pixels
+constexpr int kScratchWords = kCounterWords + kScratchTiles + kMaxQueues;
+static __device__ __align__(256) unsigned int scratch[kScratchWords];
+static_assert(kScratchTiles == BH_NOALLOC_MAX_TILES && sizeof(scratch) == BH_NOALLOC_SCRATCH_BYTES,
+              "bhgrrt.h states the size of the no-allocation scratch array");
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static unsigned *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the context's device (which ctx_check has made current).
+static int scratch_address(bh_ctx *ctx, unsigned **out) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return set_error(BH_ERR_DEVICE, "device %d out of range", ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[ctx->device]) {
+        void *ptr = nullptr;
+        BH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return set_error(BH_ERR_DEVICE, "no device address for the render scratch array");
+        g_scratch_addr[ctx->device] = static_cast<unsigned *>(ptr);
+    }
+    *out = g_scratch_addr[ctx->device];
+    return BH_OK;
+}
+
 // Validate, set up and enqueue one render launch.  List mode (ray_list non-null): only the pixels
 // ray_list[0 .. *n_list) are traced, n_list read on the device; max_list bounds it for the grid size.
+// scratch (non-null): the launch's counters go to scratch[0, kCounterWords) and its tile schedule right after
+// them, copied from the host, instead of the context's counters and its cached device schedule.
 static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision, void *d_iquv,
                           int32_t *d_steps, uint8_t *d_status, const int *ray_list, const unsigned *n_list,
-                          int max_list, bool timed) {
+                          int max_list, bool timed, unsigned *scratch = nullptr) {
     int st;
     if (!scene || !d_iquv) return set_error(BH_ERR_VALUE, "scene and iquv must not be NULL");
     if ((st = validate_scene(*scene))) return st;
@@ -1924,12 +1963,17 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
     A.iquv = d_iquv;
     A.steps = d_steps;
     A.status = d_status;
-    A.counters = ctx->d_counters;
-    A.work = reinterpret_cast<unsigned long long *>(ctx->d_counters + kWorkOffset);
+    unsigned *counters = scratch ? scratch : ctx->d_counters;
+    A.counters = counters;
+    A.work = reinterpret_cast<unsigned long long *>(counters + kWorkOffset);
     const Tuning &u = ctx->tuning[precision];
     A.tiles_x = (A.W + 7) / 8;
     const int tiles_y = (A.H + 7) / 8;
     const int n_tiles = A.tiles_x * tiles_y;
+    if (scratch && n_tiles > kScratchTiles)
+        return set_error(BH_ERR_VALUE,
+                         "a %d x %d image has %d 8x8 tiles; the no-allocation render holds at most %d (2048 x 2048 "
+                         "pixels)", A.W, A.H, n_tiles, kScratchTiles);
     A.tile_order = nullptr;
     if (ray_list) {
         A.ray_list = ray_list;
@@ -1942,7 +1986,14 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
         const int q_tiles = (n_tiles + A.queues - 1) / A.queues;
         A.q_len = q_tiles * 64;
         A.n_ids = A.queues * A.q_len;   // padded id space: every queue slot is issued; padding maps outside the image
-        if (u.order == 1) {
+        if (u.order == 1 && scratch) {
+            const std::vector<int> order = schedule_of(*scene, A.cam, A.tiles_x, tiles_y, A.queues, q_tiles);
+            int *d_sched = reinterpret_cast<int *>(scratch + kCounterWords);
+            BH_HIP(hipMemcpyAsync(d_sched, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            BH_HIP(hipStreamSynchronize(ctx->stream));   // the host vector dies with this scope
+            A.tile_order = d_sched;
+        } else if (u.order == 1) {
             if ((st = build_schedule(ctx, *scene, A.cam, A.tiles_x, tiles_y, A.queues, q_tiles))) return st;
             A.tile_order = ctx->d_order;
         }
@@ -1960,7 +2011,8 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
     const int max_useful = (A.n_ids + u.block_threads - 1) / u.block_threads;
     if (grid > max_useful) grid = max_useful;
     if (grid < 1) grid = 1;
-    if ((st = fill_u32(ctx, ctx->d_counters, 0u, kCounterWords))) return st;
+    if ((st = fill_u32(ctx, counters, 0u, kCounterWords))) return st;
+    ctx->last_counters = counters;
     if (timed) begin_timing(ctx);
     void *args[] = {&A};
     BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(fn), dim3(grid), dim3(u.block_threads), args, 0, ctx->stream));
@@ -2059,7 +2111,8 @@ int bh_ctx_last_work(bh_ctx *ctx, uint64_t *steps, uint64_t *samples, uint64_t *
     int st = ctx_check(ctx);
     if (st) return st;
     unsigned long long w[3] = {0, 0, 0};
-    BH_HIP(hipMemcpyAsync(w, ctx->d_counters + kWorkOffset, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+    const unsigned *c = ctx->last_counters ? ctx->last_counters : ctx->d_counters;
+    BH_HIP(hipMemcpyAsync(w, c + kWorkOffset, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     if (steps) *steps = w[0];
     if (samples) *samples = w[1];
@@ -2071,7 +2124,8 @@ int bh_ctx_last_coop(bh_ctx *ctx, uint64_t *split_rays, uint64_t *timeouts) {
     int st = ctx_check(ctx);
     if (st) return st;
     unsigned long long w[2] = {0, 0};
-    BH_HIP(hipMemcpyAsync(w, ctx->d_counters + kWorkOffset + 6, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
+    const unsigned *c = ctx->last_counters ? ctx->last_counters : ctx->d_counters;
+    BH_HIP(hipMemcpyAsync(w, c + kWorkOffset + 6, sizeof(w), hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     if (split_rays) *split_rays = w[0];
     if (timeouts) *timeouts = w[1];
@@ -2123,6 +2177,16 @@ int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, 
     int st = ctx_check(ctx);
     if (st) return st;
     return render_enqueue(ctx, scene, brick_id, precision, d_iquv, d_steps, d_status, nullptr, nullptr, 0, true);
+}
+
+int bh_render_device_noalloc(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision,
+                             void *d_iquv, int32_t *d_steps, uint8_t *d_status) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    unsigned *buf = nullptr;
+    if ((st = scratch_address(ctx, &buf))) return st;
+    return render_enqueue(ctx, scene, brick_id, precision, d_iquv, d_steps, d_status, nullptr, nullptr, 0, true, buf);
 }
 
 int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision, void *iquv, int32_t *steps,
@@ -2345,6 +2409,7 @@ int bh_batch_launch(bh_batch *bt, void *d_iquv, int32_t *d_steps, uint8_t *d_sta
     int grid = 0, block = 0;
     if ((st = batch_args(bt, d_iquv, d_steps, d_status, B, grid, block))) return st;
     if ((st = fill_u32(bt->ctx, bt->ctx->d_counters, 0u, kCounterWords))) return st;
+    bt->ctx->last_counters = bt->ctx->d_counters;
     begin_timing(bt->ctx);
     void *args[] = {&B};
     BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(bt->fn), dim3(grid), dim3(block), args, 0, bt->ctx->stream));
@@ -2362,6 +2427,7 @@ int bh_batch_launch_graph(bh_batch *bt, void *d_iquv, int32_t *d_steps, uint8_t 
     BatchArgs B{};
     int grid = 0, block = 0;
     if ((st = batch_args(bt, d_iquv, d_steps, d_status, B, grid, block))) return st;
+    ctx->last_counters = ctx->d_counters;
     const bool same = bt->exec && bt->g_iquv == d_iquv && bt->g_steps == (void *)d_steps && bt->g_status == (void *)d_status &&
                       bt->g_grid == grid && bt->g_block == block && bt->g_spc == B.steps_per_sync;
     if (!same) {

@@ -199,6 +199,22 @@ int bh_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id,
 int bh_render_device(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id,
                      int32_t precision, void *d_iquv, int32_t *d_steps, uint8_t *d_status);
 
+/* No counterpart in the reference: bh_render_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and results as
+ * bh_render_device, but the launch's ray-queue counters and its tile schedule
+ * live in a static device array of the library (BH_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends) instead of memory the context allocates,
+ * so the call makes no hipMalloc or hipFree.  Images up to BH_NOALLOC_MAX_TILES
+ * 8x8 tiles (2048 x 2048 pixels); larger ones give BH_ERR_VALUE.  Every call on
+ * a GPU shares that GPU's array (it keeps the last call's counters, which
+ * bh_ctx_last_work reads): calls that overlap on one GPU, from two contexts or
+ * threads, race.  The call waits for the upload of its tile schedule. */
+#define BH_NOALLOC_MAX_TILES     65536
+#define BH_NOALLOC_SCRATCH_BYTES (4 * (64 + 65536 + 8))
+int bh_render_device_noalloc(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, int32_t precision,
+                             void *d_iquv, int32_t *d_steps, uint8_t *d_status);
+
 /* ---- batched persistent launch --------------------------------------------
  * Many renders (own image size, spin, camera and brick) traced by one persistent
  * kernel launch.  All bricks of a batch must share storage and layout.  Render i
