@@ -35,11 +35,14 @@ class SceneDesc(C.Structure):
         ("fov_y_degrees", C.c_double), ("roll_degrees", C.c_double),
         ("width", C.c_int32), ("height", C.c_int32), ("step_mode", C.c_int32), ("max_steps", C.c_int32),
         ("step", C.c_double), ("escape_radius", C.c_double), ("affine_to_length_cm", C.c_double),
-        ("max_optical_depth", C.c_double), ("transfer", C.c_int32), ("reserved", C.c_int32),
+        ("max_optical_depth", C.c_double), ("transfer", C.c_int32), ("integrator", C.c_int32),
+        ("sampling", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
 TRANSFER_RK2, TRANSFER_EXACT = 0, 1
+INTEGRATOR_RK2, INTEGRATOR_RK4 = 0, 1
+SAMPLING_NATIVE, SAMPLING_NUMPY32 = 0, 1
 
 
 class Tuning(C.Structure):

@@ -10,9 +10,14 @@ Same names and argument meaning as the reference: ``AcceleratedRenderConfig`` (:
 * one Kerr ray per pixel with the same camera and pixel centres (ndc = 2(i+1/2)/W - 1,
   1 - 2(j+1/2)/H) and the same transfer rule (midpoint sample, ds = L max(|d(r,theta,phi)|, 1e-14),
   RK2 or the exact matrix step, tau cap, nan_to_num);
-* the geodesic integrator is the RK2 midpoint scheme (fixed step ``cfg.step``, or
-  h = step (r - r_+) with ``step_mode=1``); the reference's CPU tracer takes RK4 steps
-  (kerr.py:306-313), so images agree to the integrators' truncation error, not bit for bit;
+* the geodesic integrator is the reference tracer's classical RK4 (kerr.py:306-313) with its fixed
+  step ``cfg.step`` and termination rules (kerr.py:357-377); ``integrator="rk2"`` selects the RK2
+  midpoint scheme of the GPU asset and ``step_mode=1`` the adaptive step h = step (r - r_+);
+* the FP64 render samples the float32 brick in float32 arithmetic exactly as the reference's
+  sample_coefficient_brick (:73-97) runs under NumPy, so its images equal the reference renderer's
+  to rounding of the geodesic arithmetic (statuses and step counts exactly; Stokes within 1e-10 of
+  |I| on the reference's own outputs, tests/golden/accelerated_vectors.npz); FP32 and mixed renders
+  trace and sample in FP32 (errors stated in tests/test_gpu_accelerated_api.py);
 * tiles (``tile_size``) only schedule the reference's CPU loop; the GPU renders the whole image in one
   launch, and ``render_progressive_stokes_bricks`` renders every progressive level in one batched
   launch.
@@ -42,7 +47,8 @@ class PolarizedTransferConfig:
 
 @dataclass(frozen=True)
 class AcceleratedRenderConfig:
-    """accelerated_renderer.py:36-46; ``step_mode`` (0 fixed, 1 adaptive h = step (r - r_+)) is an addition."""
+    """accelerated_renderer.py:36-46; ``step_mode`` (0 fixed, 1 adaptive h = step (r - r_+)) and ``integrator``
+    ("rk4" as the reference tracer, or "rk2" midpoint) are additions."""
 
     width: int = 160
     height: int = 90
@@ -55,6 +61,7 @@ class AcceleratedRenderConfig:
     tile_size: int = 64
     progressive_min_width: int = 480
     step_mode: int = STEP_FIXED
+    integrator: str = "rk4"
 
 
 @dataclass(frozen=True)
@@ -119,7 +126,8 @@ def scene_from_config(cfg: AcceleratedRenderConfig, snapshot, width: int | None 
     return Scene(spin=_spin(snapshot), camera=cfg.camera, width=int(width or cfg.width), height=int(height or cfg.height),
                  step_mode=int(cfg.step_mode), step=float(cfg.step), max_steps=int(cfg.max_steps),
                  escape_radius=float(cfg.escape_radius), affine_to_length_cm=float(t.affine_to_length_cm or 1.0),
-                 max_optical_depth=float(t.max_optical_depth), exact_step=bool(t.use_exact_matrix_step))
+                 max_optical_depth=float(t.max_optical_depth), exact_step=bool(t.use_exact_matrix_step),
+                 integrator=str(cfg.integrator), numpy32_sampling=True)
 
 
 def _upload(rr: Renderer, bricks: Brick, precision: int):

@@ -29,7 +29,9 @@ def scene_desc(scene: Scene) -> _lib.SceneDesc:
         width=int(scene.width), height=int(scene.height), step_mode=int(scene.step_mode),
         max_steps=int(scene.max_steps), step=float(scene.step), escape_radius=float(scene.escape_radius),
         affine_to_length_cm=float(scene.affine_to_length_cm), max_optical_depth=float(scene.max_optical_depth),
-        transfer=_lib.TRANSFER_EXACT if scene.exact_step else _lib.TRANSFER_RK2)
+        transfer=_lib.TRANSFER_EXACT if scene.exact_step else _lib.TRANSFER_RK2,
+        integrator={"rk2": _lib.INTEGRATOR_RK2, "rk4": _lib.INTEGRATOR_RK4}[scene.integrator],
+        sampling=_lib.SAMPLING_NUMPY32 if scene.numpy32_sampling else _lib.SAMPLING_NATIVE)
 
 
 @dataclass

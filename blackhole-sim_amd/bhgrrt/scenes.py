@@ -9,7 +9,8 @@ The dataclasses mirror the reference's configuration objects:
   ``PolarizedTransferConfig`` (python/blackhole_sim/polarized_transfer.py:22-28)
   and the spin that ``GRMHDSnapshot.spin_a`` carries.  The step rule is either
   the fixed affine step of the reference tracer (kerr.py:337-346) or the
-  adaptive rule h = h_frac * (r - r_+).
+  adaptive rule h = h_frac * (r - r_+); the integrator is RK2 midpoint or the
+  reference tracer's classical RK4 (kerr.py:306-313).
 * ``Brick`` follows ``CoefficientBrickGrid`` (python/blackhole_sim/
   coefficient_bricks.py:28-75): float32 grids, AoS coefficients of shape
   (nr, ntheta, nphi, 11) in the channel order of ``COEFF_NAMES``
@@ -93,6 +94,10 @@ class Scene:
     affine_to_length_cm: float = 1.0
     max_optical_depth: float = 30.0
     exact_step: bool = False      # PolarizedTransferConfig.use_exact_matrix_step (else one RK2 step per segment)
+    integrator: str = "rk2"       # geodesic step: "rk2" midpoint, or "rk4" as the reference tracer (kerr.py:306-313)
+    # FP64 render: sample the float32 brick in float32 as the reference renderer does under NumPy's scalar
+    # promotion (accelerated_renderer.py:73-97), instead of the drop-in sampler's FP64 (native_kernels.py:164-186)
+    numpy32_sampling: bool = False
 
     @property
     def aspect(self) -> float:

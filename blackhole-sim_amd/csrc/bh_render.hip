@@ -60,6 +60,7 @@ struct RenderArgs {
     double r_island, th_island, pth_island;   // FP64 islands of the mixed-precision path
     float step_f, r_plus_f, r_cap_f, r_esc_f, L_f, tau_max_f;   // FP32 copies for the hot loop
     int W, H, max_steps, adaptive;
+    int lerp32;                 // FP64 render: sample as the reference renderer does on a float32 brick
     const void *coeffs;
     int nr, nt, np;
     unsigned plane_bytes, total_bytes;
@@ -524,6 +525,61 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
     return true;
 }
 
+// The reference renderer's sample (accelerated_renderer.py:73-97 with the brackets of grmhd.py:229-257)
+// as NumPy evaluates it on a float32 brick: its Python-float coordinates meet float32 grid scalars and
+// float32 coefficient arrays, and NumPy casts such Python scalars to float32, so the range tests, the
+// searchsorted brackets, the weights and all three lerp levels run in float32 arithmetic, each
+// (1 - w) a + w b as two rounded products and a rounded sum; only the phi wrap and the seam shift
+// p + 2 pi are Python (double) arithmetic.  Every float32 operation below is rounded on its own
+// (contraction off: a fused multiply-add would round (1 - w) a + w b once instead of three times), so
+// the eleven values equal NumPy's bit for bit.  FP64 render, float32 SoA brick.
+__device__ __forceinline__ float lerp_numpy32(float a, float wa, float b, float wb) {
+#pragma clang fp contract(off)
+    return wa * a + wb * b;
+}
+
+__device__ __forceinline__ bool sample_brick_numpy32(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const double *gr,
+                                                     const double *gt, const double *gp, double x, double y, double z,
+                                                     double out[kCoeffs]) {
+#pragma clang fp contract(off)
+    const float xf = float(x), yf = float(y);
+    if (xf < A.hr.first || xf > A.hr.last || yf < A.ht.first || yf > A.ht.last) return false;   // NaN passes
+    const double p = Mth<double>::wrap(z, double(A.hp.first));
+    const float pf = float(p);
+    // searchsorted(side='right') - 1 on float32 values (the double grids hold them exactly), clipped
+    const int ir = bracket_search(gr, A.nr - 2, double(xf), guess_index<true>(xf, A.hr, A.nr - 2));
+    const int it = bracket_search(gt, A.nt - 2, double(yf), guess_index<false>(yf, A.ht, A.nt - 2));
+    int ip = bracket_search(gp, A.np - 1, double(pf), guess_index<false>(pf, A.hp, A.np - 1));
+    if (pf < A.hp.first) ip = A.np - 1;   // searchsorted gave -1: the seam cell
+    const int jp = (ip + 1 == A.np) ? 0 : ip + 1;
+    const float rl = float(gr[ir]), rh = float(gr[ir + 1]), tl = float(gt[it]), th = float(gt[it + 1]);
+    const float pl = float(gp[ip]);
+    const float p_hi = (jp > ip) ? float(gp[jp]) : A.hp.first + 6.28318548f;   // grid[0] + 2 pi in float32
+    const double pp = (pf >= pl) ? p : p + kTwoPi;
+    const float wr = (xf == A.hr.last) ? 1.0f : (xf - rl) / (rh - rl);
+    const float wt = (yf == A.ht.last) ? 1.0f : (yf - tl) / (th - tl);
+    const float wp = (float(pp) - pl) / (p_hi - pl);
+    const float ar = 1.0f - wr, at = 1.0f - wt, ap = 1.0f - wp;
+    const unsigned row00 = A.brick_base / 4u + (unsigned(ir) * unsigned(A.nt) + unsigned(it)) * unsigned(A.np);
+    const unsigned row01 = row00 + unsigned(A.np);
+    const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
+    const unsigned row11 = row10 + unsigned(A.np);
+    const unsigned o[8] = {(row00 + ip) * 4u, (row00 + jp) * 4u, (row01 + ip) * 4u, (row01 + jp) * 4u,
+                           (row10 + ip) * 4u, (row10 + jp) * 4u, (row11 + ip) * 4u, (row11 + jp) * 4u};
+#pragma unroll
+    for (int k = 0; k < kCoeffs; ++k) {
+        const unsigned so = unsigned(k) * A.plane_bytes;
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = load_coeff<BH_STORE_F32>(rs, o[q], so);
+        const float c00 = lerp_numpy32(v[0], ap, v[1], wp), c01 = lerp_numpy32(v[2], ap, v[3], wp);
+        const float c10 = lerp_numpy32(v[4], ap, v[5], wp), c11 = lerp_numpy32(v[6], ap, v[7], wp);
+        const float c0 = lerp_numpy32(c00, at, c01, wt), c1 = lerp_numpy32(c10, at, c11, wt);
+        out[k] = double(lerp_numpy32(c0, ar, c1, wr));
+    }
+    return true;
+}
+
 // lib.rs:37-59 and 210-225.
 template <typename R>
 __device__ __forceinline__ void stokes_rhs(const R s[4], const R c[kCoeffs], R o[4]) {
@@ -777,6 +833,34 @@ __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr
     npt = pth + h * k2pt;
 }
 
+// Classical RK4 step of the reference tracer (kerr.py:306-313): k1 = f(y), k2 = f(y + h/2 k1),
+// k3 = f(y + h/2 k2), k4 = f(y + h k3), y + (h/6)(k1 + 2 k2 + 2 k3 + k4), summed in that order.
+// Returns the raw end theta in nt_raw and the lowest and highest theta of the three inner stages.
+template <typename G, bool ISLAND = false>
+__device__ __forceinline__ void geo_step4(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
+                                          G &np, G &npr, G &npt, G &mlo, G &mhi) {
+    G k1r, k1t, k1p, k1pr, k1pt;
+    kerr_rhs<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    const G hh = G(0.5) * h;
+    const G t2 = th + hh * k1t;
+    G k2r, k2t, k2p, k2pr, k2pt;
+    kerr_rhs<G, ISLAND>(k, r + hh * k1r, t2, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    const G t3 = th + hh * k2t;
+    G k3r, k3t, k3p, k3pr, k3pt;
+    kerr_rhs<G, ISLAND>(k, r + hh * k2r, t3, pr + hh * k2pr, pth + hh * k2pt, k3r, k3t, k3p, k3pr, k3pt);
+    const G t4 = th + h * k3t;
+    G k4r, k4t, k4p, k4pr, k4pt;
+    kerr_rhs<G, ISLAND>(k, r + h * k3r, t4, pr + h * k3pr, pth + h * k3pt, k4r, k4t, k4p, k4pr, k4pt);
+    const G w = h / G(6);
+    nr = r + w * (k1r + G(2) * k2r + G(2) * k3r + k4r);
+    nt_raw = th + w * (k1t + G(2) * k2t + G(2) * k3t + k4t);
+    np = ph + w * (k1p + G(2) * k2p + G(2) * k3p + k4p);
+    npr = pr + w * (k1pr + G(2) * k2pr + G(2) * k3pr + k4pr);
+    npt = pth + w * (k1pt + G(2) * k2pt + G(2) * k3pt + k4pt);
+    mlo = t2 < t3 ? (t2 < t4 ? t2 : t4) : (t3 < t4 ? t3 : t4);
+    mhi = t2 > t3 ? (t2 > t4 ? t2 : t4) : (t3 > t4 ? t3 : t4);
+}
+
 // The end point of one geodesic step: the state after it, whether it met the pole pathology (the
 // theta clamp fired, or the step swung phi by more than 1 rad) and whether all five values are finite.
 template <typename R>
@@ -785,14 +869,14 @@ struct GeoPoint {
     bool hit, fin;
 };
 
-// One RK2 geodesic step of the ray from (r, th, ph, pr, pth); the ray's constants of motion come from s.
-// MIXED: the step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
+// One geodesic step (RK2 midpoint, or RK4) of the ray from (r, th, ph, pr, pth); the ray's constants
+// of motion come from s.  MIXED: the step runs in FP64 inside the islands (r < 1.5 r_+, within th_island of a pole,
 // |p_theta| < pth_island near a theta turning point).
-template <typename R, bool MIXED>
+template <typename R, bool MIXED, bool RK4>
 __device__ __forceinline__ GeoPoint<R> geo_next(const RenderArgs &A, const HotParams<R> &P, const RayState<R> &s, R r,
                                                 R th, R ph, R pr, R pth) {
     const R lo = R(kEpsTheta), hi = Mth<R>::pi_minus_eps();
-    R nr, nt_raw, np, npr, npt, mt;
+    R nr, nt_raw, np, npr, npt, mlo, mhi;
     bool island = false;
     if constexpr (MIXED) {
         island = (double(r) < A.r_island) || (double(th) < A.th_island) ||
@@ -801,18 +885,30 @@ __device__ __forceinline__ GeoPoint<R> geo_next(const RenderArgs &A, const HotPa
     if (MIXED && island) {
         const double hd = A.adaptive ? A.step * (double(r) - A.r_plus) : A.step;
         const RayK<double> kd = make_k<double>(A.a, s.Ed, s.Ld);
-        double dr_, dt_, dp_, dpr_, dpt_, dm_;
-        geo_step<double, true>(kd, double(r), double(th), double(ph), double(pr), double(pth), hd, dr_, dt_, dp_, dpr_, dpt_,
-                         dm_);
+        double dr_, dt_, dp_, dpr_, dpt_, dlo_, dhi_;
+        if constexpr (RK4) {
+            geo_step4<double, true>(kd, double(r), double(th), double(ph), double(pr), double(pth), hd, dr_, dt_, dp_,
+                                    dpr_, dpt_, dlo_, dhi_);
+        } else {
+            geo_step<double, true>(kd, double(r), double(th), double(ph), double(pr), double(pth), hd, dr_, dt_, dp_,
+                                   dpr_, dpt_, dlo_);
+            dhi_ = dlo_;
+        }
         nr = R(dr_);
         nt_raw = R(dt_);
         np = R(dp_);
         npr = R(dpr_);
         npt = R(dpt_);
-        mt = R(dm_);
+        mlo = R(dlo_);
+        mhi = R(dhi_);
     } else {
         const R h = P.adaptive ? P.step * (r - P.r_plus) : P.step;
-        geo_step<R>(s.k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
+        if constexpr (RK4) {
+            geo_step4<R>(s.k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo, mhi);
+        } else {
+            geo_step<R>(s.k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo);
+            mhi = mlo;
+        }
     }
     GeoPoint<R> b;
     b.r = nr;
@@ -820,7 +916,7 @@ __device__ __forceinline__ GeoPoint<R> geo_next(const RenderArgs &A, const HotPa
     b.ph = np;
     b.pr = npr;
     b.pth = npt;
-    b.hit = (mt < lo) || (mt > hi) || (nt_raw < lo) || (nt_raw > hi) || (np - ph > R(1)) || (ph - np > R(1));
+    b.hit = (mlo < lo) || (mhi > hi) || (nt_raw < lo) || (nt_raw > hi) || (np - ph > R(1)) || (ph - np > R(1));
     b.fin = finite_(nr) && finite_(b.th) && finite_(np) && finite_(npr) && finite_(npt);
     return b;
 }
@@ -868,14 +964,24 @@ __device__ __forceinline__ int end_code(const HotParams<R> &P, const RayState<R>
 }
 
 // One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool COOP = false>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4, bool COOP = false>
 __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                         const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
-    const GeoPoint<R> b = geo_next<R, MIXED>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
+    const GeoPoint<R> b = geo_next<R, MIXED, RK4>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
     s.clamped = s.clamped || b.hit;
     R c[kCoeffs];
-    if (sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
-                                             midpoint(s.ph, b.ph), c)) {
+    bool inside;
+    if constexpr (sizeof(R) == 8 && STORE == BH_STORE_F32 && LAYOUT == BH_LAYOUT_SOA) {
+        if (A.lerp32)
+            inside = sample_brick_numpy32(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th), midpoint(s.ph, b.ph), c);
+        else
+            inside = sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
+                                                          midpoint(s.ph, b.ph), c);
+    } else {
+        inside = sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
+                                                      midpoint(s.ph, b.ph), c);
+    }
+    if (inside) {
         segment_transfer<R, EXACT>(s, segment_ds(P, s, b), c);
         ++n_samples;
     }
@@ -1059,7 +1165,7 @@ __device__ __forceinline__ T lds_wait(T *p, F pred, bool &timed_out) {
 
 // Helper side: the geodesic of the handed-over ray, one ring slot per step, until the geodesic ends the
 // ray, the lone wave says stop, or the ring stays full past the spin bound.
-template <int STORE, bool MIXED, bool EXACT>
+template <int STORE, bool MIXED, bool EXACT, bool RK4>
 __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotParams<float> &P, const float *gr,
                                                  const float *gt, const float *gp, CoopLds *c, int lane) {
     RayState<float> s = c->ray;
@@ -1073,7 +1179,7 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
             if (timed_out || head - tail >= unsigned(kCoopRing)) break;
         }
         if ((head & 7u) == 0 && lds_peek(&c->stop)) break;
-        const GeoPoint<float> b = geo_next<float, MIXED>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
+        const GeoPoint<float> b = geo_next<float, MIXED, RK4>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
         bool captured, escaped;
         end_tests<float, MIXED>(A, P, s.r, b.r, captured, escaped);
         const float x = midpoint(s.r, b.r), y = midpoint(s.th, b.th), z = midpoint(s.ph, b.ph);
@@ -1264,7 +1370,7 @@ __device__ __forceinline__ int coop_consume(const RenderArgs &A, const HotParams
 // the result.  The copies stay identical, so the wave leaves the loop together.  With coop != null and
 // coop_steps > 0, a ray still alone after coop_steps steps asks an idle wave of the workgroup to take its
 // geodesic (see above).
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                          const R *gr, const R *gt, const R *gp, RayState<R> &s, int lane, int owner,
                                          unsigned &n_steps, unsigned &n_samples, unsigned long long &iters,
@@ -1280,7 +1386,7 @@ __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R>
         // the lone loop proper; with cooperation on, it pauses every few steps for the bookkeeping below
         const unsigned pause = coop_on ? (asked ? steps + 4u : max(steps + 8u, unsigned(A.coop_steps))) : ~0u;
         do {
-            code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, true>(A, P, rs, gr, gt, gp, s, samples);
+            code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, RK4, true>(A, P, rs, gr, gt, gp, s, samples);
             ++steps;
         } while (code < 0 && steps < pause);
         if (code >= 0) break;
@@ -1326,7 +1432,7 @@ __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R>
 
 // Helper side of the handshake, for a wave with no rays of its own.  Returns true when it helped (or
 // took a request that was then withdrawn).
-template <int STORE, bool MIXED, bool EXACT>
+template <int STORE, bool MIXED, bool EXACT, bool RK4>
 __device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<float> &P, const float *gr,
                                           const float *gt, const float *gp, CoopLds *c, int lane, int wave,
                                           unsigned long long &iters) {
@@ -1337,7 +1443,7 @@ __device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<f
     bool timed_out = false;
     const int go = coop_word(kCoopGo, requester), cancel = coop_word(kCoopCancel, requester);
     const int st = lds_wait(&c->state, [go, cancel](int v) { return v == go || v == cancel; }, timed_out);
-    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED, EXACT>(A, P, gr, gt, gp, c, lane);
+    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED, EXACT, RK4>(A, P, gr, gt, gp, c, lane);
     // leave: Go / Cancel -> Free (after a time-out the lone wave sees no Free and stops using the slot)
     if (lane == 0 && !timed_out) lds_store(&c->state, coop_word(kCoopFree, requester));
     return true;
@@ -1351,7 +1457,7 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long mask) {
 #define BH_RENDER_WAVES_PER_EU 2
 #endif
 
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void render_kernel(
     RenderArgs A) {
     __shared__ R s_grid[kMaxGrid + 1];
@@ -1391,7 +1497,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const int n_idle = __popcll(idle);
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             // a wave with no rays first serves a lone ray of its workgroup that asks for help
-            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED, EXACT>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED, EXACT, RK4>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
         }
         if (exhausted != all_q && (n_idle >= A.refill_min || n_idle == 64)) {
             for (int t = 0; t < A.queues && idle; ++t) {
@@ -1439,7 +1545,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
                     if (A.coop_steps > 0) {
                         unsigned spins = 0;
                         while (lds_peek(&coop.tracing) > 0 && spins < kCoopSpin) {
-                            if (coop_help<STORE, MIXED, EXACT>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+                            if (coop_help<STORE, MIXED, EXACT, RK4>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
                             __builtin_amdgcn_s_sleep(2);
                             ++spins;
                         }
@@ -1451,7 +1557,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         }
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {   // no refill happened above: the wave serves its last ray with all lanes
-                lone_ray<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
+                lone_ray<R, STORE, LAYOUT, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
                                                   n_steps, n_samples, iters, &coop, wave);
                 active = false;
                 continue;
@@ -1461,7 +1567,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < A.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
@@ -1506,7 +1612,7 @@ struct BatchArgs {
     int queues, q_tiles, steps_per_sync;
 };
 
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void batch_kernel(
     BatchArgs B) {
     __shared__ R s_grid[kMaxGrid + 8];
@@ -1564,7 +1670,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const HotParams<R> P = loop_params<R>(A);
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {
-                lone_ray<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
+                lone_ray<R, STORE, LAYOUT, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
                                                   n_steps, n_samples, iters, nullptr, 0);
                 active = false;
                 continue;
@@ -1574,7 +1680,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         else __builtin_amdgcn_s_setprio(0);
         for (int k = 0; k < B.steps_per_sync; ++k) {
             if (active) {
-                const int code = ray_step<R, STORE, LAYOUT, MIXED, EXACT>(A, P, rs, gr, gt, gp, s, n_samples);
+                const int code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, n_samples);
                 ++n_steps;
                 if (code >= 0) {
                     ray_store(A, s, code);
@@ -1749,6 +1855,10 @@ static int validate_scene(const bh_scene_desc &d) {
         return set_error(BH_ERR_VALUE, "escape_radius, affine_to_length_cm and max_optical_depth must not be NaN");
     if (d.transfer != BH_TRANSFER_RK2 && d.transfer != BH_TRANSFER_EXACT)
         return set_error(BH_ERR_VALUE, "transfer must be BH_TRANSFER_RK2 or BH_TRANSFER_EXACT");
+    if (d.integrator != BH_INTEGRATOR_RK2 && d.integrator != BH_INTEGRATOR_RK4)
+        return set_error(BH_ERR_VALUE, "integrator must be BH_INTEGRATOR_RK2 or BH_INTEGRATOR_RK4");
+    if (d.sampling != BH_SAMPLING_NATIVE && d.sampling != BH_SAMPLING_NUMPY32)
+        return set_error(BH_ERR_VALUE, "sampling must be BH_SAMPLING_NATIVE or BH_SAMPLING_NUMPY32");
     return BH_OK;
 }
 
@@ -1823,53 +1933,61 @@ typedef void (*BatchFn)(BatchArgs);
 
 // Kernel table: precision x storage x layout x transfer rule.  The exact Stokes step is a template
 // parameter, so the RK2 kernels carry none of its code or registers.
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 struct KernelPair {
-    static RenderFn render() { return render_kernel<R, STORE, LAYOUT, MIXED, EXACT>; }
-    static BatchFn batch() { return batch_kernel<R, STORE, LAYOUT, MIXED, EXACT>; }
+    static RenderFn render() { return render_kernel<R, STORE, LAYOUT, MIXED, EXACT, RK4>; }
+    static BatchFn batch() { return batch_kernel<R, STORE, LAYOUT, MIXED, EXACT, RK4>; }
 };
 
-template <bool BATCH, typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT>
+template <bool BATCH, typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 static void *kernel_of() {
-    using K = KernelPair<R, STORE, LAYOUT, MIXED, EXACT>;
+    using K = KernelPair<R, STORE, LAYOUT, MIXED, EXACT, RK4>;
     return BATCH ? reinterpret_cast<void *>(K::batch()) : reinterpret_cast<void *>(K::render());
 }
 
-template <bool BATCH, int LAYOUT, bool EXACT>
+template <bool BATCH, int LAYOUT, bool EXACT, bool RK4>
 static void *pick_fp32(int storage, bool mixed) {
     switch (storage) {
         case BH_STORE_F32:
-            return mixed ? kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, true, EXACT>()
-                         : kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, false, EXACT>();
+            return mixed ? kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, true, EXACT, RK4>()
+                         : kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, false, EXACT, RK4>();
         case BH_STORE_BF16:
-            return mixed ? kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, true, EXACT>()
-                         : kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, false, EXACT>();
+            return mixed ? kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, true, EXACT, RK4>()
+                         : kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, false, EXACT, RK4>();
         case BH_STORE_F16:
-            return mixed ? kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, true, EXACT>()
-                         : kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, false, EXACT>();
+            return mixed ? kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, true, EXACT, RK4>()
+                         : kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, false, EXACT, RK4>();
     }
     return nullptr;
 }
 
-template <bool BATCH, bool EXACT>
+// Kernels are compiled per integrator: the RK4 step's registers would otherwise weigh on every RK2
+// kernel (a uniform branch on the rule cost the mixed batch 14 %).  RK4 is built for the FP64 render
+// and the corner-packed FP32 / mixed renders (the layouts the reference renderer's drop-in uploads).
+template <bool BATCH, bool EXACT, bool RK4>
 static void *pick_any(int precision, int storage, int layout) {
     if (precision == BH_PREC_FP64)
         return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA)
-                   ? kernel_of<BATCH, double, BH_STORE_F32, BH_LAYOUT_SOA, false, EXACT>()
+                   ? kernel_of<BATCH, double, BH_STORE_F32, BH_LAYOUT_SOA, false, EXACT, RK4>()
                    : nullptr;
     const bool mixed = precision == BH_PREC_MIXED;
-    return layout == BH_LAYOUT_CORNER ? pick_fp32<BATCH, BH_LAYOUT_CORNER, EXACT>(storage, mixed)
-                                      : pick_fp32<BATCH, BH_LAYOUT_SOA, EXACT>(storage, mixed);
+    if (layout == BH_LAYOUT_CORNER) return pick_fp32<BATCH, BH_LAYOUT_CORNER, EXACT, RK4>(storage, mixed);
+    if constexpr (RK4) return nullptr;
+    else return pick_fp32<BATCH, BH_LAYOUT_SOA, EXACT, false>(storage, mixed);
 }
 
-static RenderFn pick_kernel(int precision, int storage, int layout, bool exact) {
-    return reinterpret_cast<RenderFn>(exact ? pick_any<false, true>(precision, storage, layout)
-                                            : pick_any<false, false>(precision, storage, layout));
+template <bool BATCH>
+static void *pick_rule(int precision, int storage, int layout, bool exact, bool rk4) {
+    if (rk4) return exact ? pick_any<BATCH, true, true>(precision, storage, layout) : pick_any<BATCH, false, true>(precision, storage, layout);
+    return exact ? pick_any<BATCH, true, false>(precision, storage, layout) : pick_any<BATCH, false, false>(precision, storage, layout);
 }
 
-static BatchFn pick_batch_kernel(int precision, int storage, int layout, bool exact) {
-    return reinterpret_cast<BatchFn>(exact ? pick_any<true, true>(precision, storage, layout)
-                                           : pick_any<true, false>(precision, storage, layout));
+static RenderFn pick_kernel(int precision, int storage, int layout, bool exact, bool rk4) {
+    return reinterpret_cast<RenderFn>(pick_rule<false>(precision, storage, layout, exact, rk4));
+}
+
+static BatchFn pick_batch_kernel(int precision, int storage, int layout, bool exact, bool rk4) {
+    return reinterpret_cast<BatchFn>(pick_rule<true>(precision, storage, layout, exact, rk4));
 }
 
 static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A) {
@@ -1890,6 +2008,7 @@ static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A)
     A.W = sc.width;
     A.H = sc.height;
     A.max_steps = sc.max_steps;
+    A.lerp32 = sc.sampling == BH_SAMPLING_NUMPY32;
     A.step_f = float(A.step);
     A.r_plus_f = float(A.r_plus);
     A.r_cap_f = float(A.r_cap);
@@ -1956,8 +2075,12 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
     if (precision != BH_PREC_FP64 && precision != BH_PREC_FP32 && precision != BH_PREC_MIXED)
         return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
     const BrickDev &b = ctx->bricks[brick_id];
-    RenderFn fn = pick_kernel(precision, b.storage, b.layout, scene->transfer == BH_TRANSFER_EXACT);
-    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 render needs a float32 brick in the SoA layout");
+    const bool rk4 = scene->integrator == BH_INTEGRATOR_RK4;
+    RenderFn fn = pick_kernel(precision, b.storage, b.layout, scene->transfer == BH_TRANSFER_EXACT, rk4);
+    if (!fn)
+        return set_error(BH_ERR_VALUE, precision == BH_PREC_FP64
+                                           ? "the FP64 render needs a float32 brick in the SoA layout"
+                                           : "the RK4 integrator of the FP32 and mixed renders needs the corner layout");
     RenderArgs A{};
     if ((st = setup_args(*scene, b, A))) return st;
     A.iquv = d_iquv;
@@ -2313,9 +2436,15 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         if (tiles >= (1 << 20)) return set_error(BH_ERR_VALUE, "render %d has too many 8x8 tiles", i);
         if (items[i].scene.transfer != items[0].scene.transfer)
             return set_error(BH_ERR_VALUE, "all renders of a batch must share the transfer rule");
+        if (items[i].scene.integrator != items[0].scene.integrator)
+            return set_error(BH_ERR_VALUE, "all renders of a batch must share the integrator");
     }
-    BatchFn fn = pick_batch_kernel(precision, storage, layout, items[0].scene.transfer == BH_TRANSFER_EXACT);
-    if (!fn) return set_error(BH_ERR_VALUE, "the FP64 batch needs float32 bricks in the SoA layout");
+    BatchFn fn = pick_batch_kernel(precision, storage, layout, items[0].scene.transfer == BH_TRANSFER_EXACT,
+                                   items[0].scene.integrator == BH_INTEGRATOR_RK4);
+    if (!fn)
+        return set_error(BH_ERR_VALUE, precision == BH_PREC_FP64
+                                           ? "the FP64 batch needs float32 bricks in the SoA layout"
+                                           : "the RK4 integrator of the FP32 and mixed batches needs the corner layout");
     // merged coefficient buffer and grid table
     std::vector<size_t> coff(uniq.size());
     std::vector<int> goff(uniq.size());

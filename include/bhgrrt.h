@@ -153,11 +153,26 @@ typedef struct bh_scene_desc {
     int32_t transfer;         /* BH_TRANSFER_RK2 or BH_TRANSFER_EXACT
                                  (PolarizedTransferConfig.use_exact_matrix_step,
                                  polarized_transfer.py:51-80) */
+    int32_t integrator;       /* BH_INTEGRATOR_RK2 or BH_INTEGRATOR_RK4 */
+    int32_t sampling;         /* BH_SAMPLING_NATIVE or BH_SAMPLING_NUMPY32 */
     int32_t reserved;         /* 0 */
 } bh_scene_desc;
 
 #define BH_TRANSFER_RK2    0   /* one RK2 step of dS/ds = j - K S per segment (stokes_step_rk2) */
 #define BH_TRANSFER_EXACT  1   /* exact constant-coefficient step: exp of [[-K, j], [0, 0]] ds (stokes_step_exact) */
+
+/* Geodesic step: RK2 midpoint (the GPU asset's rk2_geodesic_step, native/rocm/kerr_stokes_kernel.hip:13),
+ * or classical RK4 as the reference tracer (rk4_step, kerr.py:306-313, called by
+ * trace_kerr_null_geodesic from render_tile_stokes_bricks, accelerated_renderer.py:142). */
+#define BH_INTEGRATOR_RK2  0
+#define BH_INTEGRATOR_RK4  1
+/* Brick sampling of the FP64 render: NATIVE brackets and interpolates in FP64 like the drop-in
+ * sampler (native_kernels.py:164-186, lib.rs:171-208); NUMPY32 reproduces the reference renderer's
+ * sample_coefficient_brick (accelerated_renderer.py:73-97) on a float32 brick as NumPy evaluates it,
+ * with coordinates, brackets, weights and lerps in float32 arithmetic.  FP32 and mixed renders always
+ * sample in their own FP32 form. */
+#define BH_SAMPLING_NATIVE   0
+#define BH_SAMPLING_NUMPY32  1
 
 #define BH_PREC_FP64   0   /* full FP64 render (reference-grade) */
 #define BH_PREC_FP32   1   /* plain FP32 geodesic + transfer */

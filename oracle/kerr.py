@@ -221,6 +221,17 @@ def rk2_step(y, h: float, a: float, p_t: float, p_phi: float) -> np.ndarray:
     return out
 
 
+def rk4_step(y, h: float, a: float, p_t: float, p_phi: float) -> np.ndarray:
+    """kerr.py:306-313: classical RK4, out = y + (h / 6) (k1 + 2 k2 + 2 k3 + k4), theta clamped."""
+    k1 = geodesic_rhs(y, a, p_t, p_phi)
+    k2 = geodesic_rhs(y + 0.5 * h * k1, a, p_t, p_phi)
+    k3 = geodesic_rhs(y + 0.5 * h * k2, a, p_t, p_phi)
+    k4 = geodesic_rhs(y + h * k3, a, p_t, p_phi)
+    out = y + (h / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    out[2] = float(safe_theta(float(out[2])))
+    return out
+
+
 # --- vectorised RHS over rays (same matrix form, component-wise) -----------
 
 def geodesic_rhs_vec(r, th, p_r, p_th, p_t, p_ph, a: float):

@@ -306,6 +306,53 @@ def sample_vec(coeffs: np.ndarray, rg, tg, pg, r, th, ph):
     return out, valid
 
 
+def sample_vec_numpy32(coeffs32: np.ndarray, rg32, tg32, pg32, r, th, ph):
+    """The reference renderer's sample_coefficient_brick (accelerated_renderer.py:73-97, brackets of
+    grmhd.py:229-257) on a float32 brick, as NumPy evaluates it.
+
+    There the coordinates are Python floats and the grids and coefficient arrays float32; NumPy casts a
+    Python float that meets a float32 scalar or array to float32, so the range tests, the searchsorted
+    brackets, the weights (x - g0) / (g1 - g0) and the three lerp levels all run in float32 arithmetic.
+    The phi wrap and the seam shift p + 2 pi stay Python (double) arithmetic; grid[0] + 2 pi is a
+    float32 sum.  Returns ((n, 11) float64 coefficients, valid mask); invalid rows are NaN.
+    """
+    f32 = np.float32
+    rg, tg, pg = (np.asarray(g, dtype=f32) for g in (rg32, tg32, pg32))
+    c = np.asarray(coeffs32, dtype=f32)
+    with np.errstate(invalid="ignore"):
+        xf, yf = np.asarray(r, dtype=np.float64).astype(f32), np.asarray(th, dtype=np.float64).astype(f32)
+        valid = ~((xf < rg[0]) | (xf > rg[-1]) | (yf < tg[0]) | (yf > tg[-1]))
+
+        def bracket(g, x):
+            i0 = np.clip(np.searchsorted(g, x, side="right") - 1, 0, g.size - 2)
+            w = ((x - g[i0]) / (g[i0 + 1] - g[i0])).astype(f32)
+            return i0, np.where(x == g[-1], f32(1.0), w).astype(f32)
+
+        r0, wr = bracket(rg, xf)
+        t0, wt = bracket(tg, yf)
+        p = wrap_phi_vec(np.asarray(ph, dtype=np.float64), float(pg[0]))
+        pf = p.astype(f32)
+        n = pg.size
+        p0 = np.searchsorted(pg, pf, side="right") - 1
+        p0 = np.where(p0 < 0, n - 1, np.minimum(p0, n - 1))
+        p1 = (p0 + 1) % n
+        hi = np.where(p1 > p0, pg[p1], f32(pg[0] + f32(TWO_PI)))
+        pp = np.where(pf >= pg[p0], p, p + TWO_PI)
+        wp = ((pp.astype(f32) - pg[p0]) / (hi - pg[p0])).astype(f32)
+        r1, t1 = r0 + 1, t0 + 1
+        ap, at, ar = (f32(1.0) - wp)[:, None], (f32(1.0) - wt)[:, None], (f32(1.0) - wr)[:, None]
+        wp_, wt_, wr_ = wp[:, None], wt[:, None], wr[:, None]
+        c00 = ap * c[r0, t0, p0] + wp_ * c[r0, t0, p1]
+        c01 = ap * c[r0, t1, p0] + wp_ * c[r0, t1, p1]
+        c10 = ap * c[r1, t0, p0] + wp_ * c[r1, t0, p1]
+        c11 = ap * c[r1, t1, p0] + wp_ * c[r1, t1, p1]
+        c0 = at * c00 + wt_ * c01
+        c1 = at * c10 + wt_ * c11
+        out = (ar * c0 + wr_ * c1).astype(np.float64)
+    out[~valid] = np.nan
+    return out, valid
+
+
 def sample_brick_trilinear_vec(coeffs, r_grid, theta_grid, phi_grid, points) -> np.ndarray:
     c, rg, tg, pg, p = coerce_sampler_inputs(coeffs, r_grid, theta_grid, phi_grid, points)
     out, _ = sample_vec(c, rg, tg, pg, p[:, 0], p[:, 1], p[:, 2])

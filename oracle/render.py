@@ -3,12 +3,16 @@
 One ray per pixel.  Each ray follows, in order:
 
 * camera launch: kerr.py:188-233 (``camera_rays_vec``);
-* RK2 midpoint geodesic steps, k1 = f(y), k2 = f(y + h/2 k1), y + h k2, theta
-  clamped (scheme of native/rocm/kerr_stokes_kernel.hip:13, RHS of
-  kerr.py:285-303); h is the fixed affine step or h_frac * (r - r_+) taken
+* geodesic steps of the scene's integrator, theta clamped after each: RK2
+  midpoint, k1 = f(y), k2 = f(y + h/2 k1), y + h k2 (scheme of
+  native/rocm/kerr_stokes_kernel.hip:13), or the reference tracer's classical
+  RK4, y + (h/6)(k1 + 2 k2 + 2 k3 + k4) (kerr.py:306-313); RHS of
+  kerr.py:285-303; h is the fixed affine step or h_frac * (r - r_+) taken
   from the state at the start of the step;
 * after each step the transfer segment of accelerated_renderer.py:112-125:
-  sample the brick at the coordinate midpoint of (previous, current) state,
+  sample the brick at the coordinate midpoint of (previous, current) state
+  (in FP64 like native_kernels.py's sampler, or with ``scene.numpy32_sampling``
+  in float32 as the reference renderer's own sampler runs under NumPy),
   skip the segment when r or theta is outside the brick, otherwise
   ds = L * max(|delta(r, theta, phi)|, 1e-14), one RK2 Stokes step (or, with
   ``scene.exact_step``, the exact step of polarized_transfer.py:51-70), and
@@ -27,9 +31,9 @@ sampler works in FP64 on the float32 grid and coefficient values, like
 native_kernels.py's FP64 sampler.
 
 Status codes (low 3 bits): 0 max_steps, 1 captured, 2 escaped, 3 invalid,
-4 optically thick.  Bit 3 (value 8) marks a ray whose RK2 steps could not
+4 optically thick.  Bit 3 (value 8) marks a ray whose steps could not
 follow the geodesic through a coordinate pole: the theta clamp of kerr.py:35-36
-fired at a midpoint or end state, or one step changed phi by more than 1 rad
+fired at a stage or end state, or one step changed phi by more than 1 rad
 (d(phi)/dlambda = L / (Sigma sin^2 theta) blows up within a step of the axis;
 regular steps move phi by well under 1 rad).  Such rays pick up huge ds and
 their Stokes vectors are not meaningful in any precision, so error statistics
@@ -63,6 +67,24 @@ def _rk2_stokes(S, c, ds):
     return S + d * k2
 
 
+def geodesic_step_vec(r, th, ph, pr, pth, pt, pph, h, a: float, rk4: bool):
+    """One RK2 midpoint or RK4 step over arrays of rays.  Returns the end state (r, raw theta, phi, p_r,
+    p_theta) and the theta of every intermediate stage (the states the RHS saw besides the start)."""
+    k1 = kerr.geodesic_rhs_vec(r, th, pr, pth, pt, pph, a)
+    hh = 0.5 * h
+    th2 = th + hh * k1[1]
+    k2 = kerr.geodesic_rhs_vec(r + hh * k1[0], th2, pr + hh * k1[3], pth + hh * k1[4], pt, pph, a)
+    if not rk4:
+        return (r + h * k2[0], th + h * k2[1], ph + h * k2[2], pr + h * k2[3], pth + h * k2[4]), (th2,)
+    th3 = th + hh * k2[1]
+    k3 = kerr.geodesic_rhs_vec(r + hh * k2[0], th3, pr + hh * k2[3], pth + hh * k2[4], pt, pph, a)
+    th4 = th + h * k3[1]
+    k4 = kerr.geodesic_rhs_vec(r + h * k3[0], th4, pr + h * k3[3], pth + h * k3[4], pt, pph, a)
+    w = h / 6.0
+    end = tuple(y + w * (k1[j] + 2.0 * k2[j] + 2.0 * k3[j] + k4[j]) for j, y in enumerate((r, th, ph, pr, pth)))
+    return end, (th2, th3, th4)
+
+
 def render_rays(scene, brick, pix: np.ndarray | None = None):
     """Render the pixels ``pix`` (flat indices y*W + x; all pixels when None).
 
@@ -92,6 +114,10 @@ def render_rays(scene, brick, pix: np.ndarray | None = None):
     tau_max = float(scene.max_optical_depth)
     r_esc = float(scene.escape_radius)
     exact = bool(getattr(scene, "exact_step", False))
+    rk4 = getattr(scene, "integrator", "rk2") == "rk4"
+    numpy32 = bool(getattr(scene, "numpy32_sampling", False))
+    if numpy32:
+        coeffs32 = brick.coeffs.astype(np.float32)
 
     S_out = np.zeros((n, 4))
     steps_out = np.zeros(n, dtype=np.int32)
@@ -106,24 +132,20 @@ def render_rays(scene, brick, pix: np.ndarray | None = None):
         if idx.size == 0:
             break
         h = h0 * (r - r_plus) if adaptive else np.full(r.shape, h0)
-        k1 = kerr.geodesic_rhs_vec(r, th, pr, pth, pt, pph, a)
-        hh = 0.5 * h
-        mth_ = th + hh * k1[1]
-        k2 = kerr.geodesic_rhs_vec(r + hh * k1[0], mth_, pr + hh * k1[3], pth + hh * k1[4], pt, pph, a)
-        nr_ = r + h * k2[0]
-        nth_raw = th + h * k2[1]
+        (nr_, nth_raw, nph, npr, npth), stages = geodesic_step_vec(r, th, ph, pr, pth, pt, pph, h, a, rk4)
         nth = kerr.safe_theta(nth_raw)
-        clamped |= (mth_ < lo) | (mth_ > hi) | (nth_raw < lo) | (nth_raw > hi)
-        nph = ph + h * k2[2]
+        for t_ in stages + (nth_raw,):
+            clamped |= (t_ < lo) | (t_ > hi)
         clamped |= np.abs(nph - ph) > POLE_DPHI
-        npr = pr + h * k2[3]
-        npth = pth + h * k2[4]
         finite = np.isfinite(nr_) & np.isfinite(nth) & np.isfinite(nph) & np.isfinite(npr) & np.isfinite(npth)
 
         # transfer segment (accelerated_renderer.py:113-125)
         with np.errstate(invalid="ignore", over="ignore"):
             mr, mth, mph = 0.5 * (r + nr_), 0.5 * (th + nth), 0.5 * (ph + nph)
-            c, valid = micro.sample_vec(coeffs, rg, tg, pg, mr, mth, mph)
+            if numpy32:
+                c, valid = micro.sample_vec_numpy32(coeffs32, brick.r, brick.theta, brick.phi, mr, mth, mph)
+            else:
+                c, valid = micro.sample_vec(coeffs, rg, tg, pg, mr, mth, mph)
             dr, dth, dph = nr_ - r, nth - th, nph - ph
             dlam = np.sqrt(dr * dr + dth * dth + dph * dph)
             ds = _pymax(L * _pymax(dlam, 1.0e-14), 0.0)
