@@ -154,12 +154,63 @@ def sampler_path(rr, brick, n: int, reps: int) -> dict:
     pick = rng.choice(n, 4096, replace=False)
     ref, _ = micro.sample_vec(coeffs, *grids, pts[pick, 0], pts[pick, 1], pts[pick, 2])
     err = float(np.max(np.abs(out[pick] - ref) / np.maximum(np.abs(ref), 1e-300)))
+    # the reference's parity gate (native_kernels.py:12-15): |a - b| <= ATOL + RTOL |b|, RTOL = ATOL = 1e-12
+    gate = float(np.max(np.abs(out[pick] - ref) / (1e-12 + 1e-12 * np.abs(ref))))
     ms = float(np.mean(kms))
     gbs = n * 112.0 / (ms * 1e-3) / 1e9
     for b in bufs + [d_out]:
         b.free()
     return {"points": n, "kernel_ms": ms, "gpoints_s": n / (ms * 1e-3) / 1e9, "algorithmic_GBs": gbs,
-            "hbm_frac": gbs / PEAK_HBM_GBS, "bytes_per_point": 112, "max_rel_err_vs_oracle": err}
+            "hbm_frac": gbs / PEAK_HBM_GBS, "bytes_per_point": 112, "max_rel_err_vs_oracle": err,
+            "checked_points": int(pick.size), "allclose_1e-12": gate <= 1.0, "allclose_margin": gate}
+
+
+def micro_path(calls: int) -> dict:
+    """The small configuration's sampler micro-benchmark: sample_and_step_stokes (the drop-in for
+    blackhole_native, invalid_policy='zero') on the 8x6x8 brick at 64 seeded uniform points (r and theta
+    drawn 10 % beyond the brick so the mask and the policy run), one iteration per call as the reference's
+    benchmark defaults (benchmark_cli.py:18-22).  Host arrays in and out: the figure is the call's latency
+    (validation, copies, launch, synchronisation), reported as the median of ``calls`` calls; the FP64
+    oracle's NumPy time for the same call is the CPU figure."""
+    from bhgrrt import native_kernels as nk
+    from bhgrrt import scenes
+    from oracle import micro
+
+    br = scenes.small_brick()
+    coeffs = br.coeffs.astype(np.float64)
+    rg, tg, pg = (g.astype(np.float64) for g in (br.r, br.theta, br.phi))
+    rng = np.random.default_rng(20240605)
+    n = 64
+    pts = np.empty((n, 3))
+    span_r, span_t = rg[-1] - rg[0], tg[-1] - tg[0]
+    pts[:, 0] = rng.uniform(rg[0] - 0.1 * span_r, rg[-1] + 0.1 * span_r, n)
+    pts[:, 1] = rng.uniform(tg[0] - 0.1 * span_t, tg[-1] + 0.1 * span_t, n)
+    pts[:, 2] = rng.uniform(0.0, 2.0 * math.pi, n)
+    init = rng.normal(size=(n, 4))
+    ds = 0.37
+
+    def gpu():
+        return nk.sample_and_step_stokes(coeffs, rg, tg, pg, pts, ds, init, invalid_policy="zero")
+
+    def cpu():
+        return micro.sample_and_step_stokes_vec(coeffs, rg, tg, pg, pts, ds, init, invalid_policy="zero")
+
+    out, ref = gpu(), cpu()
+    gate = float(np.max(np.abs(out - ref) / (1e-12 + 1e-12 * np.abs(ref))))
+    t_gpu, t_cpu = [], []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        gpu()
+        t_gpu.append(time.perf_counter() - t0)
+    for _ in range(max(1, calls // 10)):
+        t0 = time.perf_counter()
+        cpu()
+        t_cpu.append(time.perf_counter() - t0)
+    valid = nk.sample_brick_valid_mask(rg, tg, pts)
+    return {"points": n, "iterations_per_call": 1, "calls": calls, "latency_us_median": float(np.median(t_gpu)) * 1e6,
+            "latency_us_p90": float(np.quantile(t_gpu, 0.9)) * 1e6, "invalid_points": int(n - valid.sum()),
+            "allclose_1e-12_vs_oracle": gate <= 1.0, "allclose_margin": gate,
+            "cpu_oracle_latency_us_median": float(np.median(t_cpu)) * 1e6}
 
 
 def precompute_path(rr, dims, reps: int, cpu_cells: int) -> dict:
@@ -446,15 +497,45 @@ def main(argv=None) -> int:
     paths["batch64_mixed_bf16"] = {"mrays_s": rays_b / dt / 1e6, "ms_per_launch": dt * 1e3, "kernel_ms": kms,
                                    "rays": rays_b, "lane_utilisation": util, "tuning": tuned["best"],
                                    "step_histogram": {"edges": edges, "counts": hist}}
-    log(f"[bench] batch64: {paths['batch64_mixed_bf16']}")
     bt.close()
     for b in bricks:
         rr.release(b)
+    # error of every render of the batch against the FP64 batch of the same scenes
+    bricks64 = [rr.upload(b, "f32", "soa") for b in scenes.batch_bricks()]
+    bt64 = Batch(rr, [(sc, bricks64[k]) for sc, k in scenes.batch_scenes(64)], PREC_FP64)
+    bt64.launch(graph=False)
+    outs64 = bt64.results()
+    bt64.close()
+    for b in bricks64:
+        rr.release(b)
+    l1s, pix, quv, flag_diff, num, den = [], [], 0.0, 0, 0.0, 0.0
+    for m, r in zip(outs, outs64):
+        ok = ((m["status"] | r["status"]) & 8) == 0
+        I, I0 = m["iquv"][..., 0].astype(np.float64), r["iquv"][..., 0]
+        l1s.append(rel_l1(I[ok], I0[ok]))
+        num += float(np.abs(I - I0)[ok].sum())
+        den += float(np.abs(I0[ok]).sum())
+        scale = float(np.abs(I0[ok]).max())
+        big = ok & (np.abs(I0) > 1e-6 * scale)
+        pix.append(np.abs(I - I0)[big] / np.abs(I0[big]))
+        quv = max(quv, float(np.max(np.abs(m["iquv"][..., 1:].astype(np.float64) - r["iquv"][..., 1:])[ok])) / scale)
+        flag_diff += int(np.sum(((m["status"] ^ r["status"]) & 8) != 0))
+    pix = np.concatenate(pix)
+    paths["batch64_mixed_bf16"].update({
+        "rel_l1_I": num / den, "rel_l1_I_worst_render": max(l1s), "max_abs_QUV_over_max_I": quv,
+        "pixel_rel_I": {"pixels": int(pix.size), "median": float(np.median(pix)), "p99": float(np.quantile(pix, 0.99)),
+                        "max": float(pix.max())},
+        "pixels_flagged_in_one_render_only": flag_diff})
+    log(f"[bench] batch64: {paths['batch64_mixed_bf16']}")
 
     # ---- sampler at large N ----------------------------------------------------------------
     log(f"[bench] FP64 sampler, {args.sampler_points} points")
     paths["sampler_fp64"] = sampler_path(rr, brick, args.sampler_points, args.steps)
     log(f"[bench] sampler: {paths['sampler_fp64']}")
+
+    log("[bench] sampler micro-benchmark: sample_and_step_stokes, 8x6x8 brick, 64 points, 1 iteration")
+    paths["sampler_micro_64"] = micro_path(400)
+    log(f"[bench] sampler micro: {paths['sampler_micro_64']}")
 
     # ---- coefficient-brick precompute (the input side of the path) ------------------------
     dims = tuple(int(v) for v in args.precompute_grid.split("x"))

@@ -132,6 +132,8 @@ SIGNATURES = [
     ("bh_mlp_shape", C.c_int, [C.c_void_p, _i32p, _i32p, C.POINTER(C.c_int64)]),
     ("bh_mlp_set_params", C.c_int, [C.c_void_p, _fp]),
     ("bh_mlp_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("bh_mlp_get_adam", C.c_int, [C.c_void_p, _fp, _fp, C.POINTER(C.c_int64)]),
+    ("bh_mlp_set_adam", C.c_int, [C.c_void_p, _fp, _fp, C.c_int64]),
     ("bh_mlp_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     ("bh_mlp_set_fused", C.c_int, [C.c_void_p, C.c_int32]),
     ("bh_mlp_gradient", C.c_int,

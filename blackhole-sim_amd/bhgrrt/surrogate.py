@@ -100,6 +100,17 @@ class MLP:
         _lib.check(self._lib.bh_mlp_get_params(self.h, _lib.fptr(v)))
         return self.unflat(v)
 
+    def get_adam(self) -> tuple[np.ndarray, np.ndarray, int]:
+        """Adam's first and second moments (flat, the parameter layout) and its step count."""
+        m, v, t = np.empty(self.n_params, np.float32), np.empty(self.n_params, np.float32), C.c_int64(0)
+        _lib.check(self._lib.bh_mlp_get_adam(self.h, _lib.fptr(m), _lib.fptr(v), C.byref(t)))
+        return m, v, int(t.value)
+
+    def set_adam(self, m: np.ndarray, v: np.ndarray, steps: int) -> None:
+        m, v = (np.ascontiguousarray(a, dtype=np.float32) for a in (m, v))
+        assert m.size == self.n_params and v.size == self.n_params
+        _lib.check(self._lib.bh_mlp_set_adam(self.h, _lib.fptr(m), _lib.fptr(v), int(steps)))
+
     def set_fused(self, fused: bool) -> None:
         """Transposed inference through the fused all-layer kernel (default) or the per-layer GEMMs."""
         _lib.check(self._lib.bh_mlp_set_fused(self.h, int(bool(fused))))
@@ -370,18 +381,26 @@ class Surrogate:
 
     # ---- checkpoints ---------------------------------------------------------------------
     def save(self, path) -> None:
+        """Weights, normalisation, loss history and the optimizer state (Adam's moments and step count)."""
         params = self.mlp.get_params()
         arrs = {f"W{i}": W for i, (W, _) in enumerate(params)}
         arrs.update({f"b{i}": b for i, (_, b) in enumerate(params)})
+        adam_m, adam_v, adam_t = self.mlp.get_adam()
         nm = self.norm
         np.savez(path, in_mul=nm.in_mul, in_add=nm.in_add, out_mul=nm.out_mul, out_add=nm.out_add,
-                 history=np.array(self.history, dtype=np.float64).reshape(-1, 2), **arrs)
+                 history=np.array(self.history, dtype=np.float64).reshape(-1, 2), adam_m=adam_m, adam_v=adam_v,
+                 adam_t=np.array(adam_t, dtype=np.int64), **arrs)
 
     def load(self, path) -> None:
+        """Restore a checkpoint of ``save``; training continues from its Adam state when the file has one."""
         with np.load(path, allow_pickle=False) as z:
             params = [(z[f"W{i}"], z[f"b{i}"]) for i in range(len(self.mlp.shapes))]
             self.norm = Normalisation(z["in_mul"], z["in_add"], z["out_mul"], z["out_add"])
+            adam = (z["adam_m"], z["adam_v"], int(z["adam_t"])) if "adam_m" in z.files else None
+            self.history = [tuple(h) for h in z["history"]] if "history" in z.files else []
         self.mlp.set_params(params)
+        if adam is not None:
+            self.mlp.set_adam(*adam)
 
     def close(self) -> None:
         if self._work is not None and self.r.ctx:

@@ -783,6 +783,31 @@ int bh_mlp_get_params(bh_mlp *m, float *host) {
     return BH_OK;
 }
 
+int bh_mlp_get_adam(bh_mlp *m, float *host_m, float *host_v, int64_t *steps) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!host_m || !host_v || !steps) return set_error(BH_ERR_VALUE, "NULL Adam state buffer");
+    BH_HIP(hipMemcpyAsync(host_m, m->adam_m, m->n_params * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    BH_HIP(hipMemcpyAsync(host_v, m->adam_v, m->n_params * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    BH_HIP(hipStreamSynchronize(m->ctx->stream));
+    *steps = m->adam_t;
+    return BH_OK;
+}
+
+int bh_mlp_set_adam(bh_mlp *m, const float *host_m, const float *host_v, int64_t steps) {
+    clear_error();
+    int st = mlp_check(m);
+    if (st) return st;
+    if (!host_m || !host_v) return set_error(BH_ERR_VALUE, "NULL Adam state buffer");
+    if (steps < 0) return set_error(BH_ERR_VALUE, "the Adam step count must be non-negative");
+    BH_HIP(hipMemcpyAsync(m->adam_m, host_m, m->n_params * 4, hipMemcpyHostToDevice, m->ctx->stream));
+    BH_HIP(hipMemcpyAsync(m->adam_v, host_v, m->n_params * 4, hipMemcpyHostToDevice, m->ctx->stream));
+    BH_HIP(hipStreamSynchronize(m->ctx->stream));
+    m->adam_t = steps;
+    return BH_OK;
+}
+
 int bh_mlp_forward(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int32_t transposed) {
     clear_error();
     int st = mlp_check(m);

@@ -263,6 +263,10 @@ int bh_mlp_destroy(bh_mlp *mlp);
 int bh_mlp_shape(bh_mlp *mlp, int32_t *n_in_pad, int32_t *n_out_pad, int64_t *n_params);
 int bh_mlp_set_params(bh_mlp *mlp, const float *host_params);   /* also resets Adam state */
 int bh_mlp_get_params(bh_mlp *mlp, float *host_params);
+/* Adam's first and second moments (n_params floats each, host) and its step count, for checkpoints:
+ * bh_mlp_set_adam after bh_mlp_set_params resumes training where bh_mlp_get_adam left it. */
+int bh_mlp_get_adam(bh_mlp *mlp, float *host_m, float *host_v, int64_t *steps);
+int bh_mlp_set_adam(bh_mlp *mlp, const float *host_m, const float *host_v, int64_t steps);
 /* Inference: d_x [rows][n_in] -> d_y [rows][n_out], or [n_out][rows] when transposed.
  * Transposed inference of width-256 networks (n_in <= 8, n_out <= 1024) runs one fused kernel
  * (activations kept in LDS) unless bh_mlp_set_fused(mlp, 0) selects the per-layer GEMMs. */

@@ -1,9 +1,9 @@
 """Fixed workload for rocprofv3 runs: the 1024^2 Kerr Stokes scene on every path, plus a calibration copy.
 
     rocprofv3 --kernel-trace --stats --output-format csv -d OUT -o run -- \\
-        python3 blackhole-sim_amd/bhgrrt/profile_workload.py
+        python3 profile_workload.py
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
-        python3 blackhole-sim_amd/bhgrrt/profile_workload.py
+        python3 profile_workload.py
 
 Render paths are storage:layout:precision[:exact] (``exact`` selects the exact Stokes step); each is
 rendered twice (the second launch is the measured one).  ``--surrogate`` adds the learned surrogate: FP64 geodesic knots for
@@ -25,7 +25,7 @@ import math
 import sys
 from pathlib import Path
 
-_ROOT = Path(__file__).resolve().parents[2]
+_ROOT = Path(__file__).resolve().parent
 for _p in (_ROOT, _ROOT / "blackhole-sim_amd"):
     if str(_p) not in sys.path:
         sys.path.insert(0, str(_p))

@@ -2,9 +2,10 @@
 polarized_transfer.py:51-80) against the FP64 oracle, which takes scipy's expm of the augmented
 matrix like the reference.
 
-* C0 (32x18, fixed h = 20) and the scale scene at 48^2 in FP64: statuses and step counts equal,
-  relative I error median <= 1e-10 (C0) and <= 1e-9 on 99 % of unflagged pixels (scale scene).
-* FP32 and mixed-precision exact renders against the FP64 exact render: rel-L1(I) <= 2e-3 / 1e-3.
+* C0 (32x18, fixed h = 20) and the scale scene at 48^2 in FP64: statuses and step counts equal; on C0
+  relative I error <= 1e-8 on every pixel the oracle shows to be well conditioned at FP64 rounding
+  (tests/conditioning.py), on the scale scene <= 1e-9 on 99 % of unflagged pixels.
+* FP32 and mixed-precision exact renders against the FP64 exact render: rel-L1(I) <= 1e-4 / 1e-3.
 * A batch of exact renders equals the single renders; the exact image differs from the RK2 one.
 * The surrogate's knot transfer takes the RK2 step only and says so.
 """
@@ -16,6 +17,7 @@ import pytest
 
 from bhgrrt import scenes
 from bhgrrt.render import PREC_FP32, PREC_FP64, PREC_MIXED, Batch
+from conditioning import well_conditioned
 from oracle import render as orender
 
 pytestmark = pytest.mark.gpu
@@ -37,8 +39,11 @@ def test_small_scene_exact_fp64_matches_oracle(renderer):
     np.testing.assert_array_equal(out["steps"], ref["steps"])
     I, I0 = out["iquv"][..., 0], ref["iquv"][..., 0]
     rel = np.abs(I - I0) / np.maximum(np.abs(I0), 1e-300)
-    print("C0 exact fp64: median rel", np.median(rel), "max", rel.max())
-    assert np.median(rel) <= 1e-10
+    cond = well_conditioned(sc, br, 1e-12, 1e-6, ref)
+    print(f"C0 exact fp64: {cond.mean():.3f} well conditioned, max rel there {rel[cond].max():.2e}; all pixels "
+          f"median {np.median(rel):.2e} max {rel.max():.2e}")
+    assert cond.mean() >= 0.5
+    assert rel[cond].max() <= 1e-8
 
 
 @pytest.fixture(scope="module")
@@ -62,7 +67,7 @@ def test_scale_scene_exact_fp64_matches_oracle(scale_exact):
     assert np.quantile(rel, 0.99) <= 1e-9
 
 
-@pytest.mark.parametrize("storage,layout,precision,bar", [("f32", "corner", PREC_FP32, 2e-3),
+@pytest.mark.parametrize("storage,layout,precision,bar", [("f32", "corner", PREC_FP32, 1e-4),
                                                          ("bf16", "corner", PREC_MIXED, 1e-3)])
 def test_scale_scene_exact_reduced_precision(renderer, scale_exact, storage, layout, precision, bar):
     sc, br, _, out64 = scale_exact

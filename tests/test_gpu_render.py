@@ -1,22 +1,26 @@
 """GPU parity of the fused render (FP64 and plain FP32) against the FP64 oracle.
 
-* Small plain scene (a=0.9, r_obs=1000, i=60 deg, 32x18, 64 fixed steps of h=20,
-  8x6x8 brick): every pixel against the oracle.
-* Scale scene (a=0.94, i=17 deg, 1024^2, h = 0.05 (r - r_+), <= 4096 steps,
-  128x64x128 disk brick): a seeded pixel subset against the oracle, plus
-  size-independent checks on the full image (FP32 vs FP64 agreement, step and
-  status statistics).
-
-Rays flagged by the theta clamp (status bit 3) left the geodesic at a pole, so
-their Stokes values are excluded from error statistics (counted instead).  In
-the small scene, 64 fixed steps of h = 20 from r = 1000 overstep a pole on
-almost every ray (and the explicit Stokes step is unstable where alpha ds > 2),
-so there every pixel is compared: statuses and step counts exactly, values to
-the accuracy the amplification allows.
-Tolerances: FP64 GPU vs FP64 oracle -- identical status and step count,
-per-pixel relative I error <= 1e-9 on 99% of unflagged pixels of the scale
-scene; plain FP32 vs FP64 -- relative L1 error on I <= 2e-3 over unflagged
-pixels of the scale scene.
+* Small plain scene (configs[0]: a=0.9, r_obs=1000, i=60 deg, 32x18, 64 fixed RK2 steps of h=20, 8x6x8
+  brick).  Steps of 20 M jump across the coordinate axis near the hole, so every ray is pole-flagged and
+  most of them amplify rounding errors enormously (a 1-ulp change of the camera moves I by 38 % at the
+  90th percentile in the oracle itself).  Bars: statuses and step counts equal to the oracle on every
+  pixel (FP64); I within 1e-9 relative on every pixel the oracle shows to be well conditioned at FP64
+  rounding (I moves < 1e-6 under a 1e-12 change of the camera; 72 % of the pixels; measured 7e-11);
+  FP32 against the FP64 render on the pixels well conditioned at FP32 rounding (I moves < 1e-2 under a
+  1e-6 change; 64 %): statuses and steps on >= 45 % of them (measured 56 %: FP32 rounding inside a
+  step that crosses the axis is not a perturbation of the camera) and I within 5e-4 on every one of
+  those that agree (measured 4.7e-5).
+* Scale scene (configs[1]: a=0.94, i=17 deg, 1024^2, h = 0.05 (r - r_+), <= 4096 steps, 128x64x128 disk
+  brick): a seeded pixel subset of the FP64 render against the oracle (statuses and steps on >= 99 % of
+  unflagged pixels, I within 1e-9 on 99 % of those that agree); the full FP32 image against the FP64
+  render: relative L1 error of I <= 1e-4, per-pixel relative I error and max(|dQ|, |dU|, |dV|) / |I|
+  <= 1e-4 at the 99th percentile, max |dQUV| <= 3e-2 of the image's largest |I| (a few near-critical
+  rays diverge: the oracle confirms the FP64 values of the worst pixels), at most 0.1 % of the pixels
+  flagged in one render only.  Mixed precision against the FP64 render: relative L1 error
+  of I <= 1e-3 (bf16) / 2e-4 (f16), per-pixel p99 of I and Q/U/V <= 3e-3 / 5e-4, the same max and flag
+  bars.
+* Rays flagged by the theta clamp (status bit 3) left the geodesic at a pole: on the scale scene their
+  Stokes values are excluded from error statistics and counted.
 """
 
 import numpy as np
@@ -24,6 +28,7 @@ import pytest
 
 from bhgrrt import scenes
 from bhgrrt.render import PREC_FP32, PREC_FP64, PREC_MIXED
+from conditioning import well_conditioned
 from oracle import render as orender
 
 pytestmark = pytest.mark.gpu
@@ -49,19 +54,30 @@ def test_small_scene_fp64_matches_oracle(renderer, small):
     np.testing.assert_array_equal(out["steps"], ref["steps"])
     I, I0 = out["iquv"][..., 0], ref["iquv"][..., 0]
     rel = np.abs(I - I0) / np.maximum(np.abs(I0), 1e-300)
-    print("small fp64: median rel", np.median(rel), "p90", np.quantile(rel, 0.9), "max", rel.max())
-    assert np.median(rel) <= 1e-10
+    cond = well_conditioned(sc, br, 1e-12, 1e-6, ref)
+    print(f"small fp64: {cond.mean():.3f} well conditioned, max rel there {rel[cond].max():.2e}; all pixels median "
+          f"{np.median(rel):.2e} p90 {np.quantile(rel, 0.9):.2e} max {rel.max():.2e}")
+    assert cond.mean() >= 0.5
+    assert rel[cond].max() <= 1e-9
 
 
-def test_small_scene_fp32_runs_the_same_schedule(renderer, small):
-    """h = 20 makes every FP32 ray chaotic here, so only schedule-level properties are compared."""
+def test_small_scene_fp32_against_fp64_on_conditioned_rays(renderer, small):
     sc, br, db, ref = small
-    out = renderer.render(sc, db, PREC_FP32)
-    assert np.all(np.isfinite(out["iquv"]))
-    assert out["steps"].min() >= 1 and out["steps"].max() <= sc.max_steps
-    agree = (out["status"] & 7) == (ref["status"] & 7)
-    print("small fp32: termination agreement", agree.mean(), "steps agreement", (out["steps"] == ref["steps"]).mean())
-    assert agree.mean() >= 0.3
+    o32 = renderer.render(sc, db, PREC_FP32)
+    o64 = renderer.render(sc, db, PREC_FP64)
+    assert np.all(np.isfinite(o32["iquv"]))
+    assert o32["steps"].min() >= 1 and o32["steps"].max() <= sc.max_steps
+    cond = well_conditioned(sc, br, 1e-6, 1e-2, ref)
+    same = (o32["status"] == o64["status"]) & (o32["steps"] == o64["steps"])
+    I, I0 = o32["iquv"][..., 0].astype(np.float64), o64["iquv"][..., 0]
+    rel = np.abs(I - I0) / np.maximum(np.abs(I0), 1e-300)
+    agree = cond & same
+    print(f"small fp32: {cond.mean():.3f} well conditioned at FP32 rounding, statuses+steps there "
+          f"{same[cond].mean():.3f}, rel I p95 {np.quantile(rel[agree], 0.95):.2e} max {rel[agree].max():.2e}; "
+          f"all pixels: termination agreement {((o32['status'] & 7) == (o64['status'] & 7)).mean():.3f}")
+    assert cond.mean() >= 0.4
+    assert same[cond].mean() >= 0.45
+    assert rel[agree].max() <= 5e-4
 
 
 def test_scale_scene_fp32_matches_oracle_at_48(renderer):
@@ -77,7 +93,7 @@ def test_scale_scene_fp32_matches_oracle_at_48(renderer):
     assert np.mean((out["status"] == ref["status"])[ok]) >= 0.99
     e = rel_l1(out["iquv"][..., 0][ok].astype(np.float64), ref["iquv"][..., 0][ok])
     print("48^2 fp32 vs oracle rel L1 I", e)
-    assert e <= 2e-3
+    assert e <= 1e-4
 
 
 @pytest.fixture(scope="module")
@@ -108,34 +124,67 @@ def test_scale_scene_fp64_matches_oracle_on_pixel_subset(scale):
     assert rel_l1(I[ok, 0], ref["iquv"][ok, 0]) <= 1e-7
 
 
+def _image_errors(out, out64):
+    """Errors of a render against the FP64 render over pixels flagged in neither: relative L1 of I,
+    per-pixel relative I (pixels with |I| above 1e-6 of the largest), max |dQUV| / max |I|, and the number
+    of pixels flagged in one render only."""
+    ok = ((out64["status"] | out["status"]) & orender.ST_POLE_CLAMPED) == 0
+    I, I64 = out["iquv"][..., 0].astype(np.float64), out64["iquv"][..., 0]
+    scale = np.abs(I64[ok]).max()
+    big = ok & (np.abs(I64) > 1e-6 * scale)
+    pix = np.abs(I - I64)[big] / np.abs(I64[big])
+    dquv = np.max(np.abs(out["iquv"][..., 1:].astype(np.float64) - out64["iquv"][..., 1:]), axis=-1)
+    quv = float(dquv[ok].max()) / scale
+    pix_quv = dquv[big] / np.abs(I64[big])
+    flag_diff = int(np.sum(((out["status"] ^ out64["status"]) & orender.ST_POLE_CLAMPED) != 0))
+    return rel_l1(I[ok], I64[ok]), pix, quv, flag_diff, ok, pix_quv
+
+
 def test_scale_scene_fp32_error_against_fp64(scale):
     sc, br, db, out64, out32 = scale
-    ok = ((out64["status"] | out32["status"]) & orender.ST_POLE_CLAMPED) == 0
-    I64 = out64["iquv"][..., 0][ok]
-    I32 = out32["iquv"][..., 0][ok].astype(np.float64)
-    e = rel_l1(I32, I64)
-    qu = np.max(np.abs(out32["iquv"][..., 1:][ok] - out64["iquv"][..., 1:][ok]))
-    print("scale fp32 vs fp64: rel L1 I", e, "max abs QUV", qu, "flagged frac", 1 - ok.mean())
-    assert e <= 2e-3
+    e, pix, quv, flag_diff, ok, pix_quv = _image_errors(out32, out64)
+    print(f"scale fp32 vs fp64: rel L1 I {e:.2e}, pixel rel I p99 {np.quantile(pix, 0.99):.2e} max {pix.max():.2e}, "
+          f"pixel |dQUV| / |I| p99 {np.quantile(pix_quv, 0.99):.2e}, max |dQUV| / max|I| {quv:.2e}, "
+          f"flagged {1 - ok.mean():.4f}, flagged in one render only {flag_diff}")
+    assert e <= 1e-4
+    assert np.quantile(pix, 0.99) <= 1e-4
+    assert np.quantile(pix_quv, 0.99) <= 1e-4
+    assert quv <= 3e-2
+    assert flag_diff <= 0.001 * ok.size
+    # the pixels where FP32 is furthest from FP64: the FP64 render agrees with the oracle there, so the
+    # outliers are FP32 rays that diverged (near-critical orbits amplify their rounding)
+    I32, I64 = out32["iquv"][..., 0].astype(np.float64).ravel(), out64["iquv"][..., 0].ravel()
+    okf = ok.ravel() & (np.abs(I64) > 1e-6 * np.abs(I64[ok.ravel()]).max())
+    cand = np.flatnonzero(okf)
+    worst = cand[np.argsort(np.abs(I32 - I64)[cand] / np.abs(I64[cand]))[-4:]]
+    o = orender.render_rays(sc, br, worst)
+    print("worst fp32 pixels:", worst.tolist(), "steps", out64["steps"].ravel()[worst].tolist(), "fp32 rel",
+          (np.abs(I32 - I64)[worst] / np.abs(I64[worst])).tolist())
+    np.testing.assert_array_equal(o["steps"], out64["steps"].ravel()[worst])
+    np.testing.assert_allclose(o["iquv"][:, 0], I64[worst], rtol=1e-9, atol=0)
     steps = out32["steps"]
     assert steps.min() >= 1 and steps.max() <= sc.max_steps
     assert 100 < steps.mean() < 400
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f16"])
-def test_scale_scene_mixed_precision_error_against_fp64(renderer, scale, storage):
-    """BF16/FP16 brick, FP32 geodesic with FP64 islands: relative L1 on I <= 1e-3 vs the FP64 render."""
+@pytest.mark.parametrize("storage,l1_bar,p99_bar", [("bf16", 1e-3, 3e-3), ("f16", 2e-4, 5e-4)])
+def test_scale_scene_mixed_precision_error_against_fp64(renderer, scale, storage, l1_bar, p99_bar):
+    """BF16/FP16 brick, FP32 geodesic with FP64 islands, against the FP64 render."""
     sc, br, db, out64, _ = scale
     dbx = renderer.upload(br, storage, "corner")
     try:
         out = renderer.render(sc, dbx, PREC_MIXED)
     finally:
         renderer.release(dbx)
-    ok = ((out64["status"] | out["status"]) & orender.ST_POLE_CLAMPED) == 0
-    e = rel_l1(out["iquv"][..., 0][ok].astype(np.float64), out64["iquv"][..., 0][ok])
-    qu = np.max(np.abs(out["iquv"][..., 1:][ok] - out64["iquv"][..., 1:][ok]))
-    print(f"scale mixed {storage} vs fp64: rel L1 I {e:.3e} max abs QUV {qu:.3e} flagged {1 - ok.mean():.4f}")
-    assert e <= 1e-3
+    e, pix, quv, flag_diff, ok, pix_quv = _image_errors(out, out64)
+    print(f"scale mixed {storage} vs fp64: rel L1 I {e:.3e}, pixel rel I p99 {np.quantile(pix, 0.99):.2e} max "
+          f"{pix.max():.2e}, pixel |dQUV| / |I| p99 {np.quantile(pix_quv, 0.99):.2e}, max |dQUV| / max|I| "
+          f"{quv:.2e}, flagged {1 - ok.mean():.4f}, one render only {flag_diff}")
+    assert e <= l1_bar
+    assert np.quantile(pix, 0.99) <= p99_bar
+    assert np.quantile(pix_quv, 0.99) <= p99_bar
+    assert quv <= 3e-2
+    assert flag_diff <= 0.001 * ok.size
 
 
 @pytest.mark.parametrize("storage,precision", [("f32", PREC_FP32), ("bf16", PREC_FP32), ("bf16", PREC_MIXED),

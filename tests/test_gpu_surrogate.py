@@ -262,3 +262,38 @@ def test_fused_inference_matches_layer_gemms_and_oracle(renderer, rows):
     for d in (dx, dy_f, dy_g):
         d.free()
     mlp.close()
+
+
+def test_checkpoint_resumes_training_with_the_optimizer_state(renderer, tmp_path):
+    """save() keeps Adam's moments and step count; a surrogate loaded from the file takes the same next steps
+    as the one that wrote it (to the atomics' summation order), while the same weights with a fresh Adam state
+    step elsewhere."""
+    print("surrogate checkpoint resume", flush=True)
+    br = scenes.scale_brick(32, 16, 32)
+    cfg = bsur.SurrogateConfig(spin=(0.9, 0.98), inclination_deg=(10.0, 25.0))
+    a = bsur.Surrogate(renderer, br, cfg, seed=3)
+    data = a.generate(1 << 13, 5)
+    a.train(data, 20, batch=2048, lr=1e-3, log_every=10)
+    path = tmp_path / "surrogate.npz"
+    a.save(path)
+    b = bsur.Surrogate(renderer, br, cfg, seed=4)
+    b.load(path)
+    ma, va, ta = a.mlp.get_adam()
+    mb, vb, tb = b.mlp.get_adam()
+    assert ta == tb == 20
+    np.testing.assert_array_equal(ma, mb)
+    np.testing.assert_array_equal(va, vb)
+    c = bsur.Surrogate(renderer, br, cfg, seed=5)
+    c.load(path)
+    c.mlp.set_params(c.mlp.get_params())   # same weights, Adam state reset
+    for s in (a, b, c):
+        s.train(data, 5, batch=2048, lr=1e-3, log_every=5)
+    pa, pb, pc = (np.concatenate([np.concatenate([W.ravel(), bb]) for W, bb in s.mlp.get_params()]) for s in (a, b, c))
+    d_ab, d_ac = float(np.abs(pa - pb).max()), float(np.abs(pa - pc).max())
+    print(f"after 5 more steps: max |a - b| {d_ab:.2e} (resumed), max |a - c| {d_ac:.2e} (fresh Adam state)")
+    assert d_ab <= 1e-6
+    assert d_ac >= 100 * max(d_ab, 1e-9)
+    for k in ("x", "t", "w"):
+        data[k].free()
+    for s in (a, b, c):
+        s.close()
