@@ -170,6 +170,7 @@ int bh_ctx_destroy(bh_ctx *ctx) {
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_order) (void)hipFree(ctx->d_order);
     if (ctx->d_synchro) (void)hipFree(ctx->d_synchro);
+    if (ctx->d_sampler_work) (void)hipFree(ctx->d_sampler_work);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -84,6 +84,8 @@ struct bh_ctx {
     std::vector<double> order_key;      // scene parameters the schedule was built for
     unsigned *sur_count = nullptr;      // fallback counter of the last surrogate render (in its work buffer)
     void *d_synchro = nullptr;          // coefficient-precompute tables (grow-only)
+    void *d_sampler_work = nullptr;     // large-N sampler: cell counts, point keys, ranks, order (grow-only)
+    size_t sampler_work_cap = 0;
     size_t synchro_cap = 0;
     std::vector<double> h_synchro;      // host copy the table upload reads from
 };

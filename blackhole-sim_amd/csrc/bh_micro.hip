@@ -11,6 +11,7 @@
 
 #include "bh_internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace bh {
@@ -67,6 +68,22 @@ __device__ __forceinline__ void bracket_phi_d(const double *g, int n, double phi
     w = (pp - g[i]) / den;
 }
 
+// One channel's trilinear value from its 8 corner values (o000 .. o111, phi fastest): the nested
+// (1 - w) a + w b lerps of lib.rs:199-205 and native_kernels.py:180-186, every product and sum rounded
+// on its own as Rust and NumPy evaluate them (no fused multiply-add), so all sampling kernels agree bit
+// for bit.
+__device__ __forceinline__ double trilerp_d(double v000, double v001, double v010, double v011, double v100,
+                                            double v101, double v110, double v111, double wr, double wt, double wp) {
+#pragma clang fp contract(off)
+    const double c00 = (1.0 - wp) * v000 + wp * v001;
+    const double c01 = (1.0 - wp) * v010 + wp * v011;
+    const double c10 = (1.0 - wp) * v100 + wp * v101;
+    const double c11 = (1.0 - wp) * v110 + wp * v111;
+    const double c0 = (1.0 - wt) * c00 + wt * c01;
+    const double c1 = (1.0 - wt) * c10 + wt * c11;
+    return (1.0 - wr) * c0 + wr * c1;
+}
+
 // lib.rs:171-208 over an AoS FP64 brick.
 __device__ __forceinline__ bool sample_aos_d(const double *__restrict__ c, const double *rg, int nr, const double *tg,
                                              int nt, const double *pg, int np, double r, double th, double ph,
@@ -86,15 +103,9 @@ __device__ __forceinline__ bool sample_aos_d(const double *__restrict__ c, const
     const size_t o110 = ((size_t(r1) * nt + t1) * np + p0) * kCoeffs;
     const size_t o111 = ((size_t(r1) * nt + t1) * np + p1) * kCoeffs;
 #pragma unroll
-    for (int k = 0; k < kCoeffs; ++k) {
-        double c00 = (1.0 - wp) * c[o000 + k] + wp * c[o001 + k];
-        double c01 = (1.0 - wp) * c[o010 + k] + wp * c[o011 + k];
-        double c10 = (1.0 - wp) * c[o100 + k] + wp * c[o101 + k];
-        double c11 = (1.0 - wp) * c[o110 + k] + wp * c[o111 + k];
-        double c0 = (1.0 - wt) * c00 + wt * c01;
-        double c1 = (1.0 - wt) * c10 + wt * c11;
-        out[k] = (1.0 - wr) * c0 + wr * c1;
-    }
+    for (int k = 0; k < kCoeffs; ++k)
+        out[k] = trilerp_d(c[o000 + k], c[o001 + k], c[o010 + k], c[o011 + k], c[o100 + k], c[o101 + k], c[o110 + k],
+                           c[o111 + k], wr, wt, wp);
     return true;
 }
 
@@ -166,13 +177,8 @@ __device__ __forceinline__ void gather_points(const double *__restrict__ coeffs,
                 const double *v = vals[g];
                 const int k = gl;
                 // corners in order (dr, dt, dp) = q >> 2, (q >> 1) & 1, q & 1, as o000 .. o111
-                const double c00 = (1.0 - wp) * v[0 * kCoeffs + k] + wp * v[1 * kCoeffs + k];
-                const double c01 = (1.0 - wp) * v[2 * kCoeffs + k] + wp * v[3 * kCoeffs + k];
-                const double c10 = (1.0 - wp) * v[4 * kCoeffs + k] + wp * v[5 * kCoeffs + k];
-                const double c11 = (1.0 - wp) * v[6 * kCoeffs + k] + wp * v[7 * kCoeffs + k];
-                const double c0 = (1.0 - wt) * c00 + wt * c01;
-                const double c1 = (1.0 - wt) * c10 + wt * c11;
-                res = (1.0 - wr) * c0 + wr * c1;
+                res = trilerp_d(v[0 * kCoeffs + k], v[1 * kCoeffs + k], v[2 * kCoeffs + k], v[3 * kCoeffs + k],
+                                v[4 * kCoeffs + k], v[5 * kCoeffs + k], v[6 * kCoeffs + k], v[7 * kCoeffs + k], wr, wt, wp);
             }
             out[i * kCoeffs + gl] = res;
         }
@@ -202,6 +208,216 @@ __global__ __launch_bounds__(256) void sample_gather_kernel(const double *__rest
         gather_points(coeffs, sgrid, nr, sgrid + nr, nt, sgrid + nr + nt, np, pts, n, out, vals);
     } else {
         gather_points(coeffs, rg, nr, tg, nt, pg, np, pts, n, out, vals);
+    }
+}
+
+// ---- points bucketed by brick cell ------------------------------------------------------------
+// Random points gather eight random 88-byte corner records each; sorted by the trilinear cell they fall
+// in, the points of a cell and of its phi neighbours (which share four corners) sit in neighbouring lanes,
+// so one lane per point loads its corners directly: the lanes of a load instruction hit a few records,
+// which come from L1/L2 instead of the fabric, and a wave keeps 64 points in flight.  A counting sort:
+// the histogram's atomic returns each point's rank inside its cell, an exclusive scan turns the counts
+// into cell offsets, and a scatter writes each point (coordinates, index, cell) to offset + rank.  The
+// gather takes the cell from the sort key instead of searching the grids; the weights and the lerps are
+// those of the unsorted gather (same expressions on the same cell), so the results are the same bits.
+
+constexpr int kScanBlock = 1024;   // threads of the scan kernels: 16 waves of 64
+
+// Inclusive prefix sum over the 1024 threads of a block; lds holds 16 words.
+__device__ __forceinline__ unsigned block_inclusive_scan(unsigned v, unsigned *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    if (lane == 63) lds[wave] = v;
+    __syncthreads();
+    if (wave == 0) {
+        unsigned t = lane < 16 ? lds[lane] : 0u;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            const unsigned u = __shfl_up(t, d, 64);
+            if (lane >= d) t += u;
+        }
+        if (lane < 16) lds[lane] = t;
+    }
+    __syncthreads();
+    if (wave > 0) v += lds[wave - 1];
+    __syncthreads();   // lds is free for the next call
+    return v;
+}
+
+// Cell of each point (cells in the brick's memory order, phi fastest; points outside the r/theta range go
+// to the extra cell n_cells) and its rank among the points of that cell.
+__global__ __launch_bounds__(256) void bucket_count_kernel(const double *__restrict__ rg, int nr,
+                                                           const double *__restrict__ tg, int nt,
+                                                           const double *__restrict__ pg, int np,
+                                                           const double *__restrict__ pts, int64_t n,
+                                                           unsigned *__restrict__ key, unsigned *__restrict__ rank,
+                                                           unsigned *__restrict__ count) {
+    __shared__ double sgrid[kGatherGrid];
+    const bool staged = nr + nt + np <= kGatherGrid;
+    if (staged) {
+        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
+            sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
+        __syncthreads();
+    }
+    const double *r_ = staged ? sgrid : rg;
+    const double *t_ = staged ? sgrid + nr : tg;
+    const double *p_ = staged ? sgrid + nr + nt : pg;
+    const unsigned n_cells = unsigned(nr - 1) * unsigned(nt - 1) * unsigned(np);
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const double *p = pts + i * 3;
+        int r0 = 0, t0 = 0, p0 = 0, p1 = 0;
+        double w = 0.0;
+        unsigned c = n_cells;
+        if (bracket_linear_d(r_, nr, p[0], r0, w) && bracket_linear_d(t_, nt, p[1], t0, w)) {
+            bracket_phi_d(p_, np, p[2], p0, p1, w);
+            c = (unsigned(r0) * unsigned(nt - 1) + unsigned(t0)) * unsigned(np) + unsigned(p0);
+        }
+        key[i] = c;
+        rank[i] = atomicAdd(&count[c], 1u);
+    }
+}
+
+// Exclusive scan of a[0, n) in chunks of 1024; the chunk totals go to sums.
+__global__ __launch_bounds__(kScanBlock) void scan_chunks_kernel(unsigned *__restrict__ a, unsigned n,
+                                                                 unsigned *__restrict__ sums) {
+    __shared__ unsigned lds[16];
+    const unsigned i = blockIdx.x * kScanBlock + threadIdx.x;
+    const unsigned v = i < n ? a[i] : 0u;
+    const unsigned inc = block_inclusive_scan(v, lds);
+    if (i < n) a[i] = inc - v;
+    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = inc;
+}
+
+// Exclusive scan of the chunk totals, in place, by one block.
+__global__ __launch_bounds__(kScanBlock) void scan_sums_kernel(unsigned *__restrict__ sums, unsigned nb) {
+    __shared__ unsigned lds[16];
+    __shared__ unsigned carry;
+    if (threadIdx.x == 0) carry = 0u;
+    __syncthreads();
+    for (unsigned base = 0; base < nb; base += kScanBlock) {
+        const unsigned i = base + threadIdx.x;
+        const unsigned v = i < nb ? sums[i] : 0u;
+        const unsigned inc = block_inclusive_scan(v, lds);
+        const unsigned c = carry;
+        if (i < nb) sums[i] = c + inc - v;
+        __syncthreads();
+        if (threadIdx.x == kScanBlock - 1) carry = c + inc;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_add_kernel(unsigned *__restrict__ a, unsigned n,
+                                                              const unsigned *__restrict__ sums) {
+    const unsigned i = blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) a[i] += sums[blockIdx.x];
+}
+
+struct alignas(16) SortedPoint {
+    double r, th, ph;
+    unsigned idx;    // position of the point in the caller's arrays
+    unsigned cell;   // (r0 (nt - 1) + t0) np + p0, or the outside cell
+};
+
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(const double *__restrict__ pts,
+                                                             const unsigned *__restrict__ key,
+                                                             const unsigned *__restrict__ rank,
+                                                             const unsigned *__restrict__ offset, int64_t n,
+                                                             SortedPoint *__restrict__ sorted) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const unsigned c = key[i];
+        const double *p = pts + i * 3;
+        sorted[offset[c] + rank[i]] = SortedPoint{p[0], p[1], p[2], unsigned(i), c};
+    }
+}
+
+// One lane per point, the points in cell order; the weights repeat bracket_linear_d / bracket_phi_d on the
+// cell the sort found, and the corners load straight from the brick.  A wave's 64 x 11 results pass
+// through LDS so that the stores go out point by point: consecutive lanes write consecutive channels of one
+// point, each point's 88 bytes in one piece, instead of every lane scattering 11 doubles.
+__global__ __launch_bounds__(256) void sorted_gather_kernel(const double *__restrict__ coeffs,
+                                                            const double *__restrict__ rg, int nr,
+                                                            const double *__restrict__ tg, int nt,
+                                                            const double *__restrict__ pg, int np,
+                                                            const SortedPoint *__restrict__ sorted, int64_t n,
+                                                            double *__restrict__ out) {
+    __shared__ double sgrid[kGatherGrid];
+    __shared__ double res_lds[4][64 * kCoeffs];
+    __shared__ unsigned idx_lds[4][64];
+    const bool staged = nr + nt + np <= kGatherGrid;
+    if (staged) {
+        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
+            sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
+        __syncthreads();
+    }
+    const double *r_ = staged ? sgrid : rg;
+    const double *t_ = staged ? sgrid + nr : tg;
+    const double *p_ = staged ? sgrid + nr + nt : pg;
+    const unsigned n_cells = unsigned(nr - 1) * unsigned(nt - 1) * unsigned(np);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double *res = res_lds[wave];
+    unsigned *idx = idx_lds[wave];
+    // the wave's 64 points per round: a wave-uniform loop, so every lane takes part in the LDS exchange
+    for (int64_t base = (int64_t(blockIdx.x) * 4 + wave) * 64; base < n; base += int64_t(gridDim.x) * 256) {
+        const int64_t j = base + lane;
+        unsigned my_idx = ~0u;
+        if (j < n) {
+            const SortedPoint sp = sorted[j];
+            my_idx = sp.idx;
+            if (sp.cell >= n_cells) {
+#pragma unroll
+                for (int k = 0; k < kCoeffs; ++k) res[lane * kCoeffs + k] = __builtin_nan("");
+            } else {
+                const int p0 = int(sp.cell % unsigned(np));
+                const unsigned rt = sp.cell / unsigned(np);
+                const int t0 = int(rt % unsigned(nt - 1)), r0 = int(rt / unsigned(nt - 1));
+                double wr, wt;
+                if (sp.r == r_[nr - 1]) {
+                    wr = 1.0;
+                } else {
+                    double den = r_[r0 + 1] - r_[r0];
+                    den = (1.0e-300 > den) ? 1.0e-300 : den;
+                    wr = (sp.r - r_[r0]) / den;
+                }
+                if (sp.th == t_[nt - 1]) {
+                    wt = 1.0;
+                } else {
+                    double den = t_[t0 + 1] - t_[t0];
+                    den = (1.0e-300 > den) ? 1.0e-300 : den;
+                    wt = (sp.th - t_[t0]) / den;
+                }
+                const double p = wrap_phi_d(sp.ph, p_[0]);
+                const int p1 = (p0 + 1) % np;
+                const double hi = (p1 > p0) ? p_[p1] : p_[0] + kTwoPi;
+                const double pp = (p >= p_[p0]) ? p : p + kTwoPi;
+                double den = hi - p_[p0];
+                den = (1.0e-300 > den) ? 1.0e-300 : den;
+                const double wp = (pp - p_[p0]) / den;
+                const size_t row00 = (size_t(r0) * nt + t0) * np, row01 = row00 + np;
+                const size_t row10 = row00 + size_t(nt) * np, row11 = row10 + np;
+                const double *c000 = coeffs + (row00 + p0) * kCoeffs, *c001 = coeffs + (row00 + p1) * kCoeffs;
+                const double *c010 = coeffs + (row01 + p0) * kCoeffs, *c011 = coeffs + (row01 + p1) * kCoeffs;
+                const double *c100 = coeffs + (row10 + p0) * kCoeffs, *c101 = coeffs + (row10 + p1) * kCoeffs;
+                const double *c110 = coeffs + (row11 + p0) * kCoeffs, *c111 = coeffs + (row11 + p1) * kCoeffs;
+#pragma unroll
+                for (int k = 0; k < kCoeffs; ++k)
+                    res[lane * kCoeffs + k] =
+                        trilerp_d(c000[k], c001[k], c010[k], c011[k], c100[k], c101[k], c110[k], c111[k], wr, wt, wp);
+            }
+        }
+        idx[lane] = my_idx;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < kCoeffs; ++q) {   // value f of the wave's 64 x 11: point f / 11, channel f % 11
+            const int f = q * 64 + lane;
+            const int pt = f / kCoeffs;
+            const unsigned ii = idx[pt];
+            if (ii != ~0u) out[size_t(ii) * kCoeffs + (f - pt * kCoeffs)] = res[f];
+        }
+        __builtin_amdgcn_wave_barrier();   // the next round rewrites this wave's LDS
     }
 }
 
@@ -308,6 +524,51 @@ struct Stage {
 };
 
 static size_t rounded(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// The bucketed order pays for its sort when the points outnumber the cells (each record then serves
+// several points) and there are enough of them to fill the chip.
+constexpr int64_t kBucketMinPoints = int64_t(1) << 16;
+
+// Bucketed large-N sample into device buffers; the workspace (cell counts and offsets, chunk sums, key and
+// rank of every point, the points in cell order) is the context's, grown on demand.
+static int sample_bucketed(bh_ctx *ctx, const double *d_coeffs, const double *d_r, int nr, const double *d_t, int nt,
+                           const double *d_p, int np, const double *d_points, int64_t n, double *d_out) {
+    const size_t n_cells = size_t(nr - 1) * size_t(nt - 1) * size_t(np) + 1;   // + the cell of outside points
+    const size_t nb = (n_cells + kScanBlock - 1) / kScanBlock;
+    const size_t need = rounded(n_cells * 4) + rounded(nb * 4) + 2 * rounded(size_t(n) * 4) +
+                        rounded(size_t(n) * sizeof(SortedPoint));
+    if (need > ctx->sampler_work_cap) {
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_sampler_work) (void)hipFree(ctx->d_sampler_work);
+        ctx->d_sampler_work = nullptr;
+        ctx->sampler_work_cap = 0;
+        if (hipMalloc(&ctx->d_sampler_work, need) != hipSuccess)
+            return set_error(BH_ERR_MEMORY, "sampler workspace allocation of %zu bytes failed", need);
+        ctx->sampler_work_cap = need;
+    }
+    char *w = static_cast<char *>(ctx->d_sampler_work);
+    unsigned *count = reinterpret_cast<unsigned *>(w);
+    unsigned *sums = reinterpret_cast<unsigned *>(w + rounded(n_cells * 4));
+    unsigned *key = reinterpret_cast<unsigned *>(w + rounded(n_cells * 4) + rounded(nb * 4));
+    unsigned *rank = key + rounded(size_t(n) * 4) / 4;
+    SortedPoint *sorted = reinterpret_cast<SortedPoint *>(rank + rounded(size_t(n) * 4) / 4);
+    int st;
+    if ((st = fill_u32(ctx, count, 0u, int(n_cells)))) return st;
+    const int64_t cap = int64_t(ctx->cus > 0 ? ctx->cus : 256) * 16;
+    const unsigned pblocks = unsigned(std::min<int64_t>((n + 255) / 256, cap));
+    hipLaunchKernelGGL(bucket_count_kernel, dim3(pblocks), dim3(256), 0, ctx->stream, d_r, nr, d_t, nt, d_p, np, d_points, n,
+                       key, rank, count);
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3(unsigned(nb)), dim3(kScanBlock), 0, ctx->stream, count, unsigned(n_cells),
+                       sums);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, ctx->stream, sums, unsigned(nb));
+    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(nb)), dim3(kScanBlock), 0, ctx->stream, count, unsigned(n_cells), sums);
+    hipLaunchKernelGGL(bucket_scatter_kernel, dim3(pblocks), dim3(256), 0, ctx->stream, d_points, key, rank, count, n,
+                       sorted);
+    hipLaunchKernelGGL(sorted_gather_kernel, dim3(pblocks), dim3(256), 0, ctx->stream, d_coeffs, d_r, nr, d_t, nt, d_p, np,
+                       sorted, n, d_out);
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
 
 }  // namespace bh
 
@@ -472,11 +733,18 @@ int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs, const 
     if (nr < 2 || nt < 2 || np < 2) return set_error(BH_ERR_VALUE, "grids must contain at least 2 values");
     if (n < 0) return set_error(BH_ERR_VALUE, "n must be non-negative");
     if (n == 0) return BH_OK;
+    const int64_t n_cells = int64_t(nr - 1) * (nt - 1) * np + 1;
+    const bool bucketed = n >= kBucketMinPoints && n >= n_cells && n < (int64_t(1) << 32) &&
+                          n_cells < (int64_t(1) << 31);
     begin_timing(ctx);
-    hipLaunchKernelGGL(sample_gather_kernel, dim3(gather_blocks(ctx, n)), dim3(256), 0, ctx->stream, d_coeffs, d_r, int(nr),
-                       d_theta, int(nt), d_phi, int(np), d_points, n, d_out);
-    BH_HIP(hipGetLastError());
-    end_timing(ctx, 1);
+    if (bucketed) {
+        if ((st = sample_bucketed(ctx, d_coeffs, d_r, nr, d_theta, nt, d_phi, np, d_points, n, d_out))) return st;
+    } else {
+        hipLaunchKernelGGL(sample_gather_kernel, dim3(gather_blocks(ctx, n)), dim3(256), 0, ctx->stream, d_coeffs, d_r,
+                           int(nr), d_theta, int(nt), d_phi, int(np), d_points, n, d_out);
+        BH_HIP(hipGetLastError());
+    }
+    end_timing(ctx, 1);   // one timed call: the sort and the gather together
     return BH_OK;
 }
 
