@@ -151,6 +151,11 @@ SIGNATURES = [
       C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("bh_surrogate_last_fallback", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("bh_surrogate_work_bytes", C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]),
+    ("bh_surrogate_batch_create", C.c_int, [C.c_void_p, C.POINTER(BatchItem), C.c_int32, C.c_void_p,
+                                            C.POINTER(SurrogateDesc), C.POINTER(C.c_int64), C.POINTER(C.c_void_p)]),
+    ("bh_surrogate_batch_launch", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    ("bh_surrogate_batch_fallback", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("bh_surrogate_batch_destroy", C.c_int, [C.c_void_p]),
     ("bh_precompute_coefficient_bricks", C.c_int,
      [_dp, _dp, _dp, C.c_int64, C.POINTER(SynchrotronDesc), C.c_int32, C.c_void_p]),
     ("bh_precompute_coefficient_bricks_device", C.c_int,

@@ -379,6 +379,10 @@ class Surrogate:
         _lib.check(self.r._lib.bh_surrogate_last_fallback(self.r.ctx, C.byref(v)))
         return int(v.value)
 
+    def batch(self, jobs) -> "SurrogateBatch":
+        """Many renders through the surrogate in one captured sequence (bh_surrogate_batch_*)."""
+        return SurrogateBatch(self, jobs)
+
     # ---- checkpoints ---------------------------------------------------------------------
     def save(self, path) -> None:
         """Weights, normalisation, loss history and the optimizer state (Adam's moments and step count)."""
@@ -407,6 +411,74 @@ class Surrogate:
             self._work.free()
         self._work = None
         self.mlp.close()
+
+
+class SurrogateBatch:
+    """Renders (Scene, DeviceBrick) through the surrogate in one sequence: inputs of every render's rays,
+    one FP16 fused inference, the knot transfer of every tile, and all fallback rays in one persistent
+    RK2 launch; captured in a hipGraph on the first ``launch(graph=True)`` and replayed afterwards."""
+
+    def __init__(self, sur: Surrogate, jobs):
+        self.sur = sur
+        self.r = sur.r
+        self.scenes = [sc for sc, _ in jobs]
+        for sc in self.scenes:
+            if abs(float(sc.camera.r) - sur.cfg.cam_r) > 1e-9 * sur.cfg.cam_r:
+                raise ValueError(f"the surrogate was trained for a camera at r = {sur.cfg.cam_r}")
+        n = len(self.scenes)
+        items = (_lib.BatchItem * n)()
+        for i, (sc, db) in enumerate(jobs):
+            items[i].scene = scene_desc(sc)
+            items[i].brick_id = db.brick_id
+        off = (C.c_int64 * (n + 1))()
+        sd = sur.desc()
+        h = C.c_void_p()
+        _lib.check(self.r._lib.bh_surrogate_batch_create(self.r.ctx, items, n, sur.mlp.h, C.byref(sd), off, C.byref(h)))
+        self.handle = h
+        self.offsets = [int(off[i]) for i in range(n + 1)]
+        self.pixels = self.offsets[-1]
+        self._bufs = None
+
+    def buffers(self):
+        if self._bufs is None:
+            self._bufs = (self.r.alloc(self.pixels * 16), self.r.alloc(self.pixels * 4), self.r.alloc(self.pixels))
+        return self._bufs
+
+    def launch(self, graph: bool = True) -> None:
+        d_iquv, d_steps, d_status = self.buffers()
+        _lib.check(self.r._lib.bh_surrogate_batch_launch(self.handle, d_iquv.ptr, d_steps.ptr, d_status.ptr,
+                                                         int(bool(graph))))
+
+    def fallback(self) -> tuple[list, int]:
+        """Fallback rays of the last launch: per render, and in all."""
+        n = len(self.scenes)
+        per = (C.c_int64 * n)()
+        tot = C.c_int64(0)
+        _lib.check(self.r._lib.bh_surrogate_batch_fallback(self.handle, per, C.byref(tot)))
+        return [int(per[i]) for i in range(n)], int(tot.value)
+
+    def results(self) -> list[dict]:
+        """Synchronise and return per-render dicts of iquv (H, W, 4) float32, steps and status."""
+        d_iquv, d_steps, d_status = self.buffers()
+        iquv = d_iquv.to_host(np.empty((self.pixels, 4), dtype=np.float32))
+        steps = d_steps.to_host(np.empty(self.pixels, dtype=np.int32))
+        status = d_status.to_host(np.empty(self.pixels, dtype=np.uint8))
+        out = []
+        for i, sc in enumerate(self.scenes):
+            a, b = self.offsets[i], self.offsets[i + 1]
+            H, W = sc.height, sc.width
+            out.append({"iquv": iquv[a:b].reshape(H, W, 4), "steps": steps[a:b].reshape(H, W),
+                        "status": status[a:b].reshape(H, W)})
+        return out
+
+    def close(self) -> None:
+        if self._bufs is not None:
+            for b in self._bufs:
+                b.free()
+            self._bufs = None
+        if self.handle and self.r.ctx:
+            self.r._lib.bh_surrogate_batch_destroy(self.handle)
+        self.handle = None
 
 
 def _desc(cfg: SurrogateConfig, out_mul: np.ndarray, out_add: np.ndarray, in_mul=None, in_add=None):

@@ -108,6 +108,10 @@ int staging(size_t bytes, void **ptr);
 // MLP inference enqueued on the MLP's context stream, without timing events:
 // d_x [rows][n_in] -> d_y [rows][n_out] or [n_out][rows] (transposed).
 int mlp_forward_enqueue(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int transposed);
+// Fused transposed inference with FP16 outputs y [n_out][ld] (the width-256 networks of the surrogate).
+int mlp_forward_enqueue_f16(bh_mlp *m, const float *d_x, int64_t rows, _Float16 *d_y, long long ld);
+// Whether mlp_forward_enqueue_f16 serves this network.
+bool mlp_half_outputs(const bh_mlp *m);
 int mlp_in_cols(const bh_mlp *m);
 bh_ctx *mlp_context(const bh_mlp *m);
 int mlp_out_cols(const bh_mlp *m);

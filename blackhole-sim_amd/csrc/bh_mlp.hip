@@ -235,6 +235,7 @@ struct FusedArgs {
     const __bf16 *wtb;       // BF16 W^T per layer
     long long w_off[kFMaxLayers], b_off[kFMaxLayers];
     float *y;                // [n_out][ld], rows r0 .. of this launch at column offset 0
+    _Float16 *yh;            // when set, the outputs go here as FP16 instead (same layout)
     long long ld;
     unsigned wt_bytes;       // bytes of the W^T buffer (buffer-resource bound)
 };
@@ -318,6 +319,19 @@ __device__ __forceinline__ void store_output(const FusedArgs &F, const float *bi
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
             const int row = row0 + i * 16 + (lane >> 4) * 4;
+            if (F.yh) {   // FP16 outputs: four rows in one 8-byte store
+                _Float16 *h = F.yh + size_t(c) * F.ld + row;
+                if (row + 3 < F.rows && (F.ld & 3) == 0) {
+                    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+                    *reinterpret_cast<h4 *>(h) = h4{_Float16(acc[i][j][0] + bv), _Float16(acc[i][j][1] + bv),
+                                                    _Float16(acc[i][j][2] + bv), _Float16(acc[i][j][3] + bv)};
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (row + q < F.rows) h[q] = _Float16(acc[i][j][q] + bv);
+                }
+                continue;
+            }
             float *o = F.y + size_t(c) * F.ld + row;
             if (row + 3 < F.rows && (F.ld & 3) == 0) {
                 *reinterpret_cast<float4 *>(o) =
@@ -647,8 +661,9 @@ static bool fused_ok(const bh_mlp *m) {
            m->n_out_pad <= 4 * kFWidth && m->n_params * 2 < (size_t(1) << 31);
 }
 
-// Fused inference of `rows` rows into y [n_out][ld] (transposed).
-static int forward_fused(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, long long ld) {
+// Fused inference of `rows` rows into y [n_out][ld] (transposed), FP32, or FP16 into yh when it is set.
+static int forward_fused(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, long long ld,
+                         _Float16 *d_yh = nullptr) {
     FusedArgs F{};
     F.x = d_x;
     F.rows = int(rows);
@@ -664,6 +679,7 @@ static int forward_fused(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, 
         F.b_off[l] = (long long)m->b_off[l];
     }
     F.y = d_y;
+    F.yh = d_yh;
     F.ld = ld;
     hipLaunchKernelGGL(mlp_fused_kernel, dim3(unsigned((rows + kFRows - 1) / kFRows)), dim3(256), 0, m->ctx->stream, F);
     BH_HIP(hipGetLastError());
@@ -683,6 +699,15 @@ int mlp_forward_enqueue(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, i
     }
     BH_HIP(hipGetLastError());
     return BH_OK;
+}
+
+bool mlp_half_outputs(const bh_mlp *m) { return fused_ok(m); }
+
+int mlp_forward_enqueue_f16(bh_mlp *m, const float *d_x, int64_t rows, _Float16 *d_y, long long ld) {
+    if (!fused_ok(m)) return set_error(BH_ERR_VALUE, "FP16 network outputs need the fused width-256 inference");
+    if (rows <= 0) return BH_OK;
+    if (rows > (int64_t(1) << 31) - 1) return set_error(BH_ERR_VALUE, "too many network rows");
+    return forward_fused(m, d_x, rows, nullptr, ld, d_y);
 }
 
 static int mlp_check(bh_mlp *m) {

@@ -1484,9 +1484,14 @@ struct BatchArgs {
     void *iquv;
     int *steps;
     uint8_t *status;
-    unsigned *counters;          // per-queue tile counters
+    unsigned *counters;          // per-queue tile counters (list mode: counters[0] counts taken chunks)
     unsigned long long *work;    // [0] RK2 steps, [1] brick samples, [2] wave iterations x 64
     int queues, q_tiles, steps_per_sync;
+    // list mode (list non-null): render r traces the pixels list[out_off_r .. + list_count[r]), in chunks of
+    // 64 rays; chunk_prefix (n_renders + 1 entries, on the device) is the exclusive prefix of the chunks
+    const int *list;
+    const unsigned *list_count, *chunk_prefix;
+    int n_renders;
 };
 
 template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
@@ -1509,6 +1514,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     for (;;) {
         if (!__any(active)) {
             bool took = false;
+            if (B.list) {   // the next chunk of 64 listed rays of some render
+                unsigned slot = 0;
+                if (lane == 0) slot = atomicAdd(&B.counters[0], 1u);
+                slot = __builtin_amdgcn_readfirstlane(slot);
+                if (slot < B.chunk_prefix[B.n_renders]) {
+                    int lo = 0, hi = B.n_renders - 1;   // the last render whose chunks start at or before slot
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (B.chunk_prefix[mid] <= slot) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    A = B.descs[lo];
+                    const size_t esz = sizeof(R);
+                    A.iquv = static_cast<char *>(B.iquv) + size_t(A.out_off) * 4 * esz;
+                    A.steps = B.steps ? B.steps + A.out_off : nullptr;
+                    A.status = B.status ? B.status + A.out_off : nullptr;
+                    const unsigned e = (slot - B.chunk_prefix[lo]) * 64u + unsigned(lane);
+                    if (e < B.list_count[lo]) {
+                        const int pix = B.list[A.out_off + e];
+                        ray_init(A, pix % A.W, pix / A.W, s);
+                        active = true;
+                    }
+                    took = true;
+                }
+                exhausted = all_q;
+            }
             while (!took && exhausted != all_q) {
                 for (int t = 0; t < B.queues && !took; ++t) {
                     const int q = (q_own + t) % B.queues;
@@ -2624,9 +2655,9 @@ __global__ __launch_bounds__(256) void knots_kernel(KnotArgs A) {
 
 // Network input of every pixel: ((alpha, beta, a, cos i) - add) * mul, alpha/beta the image-plane
 // offsets (M) of the pixel's camera direction.
-__global__ void surrogate_input_kernel(RenderArgs A, float4 mul, float4 add, float *x) {
-    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= A.W * A.H) return;
+// Network input of pixel pix: ((alpha, beta, a, cos i) - add) * mul, alpha and beta the image-plane offsets
+// (M) of the pixel's camera direction.
+__device__ __forceinline__ void surrogate_input(const RenderArgs &A, int pix, float4 mul, float4 add, float *o) {
     const int px = pix % A.W, py = pix / A.W;
     const double ndc_x = 2.0 * (double(px) + 0.5) / double(A.W) - 1.0;
     const double ndc_y = 1.0 - 2.0 * (double(py) + 0.5) / double(A.H);
@@ -2638,16 +2669,35 @@ __global__ void surrogate_input_kernel(RenderArgs A, float4 mul, float4 add, flo
         n_th = t0;
         n_ph = t1;
     }
-    float *o = x + size_t(pix) * 4;
     o[0] = (float(n_ph * C.r) - add.x) * mul.x;
     o[1] = (float(-n_th * C.r) - add.y) * mul.y;
     o[2] = (float(A.a) - add.z) * mul.z;
     o[3] = (float(::cos(C.th)) - add.w) * mul.w;
 }
 
+__global__ void surrogate_input_kernel(RenderArgs A, float4 mul, float4 add, float *x) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= A.W * A.H) return;
+    surrogate_input(A, pix, mul, add, x + size_t(pix) * 4);
+}
+
+// The inputs of every render of a batch, row g of the batch belonging to the last render with out_off <= g.
+__global__ void surrogate_input_batch_kernel(const RenderArgs *descs, int n, long long total, float4 mul, float4 add,
+                                             float *x) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].out_off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    surrogate_input(descs[lo], int(g - descs[lo].out_off), mul, add, x + size_t(g) * 4);
+}
+
 struct SurArgs {
     RenderArgs R;            // scene, brick view and outputs (FP32 IQUV)
-    const float *y;          // network outputs, [n_out][ld]
+    const void *y;           // network outputs, [n_out][ld], FP32 or FP16 (the kernel's KT)
     long long ld;
     const float *mul, *add;  // denormalisation per output (device)
     const float *curve;      // critical curve (alpha, beta) pairs
@@ -2657,35 +2707,31 @@ struct SurArgs {
     unsigned *n_list;
 };
 
-// One wave per 8x8 pixel tile.  Pixels near the critical curve or flagged by the heads are
-// appended to the fallback list; the others integrate the Stokes transfer over the predicted
-// knots with the RK2 transfer rule (midpoint sample, Euclidean ds, accelerated_renderer.py:113-125).
-template <int STORE, int LAYOUT>
-__global__ __launch_bounds__(256) void surrogate_kernel(SurArgs S) {
-    __shared__ float s_grid[kMaxGrid + 1];
-    __shared__ float s_curve[2 * kCurveMax];
-    __shared__ float s_mul[4 * kMaxKnots + kHeads], s_add[4 * kMaxKnots + kHeads];
-    const RenderArgs &A = S.R;
-    const int ng = A.nr + A.nt + A.np;
-    for (int j = threadIdx.x; j < ng; j += blockDim.x) s_grid[j] = A.grids[j];
-    if (threadIdx.x == 0) s_grid[ng] = __builtin_huge_valf();
-    for (int j = threadIdx.x; j < 2 * S.n_curve; j += blockDim.x) s_curve[j] = S.curve[j];
-    const int n_out = 4 * S.K + kHeads;
-    for (int j = threadIdx.x; j < n_out; j += blockDim.x) {
-        s_mul[j] = S.mul[j];
-        s_add[j] = S.add[j];
-    }
-    __syncthreads();
-    const float *gr = s_grid, *gt = s_grid + A.nr, *gp = gt + A.nt;
+// The per-render settings of the surrogate, shared by every render of a batch.
+struct SurShared {
+    const float *mul, *add;
+    int K;
+    float crit2, flag_max, cap_lo, cap_hi;
+};
+
+// One 8x8 tile of a surrogate render, one lane per pixel.  Pixels near the critical curve or flagged by
+// the heads are appended to the fallback list (their pixel index in the render); the others integrate
+// the Stokes transfer over the predicted knots with the RK2 transfer rule (midpoint sample, Euclidean
+// ds, accelerated_renderer.py:113-125).  y points at the knot column of the render's pixel 0, gr at the
+// render's brick grids in LDS (+inf after phi), curve at the render's critical curve; A's outputs are
+// the render's own.
+template <int STORE, int LAYOUT, typename KT>
+__device__ __forceinline__ void surrogate_tile(const RenderArgs &A, const float *gr, const float *s_mul,
+                                               const float *s_add, const KT *y, long long ld, const float *curve,
+                                               int n_curve, const SurShared &S, int *list, unsigned *n_list,
+                                               int tile, int lane) {
+    const float *gt = gr + A.nr, *gp = gt + A.nt;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(A.coeffs), (short)0, int(A.total_bytes), int(kRsrcWord3));
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int px = (tile % A.tiles_x) * 8 + (lane & 7), py = (tile / A.tiles_x) * 8 + (lane >> 3);
     const bool inside = px < A.W && py < A.H;
     const int pix = py * A.W + px;
-    const float *yc = S.y + pix;
-    const long long ld = S.ld;
+    const KT *yc = y + pix;
     bool fb = false;
     float cap = 0.f;
     if (inside) {
@@ -2700,22 +2746,22 @@ __global__ __launch_bounds__(256) void surrogate_kernel(SurArgs S) {
         const float al = float(-xi / s0);
         const float be = float(::sqrt(fmax(Q / (E * E) + a * a * c0 * c0 - xi * xi * (c0 / s0) * (c0 / s0), 0.0)));
         float d2 = __builtin_huge_valf();
-        for (int j = 0; j < S.n_curve; ++j) {
-            const float dx = al - s_curve[2 * j], dy = be - s_curve[2 * j + 1];
+        for (int j = 0; j < n_curve; ++j) {
+            const float dx = al - curve[2 * j], dy = be - curve[2 * j + 1];
             d2 = fminf(d2, dx * dx + dy * dy);
         }
         const int hc = 4 * S.K;
-        cap = yc[hc * ld] * s_mul[hc] + s_add[hc];
-        const float flag = yc[(hc + 1) * ld] * s_mul[hc + 1] + s_add[hc + 1];
+        cap = float(yc[hc * ld]) * s_mul[hc] + s_add[hc];
+        const float flag = float(yc[(hc + 1) * ld]) * s_mul[hc + 1] + s_add[hc + 1];
         fb = d2 < S.crit2 || flag > S.flag_max || (cap > S.cap_lo && cap < S.cap_hi) || !(cap == cap);
     }
     const unsigned long long m = __ballot(fb);
     if (m) {
         const int leader = __builtin_ctzll(m);
         unsigned base = 0;
-        if (lane == leader) base = atomicAdd(S.n_list, unsigned(__popcll(m)));
+        if (lane == leader) base = atomicAdd(n_list, unsigned(__popcll(m)));
         base = __builtin_amdgcn_readlane(base, leader);
-        if (fb) S.list[base + lane_rank(m)] = pix;
+        if (fb) list[base + lane_rank(m)] = pix;
     }
     if (!inside || fb) return;
 
@@ -2723,9 +2769,9 @@ __global__ __launch_bounds__(256) void surrogate_kernel(SurArgs S) {
     const float r_lo = gr[0], r_hi = gr[A.nr - 1], ph0 = float(A.cam.ph);
     auto knot = [&](int j, float &kr, float &kt, float &kp) {
         const int c = 4 * j;
-        kr = yc[c * ld] * s_mul[c] + s_add[c];
-        kt = clamp_theta(yc[(c + 1) * ld] * s_mul[c + 1] + s_add[c + 1]);
-        kp = yc[(c + 2) * ld] * s_mul[c + 2] + s_add[c + 2] + ph0;
+        kr = float(yc[c * ld]) * s_mul[c] + s_add[c];
+        kt = clamp_theta(float(yc[(c + 1) * ld]) * s_mul[c + 1] + s_add[c + 1]);
+        kp = float(yc[(c + 2) * ld]) * s_mul[c + 2] + s_add[c + 2] + ph0;
     };
     float St[4] = {0.f, 0.f, 0.f, 0.f}, tau = 0.f;
     float r0, t0, p0;
@@ -2765,21 +2811,120 @@ __global__ __launch_bounds__(256) void surrogate_kernel(SurArgs S) {
     if (A.status) A.status[pix] = uint8_t(code | 16);
 }
 
+// One render: one wave per 8x8 tile.
+template <int STORE, int LAYOUT, typename KT>
+__global__ __launch_bounds__(256) void surrogate_kernel(SurArgs S) {
+    __shared__ float s_grid[kMaxGrid + 1];
+    __shared__ float s_curve[2 * kCurveMax];
+    __shared__ float s_mul[4 * kMaxKnots + kHeads], s_add[4 * kMaxKnots + kHeads];
+    const RenderArgs &A = S.R;
+    const int ng = A.nr + A.nt + A.np;
+    for (int j = threadIdx.x; j < ng; j += blockDim.x) s_grid[j] = A.grids[j];
+    if (threadIdx.x == 0) s_grid[ng] = __builtin_huge_valf();
+    for (int j = threadIdx.x; j < 2 * S.n_curve; j += blockDim.x) s_curve[j] = S.curve[j];
+    const int n_out = 4 * S.K + kHeads;
+    for (int j = threadIdx.x; j < n_out; j += blockDim.x) {
+        s_mul[j] = S.mul[j];
+        s_add[j] = S.add[j];
+    }
+    __syncthreads();
+    const SurShared sh{S.mul, S.add, S.K, S.crit2, S.flag_max, S.cap_lo, S.cap_hi};
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    surrogate_tile<STORE, LAYOUT, KT>(A, s_grid, s_mul, s_add, static_cast<const KT *>(S.y), S.ld, s_curve, S.n_curve, sh,
+                                      S.list, S.n_list, tile, threadIdx.x & 63);
+}
+
+// Many renders: wave w takes tile tiles[w] = render << 20 | tile.  The renders share one merged brick
+// buffer and grid table (as the batched RK2 kernel), their network outputs are columns out_off .. of y,
+// their fallback pixels go to list[out_off + counts[render]++], and their critical curves (curve_at[r],
+// curve_n[r] points) are read from global memory.
+struct SurBatchArgs {
+    const RenderArgs *descs;
+    const int *tiles;
+    int n_tiles;
+    const float *grids;
+    int ng;
+    const _Float16 *y;
+    long long ld;
+    const float *curves;
+    const int *curve_at, *curve_n;
+    SurShared sh;
+    void *iquv;
+    int *steps;
+    uint8_t *status;
+    int *list;
+    unsigned *counts;
+};
+
+template <int STORE, int LAYOUT>
+__global__ __launch_bounds__(256) void surrogate_batch_kernel(SurBatchArgs B) {
+    __shared__ float s_grid[kMaxGrid + 8];
+    __shared__ float s_mul[4 * kMaxKnots + kHeads], s_add[4 * kMaxKnots + kHeads];
+    for (int j = threadIdx.x; j < B.ng; j += blockDim.x) s_grid[j] = B.grids[j];
+    const int n_out = 4 * B.sh.K + kHeads;
+    for (int j = threadIdx.x; j < n_out; j += blockDim.x) {
+        s_mul[j] = B.sh.mul[j];
+        s_add[j] = B.sh.add[j];
+    }
+    __syncthreads();
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= B.n_tiles) return;
+    const int e = B.tiles[w];
+    const int rix = e >> 20, tile = e & 0xfffff;
+    RenderArgs A = B.descs[rix];
+    A.iquv = static_cast<float *>(B.iquv) + size_t(A.out_off) * 4;
+    A.steps = B.steps ? B.steps + A.out_off : nullptr;
+    A.status = B.status ? B.status + A.out_off : nullptr;
+    surrogate_tile<STORE, LAYOUT, _Float16>(A, s_grid + A.grid_off, s_mul, s_add, B.y + A.out_off, B.ld,
+                                            B.curves + 2 * B.curve_at[rix], B.curve_n[rix], B.sh, B.list + A.out_off,
+                                            B.counts + rix, tile, threadIdx.x & 63);
+}
+
+// Chunks of 64 fallback rays per render, as an exclusive prefix: prefix[r] .. prefix[r + 1] - 1 are the
+// batched RK2 kernel's slots for render r, prefix[n] their number.
+__global__ void fallback_chunks_kernel(const unsigned *counts, int n, unsigned *prefix) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned acc = 0;
+    for (int r = 0; r < n; ++r) {
+        prefix[r] = acc;
+        acc += (counts[r] + 63u) / 64u;
+    }
+    prefix[n] = acc;
+}
+
 __global__ void fill_u32_once(unsigned *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
 typedef void (*SurFn)(SurArgs);
+typedef void (*SurBatchFn)(SurBatchArgs);
 
-static SurFn pick_surrogate(int storage, int layout) {
+template <typename KT>
+static SurFn pick_surrogate_t(int storage, int layout) {
     if (layout == BH_LAYOUT_CORNER) {
-        if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_CORNER>;
-        if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_CORNER>;
-        return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_CORNER>;
+        if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_CORNER, KT>;
+        if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_CORNER, KT>;
+        return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_CORNER, KT>;
     }
-    if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_SOA>;
-    if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_SOA>;
-    return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_SOA>;
+    if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_SOA, KT>;
+    if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_SOA, KT>;
+    return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_SOA, KT>;
+}
+
+// half_knots: the knots come from the network as FP16; otherwise FP32 rows the caller supplies.
+static SurFn pick_surrogate(int storage, int layout, bool half_knots) {
+    return half_knots ? pick_surrogate_t<_Float16>(storage, layout) : pick_surrogate_t<float>(storage, layout);
+}
+
+static SurBatchFn pick_surrogate_batch(int storage, int layout) {
+    if (layout == BH_LAYOUT_CORNER) {
+        if (storage == BH_STORE_F32) return surrogate_batch_kernel<BH_STORE_F32, BH_LAYOUT_CORNER>;
+        if (storage == BH_STORE_BF16) return surrogate_batch_kernel<BH_STORE_BF16, BH_LAYOUT_CORNER>;
+        return surrogate_batch_kernel<BH_STORE_F16, BH_LAYOUT_CORNER>;
+    }
+    if (storage == BH_STORE_F32) return surrogate_batch_kernel<BH_STORE_F32, BH_LAYOUT_SOA>;
+    if (storage == BH_STORE_BF16) return surrogate_batch_kernel<BH_STORE_BF16, BH_LAYOUT_SOA>;
+    return surrogate_batch_kernel<BH_STORE_F16, BH_LAYOUT_SOA>;
 }
 
 static size_t surrogate_work_bytes(int64_t rows, int n_knots) {
@@ -2793,6 +2938,33 @@ static size_t surrogate_work_bytes(int64_t rows, int n_knots) {
 
 // Shared body of the surrogate entry points: knot rows come from the MLP (mlp non-null; network
 // input and output live in the work buffer) or from the caller (rows, [n_out][ld]).
+// Critical curve of a spin and inclination as (alpha, beta) pairs, thinned to kCurveMax points.
+static std::vector<float> thinned_curve(double spin, double theta) {
+    std::vector<double> al, be;
+    if (std::sin(theta) > 1e-3) critical_curve(spin, theta, al, be);
+    std::vector<float> curve;
+    const size_t stride = al.empty() ? 1 : (al.size() + kCurveMax - 1) / kCurveMax;
+    for (size_t j = 0; j < al.size(); j += stride) {
+        curve.push_back(float(al[j]));
+        curve.push_back(float(be[j]));
+    }
+    return curve;
+}
+
+static int check_surrogate_desc(const bh_surrogate_desc *sd, bh_mlp *mlp, bh_ctx *ctx) {
+    if (!sd || !sd->out_mul || !sd->out_add) return set_error(BH_ERR_VALUE, "NULL surrogate argument");
+    const int K = sd->n_knots;
+    if (K < 2 || K > kMaxKnots) return set_error(BH_ERR_VALUE, "n_knots must lie in [2, %d]", kMaxKnots);
+    if (mlp) {
+        if (mlp_context(mlp) != ctx) return set_error(BH_ERR_VALUE, "the MLP belongs to another context");
+        if (mlp_in_cols(mlp) != 4 || mlp_out_cols(mlp) != 4 * K + kHeads)
+            return set_error(BH_ERR_VALUE, "the MLP must map 4 inputs to 4 * n_knots + %d outputs", kHeads);
+    }
+    if (sd->fallback_precision != BH_PREC_FP32 && sd->fallback_precision != BH_PREC_MIXED)
+        return set_error(BH_ERR_VALUE, "fallback_precision must be BH_PREC_FP32 or BH_PREC_MIXED");
+    return BH_OK;
+}
+
 static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, bh_mlp *mlp, const float *rows_in,
                        int64_t ld_in, const bh_surrogate_desc *sd, float *d_work, size_t work_bytes, void *d_iquv,
                        int32_t *d_steps, uint8_t *d_status) {
@@ -2837,21 +3009,15 @@ static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id
     int *d_list = reinterpret_cast<int *>(take(size_t(rows) * 4));
     float *d_x = mlp ? reinterpret_cast<float *>(take(size_t(rows) * 16)) : nullptr;
     float *d_y = mlp ? reinterpret_cast<float *>(take(size_t(rows) * n_out * 4)) : nullptr;
-    // critical curve of this spin and inclination, thinned to kCurveMax points
-    std::vector<double> al, be;
-    if (std::sin(S.R.cam.th) > 1e-3) critical_curve(scene->spin, S.R.cam.th, al, be);
-    std::vector<float> curve;
-    const size_t stride = al.empty() ? 1 : (al.size() + kCurveMax - 1) / kCurveMax;
-    for (size_t j = 0; j < al.size(); j += stride) {
-        curve.push_back(float(al[j]));
-        curve.push_back(float(be[j]));
-    }
+    const std::vector<float> curve = thinned_curve(scene->spin, S.R.cam.th);
     S.n_curve = int(curve.size() / 2);
     BH_HIP(hipMemcpyAsync(d_mul, sd->out_mul, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream));
     BH_HIP(hipMemcpyAsync(d_add, sd->out_add, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream));
     if (S.n_curve) BH_HIP(hipMemcpyAsync(d_curve, curve.data(), curve.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
-    S.y = mlp ? d_y : rows_in;
+    // the fused inference writes the knots as FP16 (half the bytes to write and to read back)
+    const bool half = mlp && mlp_half_outputs(mlp);
+    S.y = mlp ? static_cast<const void *>(d_y) : static_cast<const void *>(rows_in);
     S.ld = mlp ? rows : ld_in;
     S.mul = d_mul;
     S.add = d_add;
@@ -2872,11 +3038,15 @@ static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id
         const float4 iadd = make_float4(sd->in_add[0], sd->in_add[1], sd->in_add[2], sd->in_add[3]);
         hipLaunchKernelGGL(surrogate_input_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, ctx->stream, S.R,
                            imul, iadd, d_x);
-        if ((st = mlp_forward_enqueue(mlp, d_x, rows, d_y, 1))) return st;
+        if (half) {
+            if ((st = mlp_forward_enqueue_f16(mlp, d_x, rows, reinterpret_cast<_Float16 *>(d_y), rows))) return st;
+        } else if ((st = mlp_forward_enqueue(mlp, d_x, rows, d_y, 1))) {
+            return st;
+        }
     }
     const int tiles_y = (S.R.H + 7) / 8;
     const int n_tiles = S.R.tiles_x * tiles_y;
-    hipLaunchKernelGGL(pick_surrogate(b.storage, b.layout), dim3(unsigned((n_tiles + 3) / 4)), dim3(256), 0,
+    hipLaunchKernelGGL(pick_surrogate(b.storage, b.layout, half), dim3(unsigned((n_tiles + 3) / 4)), dim3(256), 0,
                        ctx->stream, S);
     BH_HIP(hipGetLastError());
     if ((st = render_enqueue(ctx, scene, brick_id, sd->fallback_precision, d_iquv, d_steps, d_status, d_list, d_count,
@@ -2959,6 +3129,240 @@ int bh_surrogate_last_fallback(bh_ctx *ctx, int64_t *n_fallback) {
     BH_HIP(hipMemcpyAsync(&v, ctx->sur_count, 4, hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     *n_fallback = int64_t(v);
+    return BH_OK;
+}
+
+}  // extern "C"
+
+// ---- batched surrogate renders --------------------------------------------------------------
+// Many renders through the learned surrogate in one sequence: the network inputs of every render's rays,
+// one fused FP16 inference over all of them, the knot transfer of every render's tiles in one launch, and
+// every render's fallback rays pooled into one persistent batched RK2 launch (list mode).  The critical
+// curves are computed once at creation; the launch needs no host synchronisation and is captured in a
+// hipGraph on first use, then replayed.
+
+struct bh_surrogate_batch {
+    bh_ctx *ctx = nullptr;
+    bh_batch *fb = nullptr;              // merged bricks, per-render descriptors, the fallback RK2 kernel
+    bh_mlp *mlp = nullptr;
+    int n = 0, n_tiles = 0;
+    long long total = 0;
+    float4 in_mul{}, in_add{};
+    bh::SurShared sh{};
+    bh::SurBatchFn sfn = nullptr;
+    float *d_mul = nullptr, *d_add = nullptr, *d_curves = nullptr, *d_x = nullptr;
+    int *d_curve_at = nullptr, *d_curve_n = nullptr, *d_tiles = nullptr, *d_list = nullptr;
+    _Float16 *d_y = nullptr;
+    unsigned *d_counts = nullptr;        // n per-render fallback counts, then the fallback chunk counter
+    unsigned *d_prefix = nullptr;        // n + 1
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *g_iquv = nullptr, *g_steps = nullptr, *g_status = nullptr;
+    int g_grid = 0, g_block = 0, g_spc = 0;
+};
+
+namespace bh {
+
+// Enqueue the whole batched surrogate render on the context stream.
+static int surrogate_batch_enqueue(bh_surrogate_batch *sb, void *d_iquv, int32_t *d_steps, uint8_t *d_status,
+                                   const BatchArgs &B, int grid, int block) {
+    bh_ctx *ctx = sb->ctx;
+    int st;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(kCounterWords), 0, ctx->stream, ctx->d_counters, 0u, kCounterWords);
+    if ((st = fill_u32(ctx, sb->d_counts, 0u, sb->n + 1))) return st;
+    hipLaunchKernelGGL(surrogate_input_batch_kernel, dim3(unsigned((sb->total + 255) / 256)), dim3(256), 0, ctx->stream,
+                       sb->fb->d_descs, sb->n, sb->total, sb->in_mul, sb->in_add, sb->d_x);
+    if ((st = mlp_forward_enqueue_f16(sb->mlp, sb->d_x, sb->total, sb->d_y, sb->total))) return st;
+    SurBatchArgs S{};
+    S.descs = sb->fb->d_descs;
+    S.tiles = sb->d_tiles;
+    S.n_tiles = sb->n_tiles;
+    S.grids = sb->fb->d_grids;
+    S.ng = sb->fb->ng;
+    S.y = sb->d_y;
+    S.ld = sb->total;
+    S.curves = sb->d_curves;
+    S.curve_at = sb->d_curve_at;
+    S.curve_n = sb->d_curve_n;
+    S.sh = sb->sh;
+    S.iquv = d_iquv;
+    S.steps = d_steps;
+    S.status = d_status;
+    S.list = sb->d_list;
+    S.counts = sb->d_counts;
+    hipLaunchKernelGGL(sb->sfn, dim3(unsigned((sb->n_tiles + 3) / 4)), dim3(256), 0, ctx->stream, S);
+    hipLaunchKernelGGL(fallback_chunks_kernel, dim3(1), dim3(64), 0, ctx->stream, sb->d_counts, sb->n, sb->d_prefix);
+    void *args[] = {const_cast<BatchArgs *>(&B)};
+    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(sb->fb->fn), dim3(grid), dim3(block), args, 0, ctx->stream));
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
+}  // namespace bh
+
+extern "C" {
+
+int bh_surrogate_batch_destroy(bh_surrogate_batch *sb) {
+    if (!sb) return BH_OK;
+    if (sb->ctx) {
+        (void)hipSetDevice(sb->ctx->device);
+        (void)hipStreamSynchronize(sb->ctx->stream);
+    }
+    if (sb->exec) (void)hipGraphExecDestroy(sb->exec);
+    if (sb->graph) (void)hipGraphDestroy(sb->graph);
+    for (void *p : {(void *)sb->d_mul, (void *)sb->d_add, (void *)sb->d_curves, (void *)sb->d_x, (void *)sb->d_curve_at,
+                    (void *)sb->d_curve_n, (void *)sb->d_tiles, (void *)sb->d_list, (void *)sb->d_y,
+                    (void *)sb->d_counts, (void *)sb->d_prefix})
+        if (p) (void)hipFree(p);
+    if (sb->fb) bh_batch_destroy(sb->fb);
+    if (sb->ctx && sb->ctx->sur_count == sb->d_counts) sb->ctx->sur_count = nullptr;
+    delete sb;
+    return BH_OK;
+}
+
+int bh_surrogate_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, bh_mlp *mlp,
+                              const bh_surrogate_desc *sd, int64_t *offsets, bh_surrogate_batch **out) {
+    clear_error();
+    int st = ctx_check(ctx);
+    if (st) return st;
+    if (!out || !items || n < 1) return set_error(BH_ERR_VALUE, "a surrogate batch needs at least one render");
+    *out = nullptr;
+    if (!mlp) return set_error(BH_ERR_VALUE, "NULL MLP");
+    if ((st = check_surrogate_desc(sd, mlp, ctx))) return st;
+    if (!mlp_half_outputs(mlp))
+        return set_error(BH_ERR_VALUE, "the batched surrogate needs the fused width-256 inference (FP16 knots)");
+    for (int i = 0; i < n; ++i) {
+        if ((st = validate_scene(items[i].scene))) return st;
+        if (items[i].scene.transfer != BH_TRANSFER_RK2)
+            return set_error(BH_ERR_VALUE, "the surrogate's transfer over knots takes the RK2 Stokes step only");
+        if (items[i].scene.integrator != BH_INTEGRATOR_RK2)
+            return set_error(BH_ERR_VALUE, "the surrogate is trained on RK2 geodesics: integrator must be RK2");
+    }
+    bh_surrogate_batch *sb = new bh_surrogate_batch();
+    sb->ctx = ctx;
+    sb->mlp = mlp;
+    sb->n = n;
+    auto fail = [&](int code) {
+        bh_surrogate_batch_destroy(sb);
+        return code;
+    };
+    std::vector<int64_t> off(size_t(n) + 1);
+    if ((st = bh_batch_create(ctx, items, n, sd->fallback_precision, off.data(), &sb->fb))) return fail(st);
+    if (offsets) std::copy(off.begin(), off.end(), offsets);
+    sb->total = off[n];
+    if (sb->total >= (int64_t(1) << 31)) return fail(set_error(BH_ERR_VALUE, "a surrogate batch holds at most 2^31 - 1 rays"));
+    sb->sfn = pick_surrogate_batch(sb->fb->storage, sb->fb->layout);
+    const int K = sd->n_knots, n_out = 4 * K + kHeads;
+    sb->in_mul = make_float4(sd->in_mul[0], sd->in_mul[1], sd->in_mul[2], sd->in_mul[3]);
+    sb->in_add = make_float4(sd->in_add[0], sd->in_add[1], sd->in_add[2], sd->in_add[3]);
+    // critical curves and the tile table
+    std::vector<float> curves;
+    std::vector<int> curve_at(n), curve_n(n), tiles;
+    for (int i = 0; i < n; ++i) {
+        CamConst C{};
+        if ((st = camera_constants(items[i].scene, C))) return fail(st);
+        const std::vector<float> c = thinned_curve(items[i].scene.spin, C.th);
+        curve_at[i] = int(curves.size() / 2);
+        curve_n[i] = int(c.size() / 2);
+        curves.insert(curves.end(), c.begin(), c.end());
+        const int tx = (items[i].scene.width + 7) / 8, ty = (items[i].scene.height + 7) / 8;
+        if (tx * ty >= (1 << 20)) return fail(set_error(BH_ERR_VALUE, "render %d has too many 8x8 tiles", i));
+        for (int t = 0; t < tx * ty; ++t) tiles.push_back((i << 20) | t);
+    }
+    if (curves.empty()) curves.assign(2, 0.f);
+    sb->n_tiles = int(tiles.size());
+    const size_t rows = size_t(sb->total);
+    auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, bytes ? bytes : 4) == hipSuccess; };
+    if (!alloc((void **)&sb->d_mul, n_out * 4) || !alloc((void **)&sb->d_add, n_out * 4) ||
+        !alloc((void **)&sb->d_curves, curves.size() * 4) || !alloc((void **)&sb->d_curve_at, size_t(n) * 4) ||
+        !alloc((void **)&sb->d_curve_n, size_t(n) * 4) || !alloc((void **)&sb->d_tiles, tiles.size() * 4) ||
+        !alloc((void **)&sb->d_x, rows * 16) || !alloc((void **)&sb->d_y, rows * size_t(n_out) * 2) ||
+        !alloc((void **)&sb->d_list, rows * 4) || !alloc((void **)&sb->d_counts, (size_t(n) + 1) * 4) ||
+        !alloc((void **)&sb->d_prefix, (size_t(n) + 1) * 4))
+        return fail(set_error(BH_ERR_MEMORY, "surrogate batch allocation failed"));
+    hipError_t e = hipMemcpyAsync(sb->d_mul, sd->out_mul, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb->d_add, sd->out_add, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb->d_curves, curves.data(), curves.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb->d_curve_at, curve_at.data(), size_t(n) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb->d_curve_n, curve_n.data(), size_t(n) * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb->d_tiles, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(set_error(BH_ERR_DEVICE, "surrogate batch upload failed: %s", hipGetErrorString(e)));
+    sb->sh = SurShared{sb->d_mul, sb->d_add, K, sd->crit_distance * sd->crit_distance, sd->flag_max, sd->cap_lo,
+                       sd->cap_hi};
+    *out = sb;
+    return BH_OK;
+}
+
+int bh_surrogate_batch_launch(bh_surrogate_batch *sb, void *d_iquv, int32_t *d_steps, uint8_t *d_status, int32_t graph) {
+    clear_error();
+    if (!sb) return set_error(BH_ERR_STATE, "NULL surrogate batch");
+    int st = ctx_check(sb->ctx);
+    if (st) return st;
+    if (!d_iquv) return set_error(BH_ERR_VALUE, "iquv must not be NULL");
+    bh_ctx *ctx = sb->ctx;
+    BatchArgs B{};
+    int grid = 0, block = 0;
+    if ((st = batch_args(sb->fb, d_iquv, d_steps, d_status, B, grid, block))) return st;
+    B.counters = sb->d_counts + sb->n;   // the fallback chunk counter
+    B.queues = 1;
+    B.list = sb->d_list;
+    B.list_count = sb->d_counts;
+    B.chunk_prefix = sb->d_prefix;
+    B.n_renders = sb->n;
+    ctx->sur_count = nullptr;
+    ctx->last_counters = ctx->d_counters;
+    begin_timing(ctx);
+    if (!graph) {
+        if ((st = surrogate_batch_enqueue(sb, d_iquv, d_steps, d_status, B, grid, block))) return st;
+        end_timing(ctx, 1);
+        return BH_OK;
+    }
+    const bool same = sb->exec && sb->g_iquv == d_iquv && sb->g_steps == (void *)d_steps &&
+                      sb->g_status == (void *)d_status && sb->g_grid == grid && sb->g_block == block &&
+                      sb->g_spc == B.steps_per_sync;
+    if (!same) {
+        if (sb->exec) (void)hipGraphExecDestroy(sb->exec);
+        if (sb->graph) (void)hipGraphDestroy(sb->graph);
+        sb->exec = nullptr;
+        sb->graph = nullptr;
+        BH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int est = surrogate_batch_enqueue(sb, d_iquv, d_steps, d_status, B, grid, block);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+        if (est != BH_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return est;
+        }
+        if (e2 != hipSuccess || !g) return set_error(BH_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e2));
+        sb->graph = g;
+        BH_HIP(hipGraphInstantiate(&sb->exec, sb->graph, nullptr, nullptr, 0));
+        sb->g_iquv = d_iquv;
+        sb->g_steps = d_steps;
+        sb->g_status = d_status;
+        sb->g_grid = grid;
+        sb->g_block = block;
+        sb->g_spc = B.steps_per_sync;
+    }
+    BH_HIP(hipGraphLaunch(sb->exec, ctx->stream));
+    end_timing(ctx, 1);
+    return BH_OK;
+}
+
+int bh_surrogate_batch_fallback(bh_surrogate_batch *sb, int64_t *per_render, int64_t *total) {
+    clear_error();
+    if (!sb) return set_error(BH_ERR_STATE, "NULL surrogate batch");
+    int st = ctx_check(sb->ctx);
+    if (st) return st;
+    std::vector<unsigned> c(size_t(sb->n));
+    BH_HIP(hipMemcpyAsync(c.data(), sb->d_counts, c.size() * 4, hipMemcpyDeviceToHost, sb->ctx->stream));
+    BH_HIP(hipStreamSynchronize(sb->ctx->stream));
+    int64_t t = 0;
+    for (int i = 0; i < sb->n; ++i) {
+        if (per_render) per_render[i] = int64_t(c[size_t(i)]);
+        t += int64_t(c[size_t(i)]);
+    }
+    if (total) *total = t;
     return BH_OK;
 }
 

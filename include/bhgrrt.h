@@ -347,6 +347,28 @@ int bh_surrogate_last_fallback(bh_ctx *ctx, int64_t *n_fallback);
 /* Bytes of d_work bh_surrogate_render_device needs for `rows` pixels. */
 int bh_surrogate_work_bytes(int64_t rows, int32_t n_knots, size_t *bytes);
 
+/* ---- batched surrogate renders -----------------------------------------------
+ * No counterpart in the reference (its renderers have no learned part).  Many renders
+ * (own image size, spin, camera, brick; RK2 transfer and integrator) through the
+ * surrogate in one sequence: the network inputs of every render's rays, one fused
+ * inference over all of them with FP16 knots, the transfer over knots of every
+ * render's tiles in one launch, and every render's fallback rays (the rule of
+ * bh_surrogate_render_device) pooled into one persistent batched RK2 launch at
+ * sd->fallback_precision.  Critical curves are computed at creation, so a launch
+ * needs no host synchronisation; with graph != 0 the sequence is captured in a
+ * hipGraph on first use and replayed afterwards.  Outputs are laid out as in
+ * bh_batch (offsets, n + 1 entries); status bit 4 marks surrogate pixels.  All
+ * bricks must share storage and layout; the network must be the fused width-256
+ * one (4 inputs, 4 * n_knots + 3 outputs). */
+typedef struct bh_surrogate_batch bh_surrogate_batch;
+int bh_surrogate_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, bh_mlp *mlp,
+                              const bh_surrogate_desc *sd, int64_t *offsets, bh_surrogate_batch **out);
+int bh_surrogate_batch_launch(bh_surrogate_batch *batch, void *d_iquv, int32_t *d_steps, uint8_t *d_status,
+                              int32_t graph);
+/* Fallback rays of the last launch, per render (n values, may be NULL) and in all (waits for the launch). */
+int bh_surrogate_batch_fallback(bh_surrogate_batch *batch, int64_t *per_render, int64_t *total);
+int bh_surrogate_batch_destroy(bh_surrogate_batch *batch);
+
 /* ---- coefficient-brick precompute (precompute_coefficient_bricks,
  * coefficient_bricks.py:118-152, with HybridSynchrotronCoefficients,
  * synchrotron.py:231-325) -------------------------------------------------------

@@ -297,3 +297,54 @@ def test_checkpoint_resumes_training_with_the_optimizer_state(renderer, tmp_path
         data[k].free()
     for s in (a, b, c):
         s.close()
+
+
+def test_surrogate_batch_equals_single_surrogate_renders(renderer):
+    """The batched surrogate (inputs of all renders, one FP16 inference, the knot transfer of all tiles, all
+    fallback rays in one persistent RK2 launch, captured in a hipGraph and replayed) gives each render what the
+    single surrogate render gives: the same fallback pixels, the same surrogate pixels bit for bit (same FP16
+    knots, same transfer code), and the same RK2 results on the fallback pixels (statuses and steps; values to
+    the lone-ray split's rounding)."""
+    print("surrogate batch vs single renders", flush=True)
+    bricks = scenes.batch_bricks()[:2]
+    cfg = bsur.SurrogateConfig(spin=(0.0, 0.998), inclination_deg=(5.0, 85.0))
+    sur = bsur.Surrogate(renderer, bricks[0], cfg, seed=2)
+    data = sur.generate(1 << 14, 3)
+    sur.train(data, 60, batch=4096, lr=1e-3, log_every=30)
+    dbs = [renderer.upload(b, "bf16", "corner") for b in bricks]
+    jobs = [(sc.with_size(w, w), dbs[k % 2]) for k, (sc, _), w in zip(range(4), scenes.batch_scenes(4, seed=9),
+                                                                       (40, 96, 17, 64))]
+    saved = renderer.get_tuning(cfg.fallback_precision)
+    renderer.set_tuning(cfg.fallback_precision, coop_steps=0)   # the batched launch does not split lone rays
+    sb = sur.batch(jobs)
+    try:
+        sb.launch(graph=False)
+        direct = sb.results()
+        per, total = sb.fallback()
+        sb.launch(graph=True)
+        sb.launch(graph=True)
+        replay = sb.results()
+        for a, b in zip(direct, replay):
+            np.testing.assert_array_equal(a["iquv"], b["iquv"])
+            np.testing.assert_array_equal(a["status"], b["status"])
+        assert total == sum(per) and 0 < total < sb.pixels
+        for (sc, db), got, nfb in zip(jobs, direct, per):
+            ref = sur.render(sc, db)
+            assert ref["fallback"] == nfb
+            sur_px = (ref["status"] & 16) != 0
+            np.testing.assert_array_equal((got["status"] & 16) != 0, sur_px)
+            np.testing.assert_array_equal(got["iquv"][sur_px], ref["iquv"][sur_px])
+            np.testing.assert_array_equal(got["steps"][sur_px], ref["steps"][sur_px])
+            fb = ~sur_px
+            assert np.mean(got["status"][fb] == ref["status"][fb]) >= 0.99
+            assert np.mean(got["steps"][fb] == ref["steps"][fb]) >= 0.99
+            I, I0 = got["iquv"][..., 0][fb].astype(np.float64), ref["iquv"][..., 0][fb].astype(np.float64)
+            assert np.abs(I - I0).sum() <= 1e-5 * max(np.abs(I0).sum(), 1e-30)
+    finally:
+        sb.close()
+        renderer.set_tuning(cfg.fallback_precision, **saved)
+        for d in dbs:
+            renderer.release(d)
+        for k in ("x", "t", "w"):
+            data[k].free()
+        sur.close()
