@@ -275,9 +275,11 @@ def precompute_path(rr, dims, reps: int, cpu_cells: int) -> dict:
             "max_rel_err_vs_oracle": err, "cpu_oracle": cpu}
 
 
-def surrogate_path(rr, scene, brick, ref, I64, args) -> dict:
+def surrogate_path(rr, scene, brick, ref, I64, args, frame_scenes, frames64) -> tuple[dict, dict]:
     """Train the MLP surrogate on seeded RK2 geodesics, then time surrogate renders of the scene
-    (bf16 corner brick, mixed-precision RK2 fallback) and measure their error against FP64."""
+    (bf16 corner brick, mixed-precision RK2 fallback) and measure their error against FP64: one render
+    at a time, and the batched frames in one captured sequence (inputs, one FP16 inference, the knot
+    transfer of every frame, all frames' fallback rays in one persistent RK2 launch)."""
     from bhgrrt import surrogate as bsur
     from bhgrrt.render import PREC_MIXED
 
@@ -316,9 +318,40 @@ def surrogate_path(rr, scene, brick, ref, I64, args) -> dict:
     log(f"[bench] surrogate_mixed_bf16: {out}")
     for b in bufs:
         b.free()
+    # batched: the frames of the batched paths
+    sb = sur.batch([(fs, dbx) for fs in frame_scenes])
+    sb.launch(graph=True)
+    rr.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        sb.launch(graph=True)
+    rr.synchronize()
+    dtb = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    kmb = rr.last_kernel_ms()
+    fb_steps, fb_samples = rr.last_work()
+    outs = sb.results()
+    per, tot = sb.fallback()
+    num = den = 0.0
+    pix = []
+    for o, r in zip(outs, frames64):
+        okb = ((o["status"] | r["status"]) & 8) == 0
+        Ib, I0 = o["iquv"][..., 0].astype(np.float64), r["iquv"][..., 0]
+        num += float(np.abs(Ib - I0)[okb].sum())
+        den += float(np.abs(I0[okb]).sum())
+        big = okb & (np.abs(I0) > 1e-6 * np.abs(I0[okb]).max())
+        pix.append(np.abs(Ib - I0)[big] / np.abs(I0[big]))
+    pix = np.concatenate(pix)
+    nb = sb.pixels
+    batched = {"mrays_s": nb / dtb / 1e6, "ms_per_launch": dtb * 1e3, "kernel_ms": kmb, "frames": len(frame_scenes),
+               "rel_l1_I": num / den, "pixel_rel_I": {"median": float(np.median(pix)),
+                                                       "p99": float(np.quantile(pix, 0.99))},
+               "fallback_frac": tot / nb, "knots": "fp16", "graph": True,
+               "fallback_rk2_steps": fb_steps, "fallback_brick_samples": fb_samples}
+    log(f"[bench] surrogate_batched: {batched}")
+    sb.close()
     rr.release(dbx)
     sur.close()
-    return out
+    return out, batched
 
 
 def main(argv=None) -> int:
@@ -331,7 +364,7 @@ def main(argv=None) -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--frames", type=int, default=8, help="1024^2 frames per batched launch")
     ap.add_argument("--surrogate-rays", type=int, default=1 << 20, help="RK2 geodesics the surrogate trains on")
-    ap.add_argument("--surrogate-epochs", type=float, default=16.0)
+    ap.add_argument("--surrogate-epochs", type=float, default=1e6, help="epoch cap (the time limit binds first)")
     ap.add_argument("--surrogate-seconds", type=float, default=90.0, help="training time limit")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sampler-points", type=int, default=1 << 24)
@@ -456,11 +489,13 @@ def main(argv=None) -> int:
         rr.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         util = rr.last_lane_utilisation()
+        st_b, sm_b = rr.last_work()
         outs = bt.results()
         bt.close()
         rr.release(dbx)
         paths[name] = {"mrays_s": frames * n / dt / 1e6, "ms_per_launch": dt * 1e3, "kernel_ms": float(np.mean(kms)),
-                       "frames": frames, "lane_utilisation": util, "tuning": tuned["best"]}
+                       "frames": frames, "lane_utilisation": util, "tuning": tuned["best"], "rk2_steps": st_b,
+                       "brick_samples": sm_b}
         if name == "batched_fp32":
             batch32 = outs
         log(f"[bench] {name}: {paths[name]}")
@@ -504,12 +539,12 @@ def main(argv=None) -> int:
     bricks64 = [rr.upload(b, "f32", "soa") for b in scenes.batch_bricks()]
     bt64 = Batch(rr, [(sc, bricks64[k]) for sc, k in scenes.batch_scenes(64)], PREC_FP64)
     bt64.launch(graph=False)
-    outs64 = bt64.results()
+    outs64_b = bt64.results()
     bt64.close()
     for b in bricks64:
         rr.release(b)
     l1s, pix, quv, flag_diff, num, den = [], [], 0.0, 0, 0.0, 0.0
-    for m, r in zip(outs, outs64):
+    for m, r in zip(outs, outs64_b):
         ok = ((m["status"] | r["status"]) & 8) == 0
         I, I0 = m["iquv"][..., 0].astype(np.float64), r["iquv"][..., 0]
         l1s.append(rel_l1(I[ok], I0[ok]))
@@ -564,13 +599,18 @@ def main(argv=None) -> int:
                                 "round_trip_equal": bool(np.array_equal(back.coeffs, brick.coeffs))}
     log(f"[bench] brick files: {paths['brick_files']}")
 
-    # ---- learned surrogate + mixed precision --------------------------------------------
+    # ---- learned surrogate + mixed precision (+ batched) -----------------------------------
     if not args.no_surrogate:
-        paths["surrogate_mixed_bf16"] = surrogate_path(rr, scene, brick, ref, I64, args)
+        paths["surrogate_mixed_bf16"], paths["surrogate_batched"] = surrogate_path(rr, scene, brick, ref, I64, args,
+                                                                                  frame_scenes, outs64)
+        sbp = paths["surrogate_batched"]
+        sbp["ratio_vs_plain_fp32"] = sbp["mrays_s"] / value
+        # what the network takes off the RK2 integrator: the frames' RK2 steps with and without it
+        sbp["rk2_steps_replaced_frac"] = 1.0 - sbp["fallback_rk2_steps"] / max(paths["batched_mixed_bf16"]["rk2_steps"], 1)
 
     # headline of the layered method: surrogate + mixed (+ batched) vs the plain FP32 render
     target_err = 1e-3
-    sur = paths.get("surrogate_mixed_bf16")
+    sur = paths.get("surrogate_batched")
     compliant = {k: v for k, v in paths.items() if v.get("rel_l1_I") is not None and v["rel_l1_I"] <= target_err
                  and "mrays_s" in v}
     best_name = max(compliant, key=lambda k: compliant[k]["mrays_s"]) if compliant else None

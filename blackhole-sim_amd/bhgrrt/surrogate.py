@@ -293,8 +293,9 @@ class Surrogate:
 
     # ---- training ----------------------------------------------------------------
     def train(self, data: dict, steps: int, batch: int = 8192, lr: float = 1e-3, log_every: int = 100,
-              time_limit_s: float | None = None) -> dict:
-        """Adam on consecutive batches of the (iid, seeded) training rays."""
+              time_limit_s: float | None = None, first_batch: int = 0) -> dict:
+        """Adam on consecutive batches of the (iid, seeded) training rays, from batch ``first_batch`` of the
+        epoch (so that successive calls can continue through it)."""
         c = self.cfg
         n = int(data["n"])
         nb = max(n // int(batch), 1)
@@ -304,7 +305,7 @@ class Surrogate:
         t0 = time.perf_counter()
         done = 0
         for s in range(int(steps)):
-            b = s % nb
+            b = (int(first_batch) + s) % nb
             want = (s + 1) % int(log_every) == 0 or s == int(steps) - 1
             loss = self.mlp.train_step(_off(data["x"], b * batch * 16), _off(data["t"], b * batch * c.n_out * 4), d_w,
                                        batch, lr, want_loss=want, d_row_w=_off(data["w"], b * batch * 4),

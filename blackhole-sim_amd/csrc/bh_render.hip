@@ -61,7 +61,6 @@ struct RenderArgs {
     double r_island, th_island, pth_island;   // FP64 islands of the mixed-precision path
     float step_f, r_plus_f, r_cap_f, r_esc_f, L_f, tau_max_f;   // FP32 copies for the hot loop
     int W, H, max_steps, adaptive;
-    int lerp32;                 // FP64 render: sample as the reference renderer does on a float32 brick
     const void *coeffs;
     int nr, nt, np;
     unsigned plane_bytes, total_bytes;
@@ -81,6 +80,7 @@ struct RenderArgs {
     int tiles_x, n_ids, queues, q_len;
     int steps_per_sync, refill_min;
     int coop_steps;             // lone steps before a lone ray asks an idle wave for help (0: never)
+    int lerp32;                 // FP64 render: sample as the reference renderer does on a float32 brick
 };
 
 template <typename R>
@@ -1487,14 +1487,14 @@ struct BatchArgs {
     unsigned *counters;          // per-queue tile counters (list mode: counters[0] counts taken chunks)
     unsigned long long *work;    // [0] RK2 steps, [1] brick samples, [2] wave iterations x 64
     int queues, q_tiles, steps_per_sync;
-    // list mode (list non-null): render r traces the pixels list[out_off_r .. + list_count[r]), in chunks of
-    // 64 rays; chunk_prefix (n_renders + 1 entries, on the device) is the exclusive prefix of the chunks
+    // list mode (the kernels compiled with LIST): render r traces the pixels list[out_off_r .. + list_count[r]),
+    // in chunks of 64 rays; chunk_prefix (n_renders + 1 entries, on the device) is the exclusive prefix of the chunks
     const int *list;
     const unsigned *list_count, *chunk_prefix;
     int n_renders;
 };
 
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4, bool LIST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void batch_kernel(
     BatchArgs B) {
     __shared__ R s_grid[kMaxGrid + 8];
@@ -1514,7 +1514,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     for (;;) {
         if (!__any(active)) {
             bool took = false;
-            if (B.list) {   // the next chunk of 64 listed rays of some render
+            if constexpr (LIST) {   // the next chunk of 64 listed rays of some render
                 unsigned slot = 0;
                 if (lane == 0) slot = atomicAdd(&B.counters[0], 1u);
                 slot = __builtin_amdgcn_readfirstlane(slot);
@@ -2899,6 +2899,21 @@ __global__ void fill_u32_once(unsigned *p) {
 typedef void (*SurFn)(SurArgs);
 typedef void (*SurBatchFn)(SurBatchArgs);
 
+// The batched RK2 kernel in list mode, for the fallback rays of batched surrogate renders (FP32 or mixed
+// precision, RK2 transfer and integrator).
+static BatchFn pick_list_kernel(int storage, int layout, bool mixed) {
+#define BH_LIST(S, L) (mixed ? batch_kernel<float, S, L, true, false, false, true> : batch_kernel<float, S, L, false, false, false, true>)
+    if (layout == BH_LAYOUT_CORNER) {
+        if (storage == BH_STORE_F32) return BH_LIST(BH_STORE_F32, BH_LAYOUT_CORNER);
+        if (storage == BH_STORE_BF16) return BH_LIST(BH_STORE_BF16, BH_LAYOUT_CORNER);
+        return BH_LIST(BH_STORE_F16, BH_LAYOUT_CORNER);
+    }
+    if (storage == BH_STORE_F32) return BH_LIST(BH_STORE_F32, BH_LAYOUT_SOA);
+    if (storage == BH_STORE_BF16) return BH_LIST(BH_STORE_BF16, BH_LAYOUT_SOA);
+    return BH_LIST(BH_STORE_F16, BH_LAYOUT_SOA);
+#undef BH_LIST
+}
+
 template <typename KT>
 static SurFn pick_surrogate_t(int storage, int layout) {
     if (layout == BH_LAYOUT_CORNER) {
@@ -3150,6 +3165,7 @@ struct bh_surrogate_batch {
     float4 in_mul{}, in_add{};
     bh::SurShared sh{};
     bh::SurBatchFn sfn = nullptr;
+    bh::BatchFn list_fn = nullptr;       // the fallback kernel in list mode
     float *d_mul = nullptr, *d_add = nullptr, *d_curves = nullptr, *d_x = nullptr;
     int *d_curve_at = nullptr, *d_curve_n = nullptr, *d_tiles = nullptr, *d_list = nullptr;
     _Float16 *d_y = nullptr;
@@ -3193,7 +3209,7 @@ static int surrogate_batch_enqueue(bh_surrogate_batch *sb, void *d_iquv, int32_t
     hipLaunchKernelGGL(sb->sfn, dim3(unsigned((sb->n_tiles + 3) / 4)), dim3(256), 0, ctx->stream, S);
     hipLaunchKernelGGL(fallback_chunks_kernel, dim3(1), dim3(64), 0, ctx->stream, sb->d_counts, sb->n, sb->d_prefix);
     void *args[] = {const_cast<BatchArgs *>(&B)};
-    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(sb->fb->fn), dim3(grid), dim3(block), args, 0, ctx->stream));
+    BH_HIP(hipLaunchKernel(reinterpret_cast<const void *>(sb->list_fn), dim3(grid), dim3(block), args, 0, ctx->stream));
     BH_HIP(hipGetLastError());
     return BH_OK;
 }
@@ -3252,6 +3268,7 @@ int bh_surrogate_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n
     sb->total = off[n];
     if (sb->total >= (int64_t(1) << 31)) return fail(set_error(BH_ERR_VALUE, "a surrogate batch holds at most 2^31 - 1 rays"));
     sb->sfn = pick_surrogate_batch(sb->fb->storage, sb->fb->layout);
+    sb->list_fn = pick_list_kernel(sb->fb->storage, sb->fb->layout, sd->fallback_precision == BH_PREC_MIXED);
     const int K = sd->n_knots, n_out = 4 * K + kHeads;
     sb->in_mul = make_float4(sd->in_mul[0], sd->in_mul[1], sd->in_mul[2], sd->in_mul[3]);
     sb->in_add = make_float4(sd->in_add[0], sd->in_add[1], sd->in_add[2], sd->in_add[3]);

@@ -201,8 +201,11 @@ def test_knot_render_with_exact_knots_matches_rk2(renderer):
 
 
 def test_surrogate_render_gains_over_the_untrained_network(renderer):
-    """A short training run on seeded RK2 geodesics makes the surrogate image far closer to the RK2 image
-    than the same pipeline with the untrained network (what the learned part contributes)."""
+    """What the learned part gains over the method without it: after a short training run on seeded RK2
+    geodesics the surrogate takes most of the image off the RK2 integrator (the fallback render integrates
+    at most half of the plain render's RK2 steps), and its image is far closer to the RK2 image than the
+    same pipeline with the untrained network.  Its image error and its wall time are what that costs (the
+    MLP and the knot transfer are not cheaper than the RK2 steps they replace; bench.py reports both)."""
     print("surrogate short training + render", flush=True)
     sc, br = scenes.scale_plain_scene(64, 64), scenes.scale_brick()
     cfg = bsur.SurrogateConfig(spin=(0.9, 0.98), inclination_deg=(10.0, 25.0))
@@ -210,9 +213,11 @@ def test_surrogate_render_gains_over_the_untrained_network(renderer):
     data = sur.generate(1 << 15, 99)
     db = renderer.upload(br, "f32", "corner")
     ref = renderer.render(sc, db, PREC_FP32)
+    plain_steps, _ = renderer.last_work()
     out0 = sur.render(sc, db)
     info = sur.train(data, 400, batch=8192, lr=1e-3, log_every=50)
     out1 = sur.render(sc, db)
+    fallback_steps, _ = renderer.last_work()
     losses = [l for _, l in sur.history]
     assert losses[-1] < 0.5 * losses[0], losses
     I0 = ref["iquv"][..., 0].astype(np.float64).ravel()
@@ -222,8 +227,10 @@ def test_surrogate_render_gains_over_the_untrained_network(renderer):
         return np.abs(o["iquv"][..., 0].astype(np.float64).ravel()[ok] - I0[ok]).sum() / np.abs(I0[ok]).sum()
 
     e0, e1 = err(out0), err(out1)
-    print(f"rel L1 on I: untrained {e0:.3e}, trained ({info['steps']} steps) {e1:.3e}, fallback {out1['fallback']}")
+    print(f"rel L1 on I: untrained {e0:.3e}, trained ({info['steps']} steps) {e1:.3e}, fallback {out1['fallback']}; "
+          f"RK2 steps: plain {plain_steps}, surrogate fallback {fallback_steps}")
     assert e1 < 0.5 * e0
+    assert 0 < fallback_steps <= 0.5 * plain_steps
     assert out1["fallback"] < sc.width * sc.height
     renderer.release(db)
     for k in ("x", "t", "w"):
