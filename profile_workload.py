@@ -8,7 +8,8 @@
 Render paths are storage:layout:precision[:exact] (``exact`` selects the exact Stokes step); each is
 rendered twice (the second launch is the measured one).  ``--surrogate`` adds the learned surrogate: FP64 geodesic knots for
 ``--surrogate-rays`` seeded rays, ``--train-steps`` Adam steps on the BF16 MFMA kernels and two
-1024^2 surrogate renders (bf16 corner brick, mixed-precision fallback).  ``--sampler N`` adds two
+1024^2 surrogate renders (bf16 corner brick, mixed-precision fallback); ``--surrogate-frames N`` adds two
+launches of the batched surrogate over N frames (camera phi = 2 pi k / N).  ``--sampler N`` adds two
 launches of the FP64 trilinear sampler on N uniform points; ``--mlp-rows N`` times the surrogate
 network's inference on N rows (fused kernel, then per-layer GEMMs); ``--precompute NRxNTxNP`` runs the
 coefficient-brick precompute on the analytic torus.  A 256 MiB device-to-device copy comes
@@ -104,7 +105,7 @@ def run_mlp(r: Renderer, rows: int) -> None:
     mlp.close()
 
 
-def run_surrogate(r: Renderer, sc, brick, rays: int, steps: int) -> None:
+def run_surrogate(r: Renderer, sc, brick, rays: int, steps: int, frames: int) -> None:
     from bhgrrt import surrogate as bsur
 
     cfg = bsur.SurrogateConfig(fallback_precision=PREC["mixed"])
@@ -124,6 +125,16 @@ def run_surrogate(r: Renderer, sc, brick, rays: int, steps: int) -> None:
     print(f"surrogate render: {r.last_kernel_ms():.3f} ms, fallback {sur.last_fallback()} rays", flush=True)
     for b in bufs:
         b.free()
+    if frames:
+        fs = [dataclasses.replace(sc, camera=dataclasses.replace(sc.camera, phi=2.0 * math.pi * k / frames))
+              for k in range(frames)]
+        sb = sur.batch([(f, db) for f in fs])
+        for _ in range(2):
+            sb.launch(graph=True)
+            r.synchronize()
+        print(f"surrogate batch of {frames} frames: {r.last_kernel_ms():.3f} ms, fallback {sb.fallback()[1]} rays",
+              flush=True)
+        sb.close()
     r.release(db)
     sur.close()
 
@@ -135,6 +146,8 @@ def main(argv=None) -> int:
     ap.add_argument("--surrogate", action="store_true")
     ap.add_argument("--surrogate-rays", type=int, default=1 << 18)
     ap.add_argument("--train-steps", type=int, default=64)
+    ap.add_argument("--surrogate-frames", type=int, default=0, help="with --surrogate: also launch the batched "
+                    "surrogate sequence (one captured graph) twice over this many frames of the scene")
     ap.add_argument("--sampler", type=int, default=0, help="points of the FP64 sampler launch (0: none)")
     ap.add_argument("--mlp-rows", type=int, default=0, help="time transposed inference of the 4->256x4->515 "
                     "network on this many rows, fused kernel and per-layer GEMMs (0: skip)")
@@ -180,7 +193,7 @@ def main(argv=None) -> int:
     if args.sampler:
         run_sampler(r, br, args.sampler)
     if args.surrogate:
-        run_surrogate(r, sc, br, args.surrogate_rays, args.train_steps)
+        run_surrogate(r, sc, br, args.surrogate_rays, args.train_steps, args.surrogate_frames)
     a, b = r.alloc(COPY_BYTES), r.alloc(COPY_BYTES)
     _lib.check(_lib.lib().bh_copy_device(r.ctx, b.ptr, a.ptr, COPY_BYTES))
     print(f"calibration copy: {COPY_BYTES} bytes", flush=True)
