@@ -2767,19 +2767,32 @@ __device__ __forceinline__ void surrogate_tile(const RenderArgs &A, const float 
 
     const HotParams<float> P = hot_params<float>(A);
     const float r_lo = gr[0], r_hi = gr[A.nr - 1], ph0 = float(A.cam.ph);
-    auto knot = [&](int j, float &kr, float &kt, float &kp) {
+    // knot j's raw outputs; every row holds K knots, so any j < K is in bounds
+    auto load_knot = [&](int j, KT &hr, KT &ht, KT &hp) {
         const int c = 4 * j;
-        kr = float(yc[c * ld]) * s_mul[c] + s_add[c];
-        kt = clamp_theta(float(yc[(c + 1) * ld]) * s_mul[c + 1] + s_add[c + 1]);
-        kp = float(yc[(c + 2) * ld]) * s_mul[c + 2] + s_add[c + 2] + ph0;
+        hr = yc[c * ld];
+        ht = yc[(c + 1) * ld];
+        hp = yc[(c + 2) * ld];
+    };
+    auto decode_knot = [&](int j, KT hr, KT ht, KT hp, float &kr, float &kt, float &kp) {
+        const int c = 4 * j;
+        kr = float(hr) * s_mul[c] + s_add[c];
+        kt = clamp_theta(float(ht) * s_mul[c + 1] + s_add[c + 1]);
+        kp = float(hp) * s_mul[c + 2] + s_add[c + 2] + ph0;
     };
     float St[4] = {0.f, 0.f, 0.f, 0.f}, tau = 0.f;
     float r0, t0, p0;
-    knot(0, r0, t0, p0);
+    KT hr, ht, hp;
+    load_knot(0, hr, ht, hp);
+    decode_knot(0, hr, ht, hp, r0, t0, p0);
+    if (S.K > 1) load_knot(1, hr, ht, hp);
     int code = -1, seg = 0;
     for (int j = 1; j < S.K; ++j) {
         float r1, t1, p1;
-        knot(j, r1, t1, p1);
+        decode_knot(j, hr, ht, hp, r1, t1, p1);
+        // the next knot's loads leave before this segment's brick loads, so a segment waits for one
+        // memory round trip instead of two
+        if (j + 1 < S.K) load_knot(j + 1, hr, ht, hp);
         const float rm = 0.5f * (r0 + r1);
         // outside the brick's radial range and moving away from it: no later segment samples it
         if ((rm > r_hi && r1 > r0) || (rm < r_lo && r1 < r0)) {
@@ -2867,9 +2880,11 @@ __global__ __launch_bounds__(256) void surrogate_batch_kernel(SurBatchArgs B) {
         s_add[j] = B.sh.add[j];
     }
     __syncthreads();
-    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the wave's tile and render are wave-uniform: read them into SGPRs so the render descriptor is a
+    // scalar load and the brick's buffer resource needs no per-load waterfall
+    const int w = __builtin_amdgcn_readfirstlane(int(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
     if (w >= B.n_tiles) return;
-    const int e = B.tiles[w];
+    const int e = __builtin_amdgcn_readfirstlane(B.tiles[w]);
     const int rix = e >> 20, tile = e & 0xfffff;
     RenderArgs A = B.descs[rix];
     A.iquv = static_cast<float *>(B.iquv) + size_t(A.out_off) * 4;
