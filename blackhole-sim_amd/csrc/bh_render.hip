@@ -64,7 +64,7 @@ struct RenderArgs {
     const void *coeffs;
     int nr, nt, np;
     unsigned plane_bytes, total_bytes;
-    unsigned brick_base;        // byte offset of this brick inside the coefficient buffer (batches merge bricks)
+    unsigned long long brick_base;   // byte offset of this brick inside the coefficient buffer (batches merge bricks)
     int grid_off;               // offset of this brick's r grid in the LDS grid table
     long long out_off;          // first pixel of this render in the batch output arrays
     const float *grids;
@@ -290,6 +290,30 @@ __device__ __forceinline__ float load_coeff(__amdgpu_buffer_rsrc_t rs, unsigned 
     }
 }
 
+// Element idx of a corner record (STORE: f32, bf16 or f16) as FP32.  Corner records are read through
+// 64-bit addresses, so a corner brick may exceed the 4 GiB a buffer resource covers.
+template <int STORE>
+__device__ __forceinline__ float load_coeff_at(const char *rec, unsigned idx) {
+    if constexpr (STORE == BH_STORE_F32) {
+        return reinterpret_cast<const float *>(rec)[idx];
+    } else if constexpr (STORE == BH_STORE_BF16) {
+        const unsigned h = reinterpret_cast<const unsigned short *>(rec)[idx];
+        return __builtin_bit_cast(float, h << 16);
+    } else {
+        return float(reinterpret_cast<const _Float16 *>(rec)[idx]);
+    }
+}
+
+// Address of cell record `cell` (8 corners x 11 channels, corner-major) of the render's brick.
+template <int STORE>
+__device__ __forceinline__ const char *corner_ptr(const RenderArgs &A, unsigned cell) {
+    constexpr size_t rec = 8u * kCoeffs * (STORE == BH_STORE_F32 ? 4u : 2u);
+    // the index stays 32-bit arithmetic (full-rate multiply-adds); only the byte address is 64-bit, one
+    // multiply-add (folded, the whole index chain would become 64-bit multiply-adds)
+    asm("" : "+v"(cell));
+    return static_cast<const char *>(A.coeffs) + A.brick_base + size_t(cell) * rec;
+}
+
 // Index guess from the hint, clamped to [0, nmax]; NaN -> 0.  Only the r grid may be geometric
 // (theta and phi hints are always linear), so only the r guess evaluates a logarithm.
 template <bool MAYBE_LOG>
@@ -427,10 +451,10 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         (void)p1;
         const unsigned k = min(unsigned(threadIdx.x & 63), unsigned(kCoeffs - 1));   // lanes >= 11 repeat channel 10
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
-        const unsigned base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+        const char *rec = corner_ptr<STORE>(A, cell);
         float v[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, base + (unsigned(q * kCoeffs) + k) * esz, 0);
+        for (int q = 0; q < 8; ++q) v[q] = load_coeff_at<STORE>(rec, unsigned(q * kCoeffs) + k);
         __builtin_amdgcn_sched_barrier(0);
         const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
@@ -448,11 +472,11 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
         constexpr int kWords = 8 * kCoeffs * int(esz) / 4;   // 88 (f32) or 44 (16-bit)
         constexpr int kVec = kWords / 4;                      // 22 or 11 loads of 16 bytes
-        const unsigned base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+        const u32x4 *rec = reinterpret_cast<const u32x4 *>(corner_ptr<STORE>(A, cell));
         unsigned w[kWords];
 #pragma unroll
         for (int j = 0; j < kVec; ++j) {
-            const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, int(base + 16u * unsigned(j)), 0, 0);
+            const u32x4 q = rec[j];
             w[4 * j] = q.x;
             w[4 * j + 1] = q.y;
             w[4 * j + 2] = q.z;
@@ -478,7 +502,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         }
         return true;
     }
-    const unsigned row00 = A.brick_base / esz + (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
+    const unsigned row00 = unsigned(A.brick_base / esz) + (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
     const unsigned row01 = row00 + unsigned(A.np);
     const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
     const unsigned row11 = row10 + unsigned(A.np);
@@ -561,7 +585,7 @@ __device__ __forceinline__ bool sample_brick_numpy32(const RenderArgs &A, __amdg
     const float wt = (yf == A.ht.last) ? 1.0f : (yf - tl) / (th - tl);
     const float wp = (float(pp) - pl) / (p_hi - pl);
     const float ar = 1.0f - wr, at = 1.0f - wt, ap = 1.0f - wp;
-    const unsigned row00 = A.brick_base / 4u + (unsigned(ir) * unsigned(A.nt) + unsigned(it)) * unsigned(A.np);
+    const unsigned row00 = unsigned(A.brick_base / 4u) + (unsigned(ir) * unsigned(A.nt) + unsigned(it)) * unsigned(A.np);
     const unsigned row01 = row00 + unsigned(A.np);
     const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
     const unsigned row11 = row10 + unsigned(A.np);
@@ -954,7 +978,7 @@ struct alignas(16) CoopSlot {
     float x, y;
     CellGuess<float> g;
     // exact transfer, whose Stokes step costs the lone wave more: the whole lookup, done by the helper
-    unsigned cell;              // byte offset of the midpoint's cell record, ~0u outside the brick
+    unsigned cell;              // index of the midpoint's cell record, ~0u outside the brick
     float wr, wt, wp;
     unsigned pad[3];
 };
@@ -969,12 +993,9 @@ struct CoopLds {
     CoopSlot ring[kCoopRing];
 };
 
-// Byte offset of a cell's record in the corner layout (8 corners x 11 channels, corner-major).
-template <int STORE>
+// Index of a cell's record in the corner layout (8 corners x 11 channels, corner-major).
 __device__ __forceinline__ unsigned corner_record(const RenderArgs &A, int r0, int t0, int p0) {
-    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
-    const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
-    return A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+    return (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
 }
 
 // Ring indices: a wave's LDS operations execute in order, so a relaxed store of the index after the
@@ -1082,7 +1103,7 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
             sl.flags = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u) | (in ? 16u : 0u);
             sl.ds = ds;
             if constexpr (EXACT) {
-                sl.cell = in ? corner_record<STORE>(A, cp.r0, cp.t0, cp.p0) : ~0u;
+                sl.cell = in ? corner_record(A, cp.r0, cp.t0, cp.p0) : ~0u;
                 sl.wr = cp.wr;
                 sl.wt = cp.wt;
                 sl.wp = cp.wp;
@@ -1100,7 +1121,7 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
 }
 
 // One step taken from the ring by the lone wave: its end point and flags, the midpoint's cell record
-// (~0u outside the brick) and weights, ds, and channel k's 8 corner values.
+// index (~0u outside the brick) and weights, ds, and channel k's 8 corner values.
 struct CoopStep {
     float r, th, ph, pr, pth;
     unsigned flags, cell;
@@ -1146,7 +1167,7 @@ __device__ __forceinline__ bool coop_locate(const RenderArgs &A, const float *gr
     if (sl.flags & 16u) {
         CellPos<float> cp;
         locate_finish(A, gr, gt, gp, sl.x, sl.y, sl.g, cp);
-        st.cell = corner_record<STORE>(A, cp.r0, cp.t0, cp.p0);
+        st.cell = corner_record(A, cp.r0, cp.t0, cp.p0);
         st.wr = cp.wr;
         st.wt = cp.wt;
         st.wp = cp.wp;
@@ -1154,14 +1175,14 @@ __device__ __forceinline__ bool coop_locate(const RenderArgs &A, const float *gr
     return true;
 }
 
-// Channel k's 8 corner values of the record at byte offset rec.  Every step issues these 8 loads, from
-// the brick's first record when it has no cell to read, so the count of loads in flight is the same on
-// every path and the wait before a step's arithmetic leaves the next step's loads outstanding.
+// Channel k's 8 corner values of cell record `cell`.  Every step issues these 8 loads, from the brick's
+// first record when it has no cell to read, so the count of loads in flight is the same on every path
+// and the wait before a step's arithmetic leaves the next step's loads outstanding.
 template <int STORE>
-__device__ __forceinline__ void coop_loads(__amdgpu_buffer_rsrc_t rs, unsigned rec, unsigned k, CoopStep &st) {
-    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+__device__ __forceinline__ void coop_loads(const RenderArgs &A, unsigned cell, unsigned k, CoopStep &st) {
+    const char *rec = corner_ptr<STORE>(A, cell);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) st.v[q] = load_coeff<STORE>(rs, rec + (unsigned(q * kCoeffs) + k) * esz, 0);
+    for (int q = 0; q < 8; ++q) st.v[q] = load_coeff_at<STORE>(rec, unsigned(q * kCoeffs) + k);
 }
 
 // One step of the lone wave: start the next step (slot `next`: lookup and loads), then integrate the
@@ -1176,7 +1197,7 @@ __device__ __forceinline__ bool coop_step(const RenderArgs &A, const HotParams<f
     const bool geo_end = (cur.flags & 3u) != 0 || (cur.flags & 4u) == 0 || s.steps + 1u >= P.max_steps;
     bool have_next = false;
     if (!geo_end) have_next = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, nxt);
-    coop_loads<STORE>(rs, (have_next && nxt.cell != ~0u) ? nxt.cell : A.brick_base, k, nxt);
+    coop_loads<STORE>(A, (have_next && nxt.cell != ~0u) ? nxt.cell : 0u, k, nxt);
     s.clamped = s.clamped || (cur.flags & 8u);
     if (cur.cell != ~0u) {
         float w8[8];
@@ -1229,7 +1250,7 @@ __device__ __forceinline__ int coop_consume(const RenderArgs &A, const HotParams
     unsigned next = 0;   // ring slots taken
     CoopStep a, b;
     const bool ok = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, a);
-    coop_loads<STORE>(rs, (ok && a.cell != ~0u) ? a.cell : A.brick_base, k, a);
+    coop_loads<STORE>(A, (ok && a.cell != ~0u) ? a.cell : 0u, k, a);
     if (!ok) {
         if (lane == 0) lds_store(&c->stop, 1);
         return -1;
@@ -1929,7 +1950,7 @@ static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A)
     A.np = b.np;
     const unsigned esz = b.storage == BH_STORE_F32 ? 4u : 2u;
     A.plane_bytes = unsigned(b.plane * esz);
-    A.total_bytes = unsigned(b.bytes);
+    A.total_bytes = unsigned(std::min(b.bytes, size_t(0xffffffffu)));   // the SoA buffer resource's bound
     A.grids = b.grids;
     A.hr = b.hr;
     A.ht = b.ht;
@@ -2083,7 +2104,9 @@ int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, i
     const size_t esz = storage == BH_STORE_F32 ? 4 : 2;
     const size_t cells = size_t(nr - 1) * (nt - 1) * np;
     const size_t nbytes = layout == BH_LAYOUT_SOA ? plane * kCoeffs * esz : cells * 8 * kCoeffs * esz;
-    if (nbytes >= (size_t(1) << 31)) return set_error(BH_ERR_VALUE, "coefficient brick is too large");
+    // SoA planes are read through one buffer resource (32-bit offsets); corner records through 64-bit addresses
+    if (layout == BH_LAYOUT_SOA && nbytes >= (size_t(1) << 31))
+        return set_error(BH_ERR_VALUE, "SoA coefficient brick exceeds 2 GiB (use the corner layout)");
     for (size_t i = 0; i < plane * kCoeffs; ++i)
         if (!std::isfinite(coeffs_aos[i])) return set_error(BH_ERR_VALUE, "coefficient brick contains non-finite values");
     std::vector<unsigned char> host(nbytes);
@@ -2279,7 +2302,7 @@ static int batch_args(bh_batch *bt, void *d_iquv, int32_t *d_steps, uint8_t *d_s
     B.grids = bt->d_grids;
     B.ng = bt->ng;
     B.coeffs = bt->d_coeffs;
-    B.total_bytes = unsigned(bt->coeff_bytes);
+    B.total_bytes = unsigned(std::min(bt->coeff_bytes, size_t(0xffffffffu)));
     B.iquv = d_iquv;
     B.steps = d_steps;
     B.status = d_status;
@@ -2365,7 +2388,9 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         goff[k] = ng;
         ng += b.nr + b.nt + b.np + 1;
     }
-    if (total >= (size_t(1) << 31)) return set_error(BH_ERR_VALUE, "the bricks of a batch exceed 2 GiB");
+    // SoA bricks are read through one buffer resource (32-bit offsets); corner records through 64-bit addresses
+    if (layout == BH_LAYOUT_SOA && total >= (size_t(1) << 31))
+        return set_error(BH_ERR_VALUE, "the SoA bricks of a batch exceed 2 GiB (corner bricks have no such limit)");
     if (ng > kMaxGrid) return set_error(BH_ERR_VALUE, "the bricks of a batch hold more than %d grid values", kMaxGrid);
     bh_batch *bt = new bh_batch();
     bt->ctx = ctx;
@@ -2404,8 +2429,8 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         A = RenderArgs{};
         if ((st = setup_args(items[i].scene, b, A))) return fail(st);
         A.coeffs = bt->d_coeffs;
-        A.total_bytes = unsigned(total);
-        A.brick_base = unsigned(coff[kmap[i]]);
+        A.total_bytes = unsigned(std::min(total, size_t(0xffffffffu)));   // SoA batches stay below 2 GiB
+        A.brick_base = coff[kmap[i]];
         A.grid_off = goff[kmap[i]];
         A.out_off = pix;
         if (offsets) offsets[i] = pix;

@@ -125,7 +125,8 @@ int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs,
 
 /* Upload an AoS float32 brick (nr, nt, np, 11) with float32 grids, stored on
  * the device in the requested type and layout.  The FP64 render needs
- * BH_STORE_F32 + BH_LAYOUT_SOA. */
+ * BH_STORE_F32 + BH_LAYOUT_SOA.  nr + nt + np <= 2048; a SoA brick stays below
+ * 2 GiB, a corner-packed one has no limit beyond device memory. */
 int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos,
                     const float *r_grid, int32_t nr,
                     const float *theta_grid, int32_t nt,

@@ -207,6 +207,36 @@ def test_corner_layout_matches_soa(renderer, storage, precision):
     assert rel_l1(ob["iquv"][..., 0][ok].astype(np.float64), oa["iquv"][..., 0][ok].astype(np.float64)) <= 1e-5
 
 
+def test_corner_brick_beyond_4_gib():
+    """Corner records are read through 64-bit addresses: a 256 x 192 x 256 FP32 brick (4.39 GB of corner
+    records, past what a 32-bit buffer offset reaches) renders like its SoA copy (0.55 GB).  Rays from
+    the camera enter the brick through its outermost radial cells, whose records lie beyond 4 GiB."""
+    from bhgrrt.render import Renderer
+
+    print("corner brick > 4 GiB: building 256x192x256", flush=True)
+    br = scenes.make_brick(256, 192, 256, spin=0.94, seed=7, disk=True)
+    cells = 255 * 191 * 256
+    assert cells * 8 * 11 * 4 > (1 << 32)
+    sc = scenes.scale_plain_scene(128, 128)
+    rr = Renderer(0)
+    try:
+        a = rr.upload(br, "f32", "soa")
+        b = rr.upload(br, "f32", "corner")
+        print("corner brick > 4 GiB: uploaded, rendering", flush=True)
+        oa = rr.render(sc, a, PREC_FP32)
+        ob = rr.render(sc, b, PREC_FP32)
+        _, samples = rr.last_work()
+    finally:
+        rr.close()
+    ok = ((oa["status"] | ob["status"]) & orender.ST_POLE_CLAMPED) == 0
+    l1 = rel_l1(ob["iquv"][..., 0][ok].astype(np.float64), oa["iquv"][..., 0][ok].astype(np.float64))
+    print(f"corner brick > 4 GiB: {samples} samples, statuses equal on {np.mean(oa['status'] == ob['status']):.4f}, "
+          f"rel L1 I {l1:.2e}")
+    assert samples > 0
+    assert np.mean(oa["status"] == ob["status"]) >= 0.995
+    assert l1 <= 1e-5
+
+
 @pytest.mark.parametrize("storage,layout,precision", [("f32", "corner", PREC_FP32), ("bf16", "corner", PREC_MIXED),
                                                       ("f32", "soa", PREC_FP64)])
 def test_batch_matches_individual_renders(renderer, storage, layout, precision):
