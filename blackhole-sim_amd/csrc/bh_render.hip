@@ -64,7 +64,7 @@ struct RenderArgs {
     const void *coeffs;
     int nr, nt, np;
     unsigned plane_bytes, total_bytes;
-    unsigned long long brick_base;   // byte offset of this brick inside the coefficient buffer (batches merge bricks)
+    unsigned brick_base;        // byte offset of this brick inside the coefficient buffer (batches merge bricks), low 32 bits
     int grid_off;               // offset of this brick's r grid in the LDS grid table
     long long out_off;          // first pixel of this render in the batch output arrays
     const float *grids;
@@ -81,6 +81,7 @@ struct RenderArgs {
     int steps_per_sync, refill_min;
     int coop_steps;             // lone steps before a lone ray asks an idle wave for help (0: never)
     int lerp32;                 // FP64 render: sample as the reference renderer does on a float32 brick
+    unsigned brick_base_hi;     // high 32 bits of brick_base (merged corner bricks of 4 GiB and more)
 };
 
 template <typename R>
@@ -277,11 +278,18 @@ __device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R
 
 // ---- brick sampler -----------------------------------------------------------
 
+// Storage codes inside the kernels: BH_STORE_* in bits 0-1, plus kStoreWide for corner bricks of 2 GiB and
+// more, whose records are read through 64-bit addresses; smaller bricks are read through a buffer
+// resource (32-bit offsets, one cheaper address operation per sample), so they get kernels of their own.
+constexpr int kStoreWide = 4;
+__host__ __device__ constexpr int store_of(int s) { return s & 3; }
+__host__ __device__ constexpr bool wide_of(int s) { return (s & kStoreWide) != 0; }
+
 template <int STORE>
 __device__ __forceinline__ float load_coeff(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff) {
-    if constexpr (STORE == BH_STORE_F32) {
+    if constexpr (store_of(STORE) == BH_STORE_F32) {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, int(voff), int(soff), 0));
-    } else if constexpr (STORE == BH_STORE_BF16) {
+    } else if constexpr (store_of(STORE) == BH_STORE_BF16) {
         unsigned h = __builtin_amdgcn_raw_buffer_load_b16(rs, int(voff), int(soff), 0);
         return __builtin_bit_cast(float, h << 16);
     } else {
@@ -290,13 +298,13 @@ __device__ __forceinline__ float load_coeff(__amdgpu_buffer_rsrc_t rs, unsigned 
     }
 }
 
-// Element idx of a corner record (STORE: f32, bf16 or f16) as FP32.  Corner records are read through
-// 64-bit addresses, so a corner brick may exceed the 4 GiB a buffer resource covers.
+// Element idx of a corner record (STORE: f32, bf16 or f16) as FP32, through a 64-bit address (wide
+// bricks).
 template <int STORE>
 __device__ __forceinline__ float load_coeff_at(const char *rec, unsigned idx) {
-    if constexpr (STORE == BH_STORE_F32) {
+    if constexpr (store_of(STORE) == BH_STORE_F32) {
         return reinterpret_cast<const float *>(rec)[idx];
-    } else if constexpr (STORE == BH_STORE_BF16) {
+    } else if constexpr (store_of(STORE) == BH_STORE_BF16) {
         const unsigned h = reinterpret_cast<const unsigned short *>(rec)[idx];
         return __builtin_bit_cast(float, h << 16);
     } else {
@@ -304,14 +312,15 @@ __device__ __forceinline__ float load_coeff_at(const char *rec, unsigned idx) {
     }
 }
 
-// Address of cell record `cell` (8 corners x 11 channels, corner-major) of the render's brick.
+// Address of cell record `cell` (8 corners x 11 channels, corner-major) of the render's brick (wide bricks).
 template <int STORE>
 __device__ __forceinline__ const char *corner_ptr(const RenderArgs &A, unsigned cell) {
-    constexpr size_t rec = 8u * kCoeffs * (STORE == BH_STORE_F32 ? 4u : 2u);
+    constexpr size_t rec = 8u * kCoeffs * (store_of(STORE) == BH_STORE_F32 ? 4u : 2u);
     // the index stays 32-bit arithmetic (full-rate multiply-adds); only the byte address is 64-bit, one
     // multiply-add (folded, the whole index chain would become 64-bit multiply-adds)
     asm("" : "+v"(cell));
-    return static_cast<const char *>(A.coeffs) + A.brick_base + size_t(cell) * rec;
+    const size_t base = (size_t(A.brick_base_hi) << 32) | A.brick_base;
+    return static_cast<const char *>(A.coeffs) + base + size_t(cell) * rec;
 }
 
 // Index guess from the hint, clamped to [0, nmax]; NaN -> 0.  Only the r grid may be geometric
@@ -445,16 +454,22 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
     if (!locate(A, gr, gt, gp, x, y, z, cp)) return false;
     const int r0 = cp.r0, t0 = cp.t0, p0 = cp.p0, p1 = cp.p1;
     const R wr = cp.wr, wt = cp.wt, wp = cp.wp;
-    constexpr unsigned esz = STORE == BH_STORE_F32 ? 4u : 2u;
+    constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
     if constexpr (COOP) {
         static_assert(sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER, "the cooperative sample is FP32, corner layout");
         (void)p1;
         const unsigned k = min(unsigned(threadIdx.x & 63), unsigned(kCoeffs - 1));   // lanes >= 11 repeat channel 10
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
-        const char *rec = corner_ptr<STORE>(A, cell);
         float v[8];
+        if constexpr (wide_of(STORE)) {
+            const char *rec = corner_ptr<STORE>(A, cell);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = load_coeff_at<STORE>(rec, unsigned(q * kCoeffs) + k);
+            for (int q = 0; q < 8; ++q) v[q] = load_coeff_at<STORE>(rec, unsigned(q * kCoeffs) + k);
+        } else {
+            const unsigned base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, base + (unsigned(q * kCoeffs) + k) * esz, 0);
+        }
         __builtin_amdgcn_sched_barrier(0);
         const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
@@ -472,11 +487,16 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
         constexpr int kWords = 8 * kCoeffs * int(esz) / 4;   // 88 (f32) or 44 (16-bit)
         constexpr int kVec = kWords / 4;                      // 22 or 11 loads of 16 bytes
-        const u32x4 *rec = reinterpret_cast<const u32x4 *>(corner_ptr<STORE>(A, cell));
         unsigned w[kWords];
+        const u32x4 *rec = nullptr;
+        unsigned base = 0;
+        if constexpr (wide_of(STORE)) rec = reinterpret_cast<const u32x4 *>(corner_ptr<STORE>(A, cell));
+        else base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
 #pragma unroll
         for (int j = 0; j < kVec; ++j) {
-            const u32x4 q = rec[j];
+            u32x4 q;
+            if constexpr (wide_of(STORE)) q = rec[j];
+            else q = __builtin_amdgcn_raw_buffer_load_b128(rs, int(base + 16u * unsigned(j)), 0, 0);
             w[4 * j] = q.x;
             w[4 * j + 1] = q.y;
             w[4 * j + 2] = q.z;
@@ -494,15 +514,15 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
             for (int q = 0; q < 8; ++q) {
                 const int e = q * kCoeffs + k;
                 float v;
-                if constexpr (STORE == BH_STORE_F32) v = __builtin_bit_cast(float, w[e]);
-                else v = (e & 1) ? half_hi(w[e >> 1], STORE) : half_lo(w[e >> 1], STORE);
+                if constexpr (store_of(STORE) == BH_STORE_F32) v = __builtin_bit_cast(float, w[e]);
+                else v = (e & 1) ? half_hi(w[e >> 1], store_of(STORE)) : half_lo(w[e >> 1], store_of(STORE));
                 acc = (q == 0) ? w8[0] * R(v) : acc + w8[q] * R(v);
             }
             out[k] = acc;
         }
         return true;
     }
-    const unsigned row00 = unsigned(A.brick_base / esz) + (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
+    const unsigned row00 = A.brick_base / esz + (unsigned(r0) * unsigned(A.nt) + unsigned(t0)) * unsigned(A.np);
     const unsigned row01 = row00 + unsigned(A.np);
     const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
     const unsigned row11 = row10 + unsigned(A.np);
@@ -585,7 +605,7 @@ __device__ __forceinline__ bool sample_brick_numpy32(const RenderArgs &A, __amdg
     const float wt = (yf == A.ht.last) ? 1.0f : (yf - tl) / (th - tl);
     const float wp = (float(pp) - pl) / (p_hi - pl);
     const float ar = 1.0f - wr, at = 1.0f - wt, ap = 1.0f - wp;
-    const unsigned row00 = unsigned(A.brick_base / 4u) + (unsigned(ir) * unsigned(A.nt) + unsigned(it)) * unsigned(A.np);
+    const unsigned row00 = A.brick_base / 4u + (unsigned(ir) * unsigned(A.nt) + unsigned(it)) * unsigned(A.np);
     const unsigned row01 = row00 + unsigned(A.np);
     const unsigned row10 = row00 + unsigned(A.nt) * unsigned(A.np);
     const unsigned row11 = row10 + unsigned(A.np);
@@ -872,7 +892,7 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> 
     s.clamped = s.clamped || b.hit;
     R c[kCoeffs];
     bool inside;
-    if constexpr (sizeof(R) == 8 && STORE == BH_STORE_F32 && LAYOUT == BH_LAYOUT_SOA) {
+    if constexpr (sizeof(R) == 8 && store_of(STORE) == BH_STORE_F32 && LAYOUT == BH_LAYOUT_SOA) {
         if (A.lerp32)
             inside = sample_brick_numpy32(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th), midpoint(s.ph, b.ph), c);
         else
@@ -978,7 +998,7 @@ struct alignas(16) CoopSlot {
     float x, y;
     CellGuess<float> g;
     // exact transfer, whose Stokes step costs the lone wave more: the whole lookup, done by the helper
-    unsigned cell;              // index of the midpoint's cell record, ~0u outside the brick
+    unsigned cell;              // corner_record of the midpoint's cell, ~0u outside the brick
     float wr, wt, wp;
     unsigned pad[3];
 };
@@ -993,9 +1013,20 @@ struct CoopLds {
     CoopSlot ring[kCoopRing];
 };
 
-// Index of a cell's record in the corner layout (8 corners x 11 channels, corner-major).
+// A cell's record in the corner layout (8 corners x 11 channels, corner-major): its byte offset in the
+// coefficient buffer, or for wide bricks (whose offsets may pass 32 bits) its index in the brick.
+template <int STORE>
 __device__ __forceinline__ unsigned corner_record(const RenderArgs &A, int r0, int t0, int p0) {
-    return (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
+    const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
+    if constexpr (wide_of(STORE)) return cell;
+    constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
+    return A.brick_base + cell * unsigned(8 * kCoeffs * esz);
+}
+// corner_record of the brick's first cell
+template <int STORE>
+__device__ __forceinline__ unsigned corner_first(const RenderArgs &A) {
+    if constexpr (wide_of(STORE)) return 0u;
+    return A.brick_base;
 }
 
 // Ring indices: a wave's LDS operations execute in order, so a relaxed store of the index after the
@@ -1103,7 +1134,7 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
             sl.flags = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u) | (in ? 16u : 0u);
             sl.ds = ds;
             if constexpr (EXACT) {
-                sl.cell = in ? corner_record(A, cp.r0, cp.t0, cp.p0) : ~0u;
+                sl.cell = in ? corner_record<STORE>(A, cp.r0, cp.t0, cp.p0) : ~0u;
                 sl.wr = cp.wr;
                 sl.wt = cp.wt;
                 sl.wp = cp.wp;
@@ -1121,7 +1152,7 @@ __device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotP
 }
 
 // One step taken from the ring by the lone wave: its end point and flags, the midpoint's cell record
-// index (~0u outside the brick) and weights, ds, and channel k's 8 corner values.
+// (corner_record, ~0u outside the brick) and weights, ds, and channel k's 8 corner values.
 struct CoopStep {
     float r, th, ph, pr, pth;
     unsigned flags, cell;
@@ -1167,7 +1198,7 @@ __device__ __forceinline__ bool coop_locate(const RenderArgs &A, const float *gr
     if (sl.flags & 16u) {
         CellPos<float> cp;
         locate_finish(A, gr, gt, gp, sl.x, sl.y, sl.g, cp);
-        st.cell = corner_record(A, cp.r0, cp.t0, cp.p0);
+        st.cell = corner_record<STORE>(A, cp.r0, cp.t0, cp.p0);
         st.wr = cp.wr;
         st.wt = cp.wt;
         st.wp = cp.wp;
@@ -1175,14 +1206,21 @@ __device__ __forceinline__ bool coop_locate(const RenderArgs &A, const float *gr
     return true;
 }
 
-// Channel k's 8 corner values of cell record `cell`.  Every step issues these 8 loads, from the brick's
-// first record when it has no cell to read, so the count of loads in flight is the same on every path
-// and the wait before a step's arithmetic leaves the next step's loads outstanding.
+// Channel k's 8 corner values of cell record `rec` (corner_record).  Every step issues these 8 loads, from
+// the brick's first record when it has no cell to read, so the count of loads in flight is the same on
+// every path and the wait before a step's arithmetic leaves the next step's loads outstanding.
 template <int STORE>
-__device__ __forceinline__ void coop_loads(const RenderArgs &A, unsigned cell, unsigned k, CoopStep &st) {
-    const char *rec = corner_ptr<STORE>(A, cell);
+__device__ __forceinline__ void coop_loads(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, unsigned rec, unsigned k,
+                                           CoopStep &st) {
+    if constexpr (wide_of(STORE)) {
+        const char *p = corner_ptr<STORE>(A, rec);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) st.v[q] = load_coeff_at<STORE>(rec, unsigned(q * kCoeffs) + k);
+        for (int q = 0; q < 8; ++q) st.v[q] = load_coeff_at<STORE>(p, unsigned(q * kCoeffs) + k);
+    } else {
+        constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) st.v[q] = load_coeff<STORE>(rs, rec + (unsigned(q * kCoeffs) + k) * esz, 0);
+    }
 }
 
 // One step of the lone wave: start the next step (slot `next`: lookup and loads), then integrate the
@@ -1197,7 +1235,7 @@ __device__ __forceinline__ bool coop_step(const RenderArgs &A, const HotParams<f
     const bool geo_end = (cur.flags & 3u) != 0 || (cur.flags & 4u) == 0 || s.steps + 1u >= P.max_steps;
     bool have_next = false;
     if (!geo_end) have_next = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, nxt);
-    coop_loads<STORE>(A, (have_next && nxt.cell != ~0u) ? nxt.cell : 0u, k, nxt);
+    coop_loads<STORE>(A, rs, (have_next && nxt.cell != ~0u) ? nxt.cell : corner_first<STORE>(A), k, nxt);
     s.clamped = s.clamped || (cur.flags & 8u);
     if (cur.cell != ~0u) {
         float w8[8];
@@ -1250,7 +1288,7 @@ __device__ __forceinline__ int coop_consume(const RenderArgs &A, const HotParams
     unsigned next = 0;   // ring slots taken
     CoopStep a, b;
     const bool ok = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, a);
-    coop_loads<STORE>(A, (ok && a.cell != ~0u) ? a.cell : 0u, k, a);
+    coop_loads<STORE>(A, rs, (ok && a.cell != ~0u) ? a.cell : corner_first<STORE>(A), k, a);
     if (!ok) {
         if (lane == 0) lds_store(&c->stop, 1);
         return -1;
@@ -1874,18 +1912,19 @@ static void *kernel_of() {
     return BATCH ? reinterpret_cast<void *>(K::batch()) : reinterpret_cast<void *>(K::render());
 }
 
-template <bool BATCH, int LAYOUT, bool EXACT, bool RK4>
+// W: kStoreWide for the 64-bit-addressed corner kernels (corner bricks of 2 GiB and more), else 0.
+template <bool BATCH, int LAYOUT, bool EXACT, bool RK4, int W = 0>
 static void *pick_fp32(int storage, bool mixed) {
     switch (storage) {
         case BH_STORE_F32:
-            return mixed ? kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, true, EXACT, RK4>()
-                         : kernel_of<BATCH, float, BH_STORE_F32, LAYOUT, false, EXACT, RK4>();
+            return mixed ? kernel_of<BATCH, float, BH_STORE_F32 | W, LAYOUT, true, EXACT, RK4>()
+                         : kernel_of<BATCH, float, BH_STORE_F32 | W, LAYOUT, false, EXACT, RK4>();
         case BH_STORE_BF16:
-            return mixed ? kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, true, EXACT, RK4>()
-                         : kernel_of<BATCH, float, BH_STORE_BF16, LAYOUT, false, EXACT, RK4>();
+            return mixed ? kernel_of<BATCH, float, BH_STORE_BF16 | W, LAYOUT, true, EXACT, RK4>()
+                         : kernel_of<BATCH, float, BH_STORE_BF16 | W, LAYOUT, false, EXACT, RK4>();
         case BH_STORE_F16:
-            return mixed ? kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, true, EXACT, RK4>()
-                         : kernel_of<BATCH, float, BH_STORE_F16, LAYOUT, false, EXACT, RK4>();
+            return mixed ? kernel_of<BATCH, float, BH_STORE_F16 | W, LAYOUT, true, EXACT, RK4>()
+                         : kernel_of<BATCH, float, BH_STORE_F16 | W, LAYOUT, false, EXACT, RK4>();
     }
     return nullptr;
 }
@@ -1894,30 +1933,35 @@ static void *pick_fp32(int storage, bool mixed) {
 // kernel (a uniform branch on the rule cost the mixed batch 14 %).  RK4 is built for the FP64 render
 // and the corner-packed FP32 / mixed renders (the layouts the reference renderer's drop-in uploads).
 template <bool BATCH, bool EXACT, bool RK4>
-static void *pick_any(int precision, int storage, int layout) {
+static void *pick_any(int precision, int storage, int layout, bool wide) {
     if (precision == BH_PREC_FP64)
         return (storage == BH_STORE_F32 && layout == BH_LAYOUT_SOA)
                    ? kernel_of<BATCH, double, BH_STORE_F32, BH_LAYOUT_SOA, false, EXACT, RK4>()
                    : nullptr;
     const bool mixed = precision == BH_PREC_MIXED;
-    if (layout == BH_LAYOUT_CORNER) return pick_fp32<BATCH, BH_LAYOUT_CORNER, EXACT, RK4>(storage, mixed);
+    if (layout == BH_LAYOUT_CORNER)
+        return wide ? pick_fp32<BATCH, BH_LAYOUT_CORNER, EXACT, RK4, kStoreWide>(storage, mixed)
+                    : pick_fp32<BATCH, BH_LAYOUT_CORNER, EXACT, RK4>(storage, mixed);
     if constexpr (RK4) return nullptr;
     else return pick_fp32<BATCH, BH_LAYOUT_SOA, EXACT, false>(storage, mixed);
 }
 
 template <bool BATCH>
-static void *pick_rule(int precision, int storage, int layout, bool exact, bool rk4) {
-    if (rk4) return exact ? pick_any<BATCH, true, true>(precision, storage, layout) : pick_any<BATCH, false, true>(precision, storage, layout);
-    return exact ? pick_any<BATCH, true, false>(precision, storage, layout) : pick_any<BATCH, false, false>(precision, storage, layout);
+static void *pick_rule(int precision, int storage, int layout, bool exact, bool rk4, bool wide) {
+    if (rk4) return exact ? pick_any<BATCH, true, true>(precision, storage, layout, wide) : pick_any<BATCH, false, true>(precision, storage, layout, wide);
+    return exact ? pick_any<BATCH, true, false>(precision, storage, layout, wide) : pick_any<BATCH, false, false>(precision, storage, layout, wide);
 }
 
-static RenderFn pick_kernel(int precision, int storage, int layout, bool exact, bool rk4) {
-    return reinterpret_cast<RenderFn>(pick_rule<false>(precision, storage, layout, exact, rk4));
+static RenderFn pick_kernel(int precision, int storage, int layout, bool exact, bool rk4, bool wide) {
+    return reinterpret_cast<RenderFn>(pick_rule<false>(precision, storage, layout, exact, rk4, wide));
 }
 
-static BatchFn pick_batch_kernel(int precision, int storage, int layout, bool exact, bool rk4) {
-    return reinterpret_cast<BatchFn>(pick_rule<true>(precision, storage, layout, exact, rk4));
+static BatchFn pick_batch_kernel(int precision, int storage, int layout, bool exact, bool rk4, bool wide) {
+    return reinterpret_cast<BatchFn>(pick_rule<true>(precision, storage, layout, exact, rk4, wide));
 }
+
+// Corner bricks (or a batch's merged corner bricks) of 2 GiB and more take the 64-bit-addressed kernels.
+static bool wide_corner(int layout, size_t bytes) { return layout == BH_LAYOUT_CORNER && bytes >= (size_t(1) << 31); }
 
 static int setup_args(const bh_scene_desc &sc, const BrickDev &b, RenderArgs &A) {
     int st;
@@ -2005,7 +2049,8 @@ static int render_enqueue(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick
         return set_error(BH_ERR_VALUE, "precision must be BH_PREC_FP64, BH_PREC_FP32 or BH_PREC_MIXED");
     const BrickDev &b = ctx->bricks[brick_id];
     const bool rk4 = scene->integrator == BH_INTEGRATOR_RK4;
-    RenderFn fn = pick_kernel(precision, b.storage, b.layout, scene->transfer == BH_TRANSFER_EXACT, rk4);
+    RenderFn fn = pick_kernel(precision, b.storage, b.layout, scene->transfer == BH_TRANSFER_EXACT, rk4,
+                              wide_corner(b.layout, b.bytes));
     if (!fn)
         return set_error(BH_ERR_VALUE, precision == BH_PREC_FP64
                                            ? "the FP64 render needs a float32 brick in the SoA layout"
@@ -2370,12 +2415,6 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         if (items[i].scene.integrator != items[0].scene.integrator)
             return set_error(BH_ERR_VALUE, "all renders of a batch must share the integrator");
     }
-    BatchFn fn = pick_batch_kernel(precision, storage, layout, items[0].scene.transfer == BH_TRANSFER_EXACT,
-                                   items[0].scene.integrator == BH_INTEGRATOR_RK4);
-    if (!fn)
-        return set_error(BH_ERR_VALUE, precision == BH_PREC_FP64
-                                           ? "the FP64 batch needs float32 bricks in the SoA layout"
-                                           : "the RK4 integrator of the FP32 and mixed batches needs the corner layout");
     // merged coefficient buffer and grid table
     std::vector<size_t> coff(uniq.size());
     std::vector<int> goff(uniq.size());
@@ -2388,9 +2427,16 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         goff[k] = ng;
         ng += b.nr + b.nt + b.np + 1;
     }
-    // SoA bricks are read through one buffer resource (32-bit offsets); corner records through 64-bit addresses
+    // SoA bricks are read through one buffer resource (32-bit offsets); corner records of 2 GiB and more
+    // through 64-bit addresses
     if (layout == BH_LAYOUT_SOA && total >= (size_t(1) << 31))
         return set_error(BH_ERR_VALUE, "the SoA bricks of a batch exceed 2 GiB (corner bricks have no such limit)");
+    BatchFn fn = pick_batch_kernel(precision, storage, layout, items[0].scene.transfer == BH_TRANSFER_EXACT,
+                                   items[0].scene.integrator == BH_INTEGRATOR_RK4, wide_corner(layout, total));
+    if (!fn)
+        return set_error(BH_ERR_VALUE, precision == BH_PREC_FP64
+                                           ? "the FP64 batch needs float32 bricks in the SoA layout"
+                                           : "the RK4 integrator of the FP32 and mixed batches needs the corner layout");
     if (ng > kMaxGrid) return set_error(BH_ERR_VALUE, "the bricks of a batch hold more than %d grid values", kMaxGrid);
     bh_batch *bt = new bh_batch();
     bt->ctx = ctx;
@@ -2430,7 +2476,8 @@ int bh_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n, int32_t 
         if ((st = setup_args(items[i].scene, b, A))) return fail(st);
         A.coeffs = bt->d_coeffs;
         A.total_bytes = unsigned(std::min(total, size_t(0xffffffffu)));   // SoA batches stay below 2 GiB
-        A.brick_base = coff[kmap[i]];
+        A.brick_base = unsigned(coff[kmap[i]]);
+        A.brick_base_hi = unsigned(coff[kmap[i]] >> 32);
         A.grid_off = goff[kmap[i]];
         A.out_off = pix;
         if (offsets) offsets[i] = pix;
@@ -2940,39 +2987,55 @@ typedef void (*SurFn)(SurArgs);
 typedef void (*SurBatchFn)(SurBatchArgs);
 
 // The batched RK2 kernel in list mode, for the fallback rays of batched surrogate renders (FP32 or mixed
-// precision, RK2 transfer and integrator).
-static BatchFn pick_list_kernel(int storage, int layout, bool mixed) {
+// precision, RK2 transfer and integrator); W as in pick_fp32.
+template <int W>
+static BatchFn pick_list_kernel_w(int storage, int layout, bool mixed) {
 #define BH_LIST(S, L) (mixed ? batch_kernel<float, S, L, true, false, false, true> : batch_kernel<float, S, L, false, false, false, true>)
     if (layout == BH_LAYOUT_CORNER) {
-        if (storage == BH_STORE_F32) return BH_LIST(BH_STORE_F32, BH_LAYOUT_CORNER);
-        if (storage == BH_STORE_BF16) return BH_LIST(BH_STORE_BF16, BH_LAYOUT_CORNER);
-        return BH_LIST(BH_STORE_F16, BH_LAYOUT_CORNER);
+        if (storage == BH_STORE_F32) return BH_LIST(BH_STORE_F32 | W, BH_LAYOUT_CORNER);
+        if (storage == BH_STORE_BF16) return BH_LIST(BH_STORE_BF16 | W, BH_LAYOUT_CORNER);
+        return BH_LIST(BH_STORE_F16 | W, BH_LAYOUT_CORNER);
     }
-    if (storage == BH_STORE_F32) return BH_LIST(BH_STORE_F32, BH_LAYOUT_SOA);
-    if (storage == BH_STORE_BF16) return BH_LIST(BH_STORE_BF16, BH_LAYOUT_SOA);
-    return BH_LIST(BH_STORE_F16, BH_LAYOUT_SOA);
+    if constexpr (W == 0) {
+        if (storage == BH_STORE_F32) return BH_LIST(BH_STORE_F32, BH_LAYOUT_SOA);
+        if (storage == BH_STORE_BF16) return BH_LIST(BH_STORE_BF16, BH_LAYOUT_SOA);
+        return BH_LIST(BH_STORE_F16, BH_LAYOUT_SOA);
+    }
+    return nullptr;
 #undef BH_LIST
 }
+static BatchFn pick_list_kernel(int storage, int layout, bool mixed, bool wide) {
+    return wide ? pick_list_kernel_w<kStoreWide>(storage, layout, mixed) : pick_list_kernel_w<0>(storage, layout, mixed);
+}
 
-template <typename KT>
+template <typename KT, int W>
 static SurFn pick_surrogate_t(int storage, int layout) {
     if (layout == BH_LAYOUT_CORNER) {
-        if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_CORNER, KT>;
-        if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_CORNER, KT>;
-        return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_CORNER, KT>;
+        if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32 | W, BH_LAYOUT_CORNER, KT>;
+        if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16 | W, BH_LAYOUT_CORNER, KT>;
+        return surrogate_kernel<BH_STORE_F16 | W, BH_LAYOUT_CORNER, KT>;
     }
-    if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_SOA, KT>;
-    if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_SOA, KT>;
-    return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_SOA, KT>;
+    if constexpr (W == 0) {
+        if (storage == BH_STORE_F32) return surrogate_kernel<BH_STORE_F32, BH_LAYOUT_SOA, KT>;
+        if (storage == BH_STORE_BF16) return surrogate_kernel<BH_STORE_BF16, BH_LAYOUT_SOA, KT>;
+        return surrogate_kernel<BH_STORE_F16, BH_LAYOUT_SOA, KT>;
+    }
+    return nullptr;
 }
 
 // half_knots: the knots come from the network as FP16; otherwise FP32 rows the caller supplies.
-static SurFn pick_surrogate(int storage, int layout, bool half_knots) {
-    return half_knots ? pick_surrogate_t<_Float16>(storage, layout) : pick_surrogate_t<float>(storage, layout);
+static SurFn pick_surrogate(int storage, int layout, bool half_knots, bool wide) {
+    if (wide) return half_knots ? pick_surrogate_t<_Float16, kStoreWide>(storage, layout) : pick_surrogate_t<float, kStoreWide>(storage, layout);
+    return half_knots ? pick_surrogate_t<_Float16, 0>(storage, layout) : pick_surrogate_t<float, 0>(storage, layout);
 }
 
-static SurBatchFn pick_surrogate_batch(int storage, int layout) {
+static SurBatchFn pick_surrogate_batch(int storage, int layout, bool wide) {
     if (layout == BH_LAYOUT_CORNER) {
+        if (wide) {
+            if (storage == BH_STORE_F32) return surrogate_batch_kernel<BH_STORE_F32 | kStoreWide, BH_LAYOUT_CORNER>;
+            if (storage == BH_STORE_BF16) return surrogate_batch_kernel<BH_STORE_BF16 | kStoreWide, BH_LAYOUT_CORNER>;
+            return surrogate_batch_kernel<BH_STORE_F16 | kStoreWide, BH_LAYOUT_CORNER>;
+        }
         if (storage == BH_STORE_F32) return surrogate_batch_kernel<BH_STORE_F32, BH_LAYOUT_CORNER>;
         if (storage == BH_STORE_BF16) return surrogate_batch_kernel<BH_STORE_BF16, BH_LAYOUT_CORNER>;
         return surrogate_batch_kernel<BH_STORE_F16, BH_LAYOUT_CORNER>;
@@ -3101,7 +3164,8 @@ static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id
     }
     const int tiles_y = (S.R.H + 7) / 8;
     const int n_tiles = S.R.tiles_x * tiles_y;
-    hipLaunchKernelGGL(pick_surrogate(b.storage, b.layout, half), dim3(unsigned((n_tiles + 3) / 4)), dim3(256), 0,
+    hipLaunchKernelGGL(pick_surrogate(b.storage, b.layout, half, wide_corner(b.layout, b.bytes)),
+                       dim3(unsigned((n_tiles + 3) / 4)), dim3(256), 0,
                        ctx->stream, S);
     BH_HIP(hipGetLastError());
     if ((st = render_enqueue(ctx, scene, brick_id, sd->fallback_precision, d_iquv, d_steps, d_status, d_list, d_count,
@@ -3307,8 +3371,9 @@ int bh_surrogate_batch_create(bh_ctx *ctx, const bh_batch_item *items, int32_t n
     if (offsets) std::copy(off.begin(), off.end(), offsets);
     sb->total = off[n];
     if (sb->total >= (int64_t(1) << 31)) return fail(set_error(BH_ERR_VALUE, "a surrogate batch holds at most 2^31 - 1 rays"));
-    sb->sfn = pick_surrogate_batch(sb->fb->storage, sb->fb->layout);
-    sb->list_fn = pick_list_kernel(sb->fb->storage, sb->fb->layout, sd->fallback_precision == BH_PREC_MIXED);
+    const bool wide = wide_corner(sb->fb->layout, sb->fb->coeff_bytes);
+    sb->sfn = pick_surrogate_batch(sb->fb->storage, sb->fb->layout, wide);
+    sb->list_fn = pick_list_kernel(sb->fb->storage, sb->fb->layout, sd->fallback_precision == BH_PREC_MIXED, wide);
     const int K = sd->n_knots, n_out = 4 * K + kHeads;
     sb->in_mul = make_float4(sd->in_mul[0], sd->in_mul[1], sd->in_mul[2], sd->in_mul[3]);
     sb->in_add = make_float4(sd->in_add[0], sd->in_add[1], sd->in_add[2], sd->in_add[3]);
