@@ -47,7 +47,9 @@ __device__ __forceinline__ bool bracket_linear_d(const double *g, int n, double 
 
 // Python float %: fmod, shifted into [0, 2 pi) when negative (native_kernels.py:133-134).
 __device__ __forceinline__ double wrap_phi_d(double phi, double base) {
-    double m = fmod(phi - base, kTwoPi);
+    const double d = phi - base;
+    // fmod returns d itself inside [0, 2 pi); the library call (a loop) only runs outside it
+    double m = (d >= 0.0 && d < kTwoPi) ? d : fmod(d, kTwoPi);
     if (m < 0.0) m += kTwoPi;
     return m + base;
 }
@@ -211,17 +213,33 @@ __global__ __launch_bounds__(256) void sample_gather_kernel(const double *__rest
     }
 }
 
-// ---- points bucketed by brick cell ------------------------------------------------------------
-// Random points gather eight random 88-byte corner records each; sorted by the trilinear cell they fall
-// in, the points of a cell and of its phi neighbours (which share four corners) sit in neighbouring lanes,
-// so one lane per point loads its corners directly: the lanes of a load instruction hit a few records,
-// which come from L1/L2 instead of the fabric, and a wave keeps 64 points in flight.  A counting sort:
-// the histogram's atomic returns each point's rank inside its cell, an exclusive scan turns the counts
-// into cell offsets, and a scatter writes each point (coordinates, index, cell) to offset + rank.  The
-// gather takes the cell from the sort key instead of searching the grids; the weights and the lerps are
-// those of the unsorted gather (same expressions on the same cell), so the results are the same bits.
+// ---- points partitioned by block of brick cells ------------------------------------------------
+// Random points gather eight random 88-byte corner records each, from a brick (92 MB at 128x64x128)
+// that does not fit an XCD's 4 MB L2.  The large-N form partitions the points by block of brick cells
+// (br x bt x bp cells), and one workgroup per block copies the block's corner nodes into LDS once and
+// samples all of the block's points from there: the brick is read about once per call, and the 88
+// corner values of a point come from LDS.  The partition is a counting sort over blocks in two passes
+// over the points, each workgroup counting a fixed chunk of points into an LDS histogram (no
+// device-scope atomics): pass one writes the chunk's count per block, an exclusive scan over
+// (block, chunk) gives each chunk its offset inside every block's segment, and pass two writes each
+// point's 32-byte record there.  Cells are those of bracket_linear_d / bracket_phi_d; the gather repeats
+// the weight expressions of sample_aos_d on the cell and the same lerps on the same values, so its
+// results equal the unsorted gather's bit for bit.
 
-constexpr int kScanBlock = 1024;   // threads of the scan kernels: 16 waves of 64
+constexpr int kScanBlock = 1024;      // threads of the scan kernels: 16 waves of 64
+constexpr int kPartThreads = 1024;    // threads of the two partition passes
+constexpr int kMaxBlocks = 8192;      // blocks of one partition; the LDS histogram holds kMaxBlocks + 1
+constexpr int kBlockThreads = 256;    // threads of the block gather
+constexpr int kOutsideGroups = 64;    // workgroups that write the NaN rows of points outside the r/theta range
+constexpr int kMaxRegionBytes = 64 * 1024;
+
+struct BlockGeom {
+    int br, bt, bp;      // cells per block in r, theta, phi (powers of two)
+    int lbr, lbt, lbp;   // their base-2 logarithms
+    int nbr, nbt, nbp;   // blocks per dimension
+    int nblocks;         // nbr nbt nbp; block nblocks holds the points outside the r/theta range
+    int split;           // workgroups per block in the gather
+};
 
 // Inclusive prefix sum over the 1024 threads of a block; lds holds 16 words.
 __device__ __forceinline__ unsigned block_inclusive_scan(unsigned v, unsigned *lds) {
@@ -248,51 +266,7 @@ __device__ __forceinline__ unsigned block_inclusive_scan(unsigned v, unsigned *l
     return v;
 }
 
-// Cell of each point (cells in the brick's memory order, phi fastest; points outside the r/theta range go
-// to the extra cell n_cells) and its rank among the points of that cell.
-__global__ __launch_bounds__(256) void bucket_count_kernel(const double *__restrict__ rg, int nr,
-                                                           const double *__restrict__ tg, int nt,
-                                                           const double *__restrict__ pg, int np,
-                                                           const double *__restrict__ pts, int64_t n,
-                                                           unsigned *__restrict__ key, unsigned *__restrict__ rank,
-                                                           unsigned *__restrict__ count) {
-    __shared__ double sgrid[kGatherGrid];
-    const bool staged = nr + nt + np <= kGatherGrid;
-    if (staged) {
-        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
-            sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
-        __syncthreads();
-    }
-    const double *r_ = staged ? sgrid : rg;
-    const double *t_ = staged ? sgrid + nr : tg;
-    const double *p_ = staged ? sgrid + nr + nt : pg;
-    const unsigned n_cells = unsigned(nr - 1) * unsigned(nt - 1) * unsigned(np);
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        const double *p = pts + i * 3;
-        int r0 = 0, t0 = 0, p0 = 0, p1 = 0;
-        double w = 0.0;
-        unsigned c = n_cells;
-        if (bracket_linear_d(r_, nr, p[0], r0, w) && bracket_linear_d(t_, nt, p[1], t0, w)) {
-            bracket_phi_d(p_, np, p[2], p0, p1, w);
-            c = (unsigned(r0) * unsigned(nt - 1) + unsigned(t0)) * unsigned(np) + unsigned(p0);
-        }
-        key[i] = c;
-        rank[i] = atomicAdd(&count[c], 1u);
-    }
-}
-
-// Exclusive scan of a[0, n) in chunks of 1024; the chunk totals go to sums.
-__global__ __launch_bounds__(kScanBlock) void scan_chunks_kernel(unsigned *__restrict__ a, unsigned n,
-                                                                 unsigned *__restrict__ sums) {
-    __shared__ unsigned lds[16];
-    const unsigned i = blockIdx.x * kScanBlock + threadIdx.x;
-    const unsigned v = i < n ? a[i] : 0u;
-    const unsigned inc = block_inclusive_scan(v, lds);
-    if (i < n) a[i] = inc - v;
-    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = inc;
-}
-
-// Exclusive scan of the chunk totals, in place, by one block.
+// Exclusive scan of sums[0, nb), in place, by one block.
 __global__ __launch_bounds__(kScanBlock) void scan_sums_kernel(unsigned *__restrict__ sums, unsigned nb) {
     __shared__ unsigned lds[16];
     __shared__ unsigned carry;
@@ -310,112 +284,314 @@ __global__ __launch_bounds__(kScanBlock) void scan_sums_kernel(unsigned *__restr
     }
 }
 
-__global__ __launch_bounds__(kScanBlock) void scan_add_kernel(unsigned *__restrict__ a, unsigned n,
-                                                              const unsigned *__restrict__ sums) {
-    const unsigned i = blockIdx.x * kScanBlock + threadIdx.x;
-    if (i < n) a[i] += sums[blockIdx.x];
+// Per-block offsets of the partition: h[c][b] (chunk-major, nb1 blocks per chunk row) holds chunk c's count
+// of block b and becomes the exclusive prefix over chunks, sum over c' < c of h[c'][b]; tot[b] receives the
+// block's total.  A workgroup takes 64 blocks; its 16 waves each sum a sixteenth of the chunks, an LDS pass
+// turns those sums into the waves' starting offsets, and each wave rewrites its rows.
+__global__ __launch_bounds__(1024) void column_scan_kernel(unsigned *__restrict__ h, int chunks, int nb1,
+                                                           unsigned *__restrict__ tot) {
+    __shared__ unsigned part[16][64];
+    const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+    const int b = int(blockIdx.x) * 64 + lane;
+    const int per = (chunks + 15) / 16;
+    const int c0 = seg * per < chunks ? seg * per : chunks;
+    const int c1 = c0 + per < chunks ? c0 + per : chunks;
+    unsigned s = 0u;
+    if (b < nb1) {
+#pragma unroll 8
+        for (int c = c0; c < c1; ++c) s += h[size_t(c) * nb1 + b];
+    }
+    part[seg][lane] = s;
+    __syncthreads();
+    unsigned pre = 0u;
+    for (int k = 0; k < seg; ++k) pre += part[k][lane];
+    if (b < nb1) {
+        for (int c = c0; c < c1; ++c) {
+            const unsigned v = h[size_t(c) * nb1 + b];
+            h[size_t(c) * nb1 + b] = pre;
+            pre += v;
+        }
+        if (seg == 15) tot[b] = pre;
+    }
 }
 
 struct alignas(16) SortedPoint {
     double r, th, ph;
     unsigned idx;    // position of the point in the caller's arrays
-    unsigned cell;   // (r0 (nt - 1) + t0) np + p0, or the outside cell
+    unsigned cell;   // the cell inside its block: r0 % br | (t0 % bt) << 8 | (p0 % bp) << 16; ~0u outside
 };
 
-__global__ __launch_bounds__(256) void bucket_scatter_kernel(const double *__restrict__ pts,
-                                                             const unsigned *__restrict__ key,
-                                                             const unsigned *__restrict__ rank,
-                                                             const unsigned *__restrict__ offset, int64_t n,
-                                                             SortedPoint *__restrict__ sorted) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        const unsigned c = key[i];
-        const double *p = pts + i * 3;
-        sorted[offset[c] + rank[i]] = SortedPoint{p[0], p[1], p[2], unsigned(i), c};
+// Cell lookup of the partition passes: searchsorted-right - 1 from a guess.  Each grid gets a table of
+// kGuessBins + 1 entries over its range ([g0, g_last], and [g0, g0 + 2 pi] for the wrapped phi), entry k the
+// exact index of the bin's left edge; a point starts from its bin's entry and walks to the index where
+// g[i] <= x < g[i + 1], which is searchsorted-right - 1 whatever the start, so the rounding of the bin only
+// changes the length of the walk (a step or two for grids whose cells do not vary a lot in width).
+constexpr int kGuessBins = 512;
+
+struct Guess {
+    const double *g;
+    int n;
+    double lo, scale;   // bin of x: (x - lo) * scale
+    const int *t;       // kGuessBins + 1 start indices
+};
+
+__device__ __forceinline__ int guess_search(const Guess &q, double x) {
+    const double u = (x - q.lo) * q.scale;
+    int i = q.t[u > 0.0 ? (u < double(kGuessBins) ? int(u) : kGuessBins) : 0];
+    while (i + 1 < q.n && q.g[i + 1] <= x) ++i;
+    while (i > 0 && q.g[i] > x) --i;
+    return i;
+}
+
+// Grids of up to kGatherGrid values in all are staged in LDS, and the three guess tables built in LDS
+// (tab holds 3 (kGuessBins + 1) ints).
+__device__ __forceinline__ void stage_guesses(const double *rg, int nr, const double *tg, int nt, const double *pg,
+                                              int np, double *sgrid, int *tab, Guess q[3]) {
+    const bool staged = nr + nt + np <= kGatherGrid;
+    if (staged)
+        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
+            sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
+    q[0] = Guess{staged ? sgrid : rg, nr, rg[0], double(kGuessBins) / (rg[nr - 1] - rg[0]), tab};
+    q[1] = Guess{staged ? sgrid + nr : tg, nt, tg[0], double(kGuessBins) / (tg[nt - 1] - tg[0]), tab + kGuessBins + 1};
+    q[2] = Guess{staged ? sgrid + nr + nt : pg, np, pg[0], double(kGuessBins) / kTwoPi, tab + 2 * (kGuessBins + 1)};
+    for (int e = threadIdx.x; e < 3 * (kGuessBins + 1); e += blockDim.x) {
+        const int w = e / (kGuessBins + 1), k = e - w * (kGuessBins + 1);
+        const double *g = w == 0 ? rg : (w == 1 ? tg : pg);
+        const int n = w == 0 ? nr : (w == 1 ? nt : np);
+        const double edge = q[w].lo + double(k) / q[w].scale;
+        const int i = search_right_d(g, n, edge) - 1;
+        tab[e] = i < 0 ? 0 : i;
+    }
+    __syncthreads();
+}
+
+// Block of a point and its cell inside the block, the cell being the one bracket_linear_d / bracket_phi_d
+// find (the r and theta index clamped to [0, n - 2], so x == g[n - 1] lands in the last cell; the phi index of
+// the wrapped angle, n - 1 when out of [0, n)); false (block nblocks) outside the r/theta range or for a NaN
+// coordinate.
+__device__ __forceinline__ bool locate_point(const Guess q[3], const BlockGeom &g, double r, double th, double ph,
+                                             unsigned &cell, unsigned &blk) {
+    if (!(r >= q[0].g[0] && r <= q[0].g[q[0].n - 1] && th >= q[1].g[0] && th <= q[1].g[q[1].n - 1])) {
+        cell = ~0u;
+        blk = unsigned(g.nblocks);
+        return false;
+    }
+    int r0 = guess_search(q[0], r), t0 = guess_search(q[1], th);
+    r0 = r0 > q[0].n - 2 ? q[0].n - 2 : r0;
+    t0 = t0 > q[1].n - 2 ? q[1].n - 2 : t0;
+    int p0 = guess_search(q[2], wrap_phi_d(ph, q[2].g[0]));
+    p0 = (p0 < 0 || p0 >= q[2].n) ? q[2].n - 1 : p0;
+    cell = unsigned(r0 & (g.br - 1)) | unsigned(t0 & (g.bt - 1)) << 8 | unsigned(p0 & (g.bp - 1)) << 16;
+    blk = (unsigned(r0 >> g.lbr) * unsigned(g.nbt) + unsigned(t0 >> g.lbt)) * unsigned(g.nbp) + unsigned(p0 >> g.lbp);
+    return true;
+}
+
+// Pass one: workgroup c counts the points [c chunk, (c + 1) chunk) per block into hist[c][b].
+__global__ __launch_bounds__(kPartThreads) void part_count_kernel(const double *__restrict__ rg, int nr,
+                                                                  const double *__restrict__ tg, int nt,
+                                                                  const double *__restrict__ pg, int np,
+                                                                  const double *__restrict__ pts, int64_t n,
+                                                                  int64_t chunk, BlockGeom g,
+                                                                  unsigned *__restrict__ hist) {
+    __shared__ double sgrid[kGatherGrid];
+    __shared__ int tab[3 * (kGuessBins + 1)];
+    __shared__ unsigned lh[kMaxBlocks + 1];
+    for (int j = threadIdx.x; j <= g.nblocks; j += blockDim.x) lh[j] = 0u;
+    Guess q[3];
+    stage_guesses(rg, nr, tg, nt, pg, np, sgrid, tab, q);
+    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    constexpr int kAhead = 4;   // points per thread whose coordinates load together
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kAhead * int64_t(blockDim.x)) {
+        double c[kAhead][3];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const int64_t i = i0 + u * int64_t(blockDim.x);
+            const double *p = pts + (i < hi ? i : i0) * 3;
+            c[u][0] = p[0];
+            c[u][1] = p[1];
+            c[u][2] = p[2];
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            unsigned cell, blk;
+            locate_point(q, g, c[u][0], c[u][1], c[u][2], cell, blk);
+            if (i0 + u * int64_t(blockDim.x) < hi) atomicAdd(&lh[blk], 1u);
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j <= g.nblocks; j += blockDim.x) hist[size_t(blockIdx.x) * (g.nblocks + 1) + j] = lh[j];
+}
+
+// Pass two: the same chunks; each point's record goes to its block's segment (start[b]) at the chunk's offset
+// inside it (offs[c][b], the column-scanned histogram) plus its rank among the chunk's points of that block,
+// and inv[i] records where point i went.
+__global__ __launch_bounds__(kPartThreads) void part_scatter_kernel(const double *__restrict__ rg, int nr,
+                                                                    const double *__restrict__ tg, int nt,
+                                                                    const double *__restrict__ pg, int np,
+                                                                    const double *__restrict__ pts, int64_t n,
+                                                                    int64_t chunk, BlockGeom g,
+                                                                    const unsigned *__restrict__ offs,
+                                                                    const unsigned *__restrict__ start,
+                                                                    SortedPoint *__restrict__ sorted,
+                                                                    unsigned *__restrict__ inv) {
+    __shared__ double sgrid[kGatherGrid];
+    __shared__ int tab[3 * (kGuessBins + 1)];
+    __shared__ unsigned pos[kMaxBlocks + 1];
+    for (int j = threadIdx.x; j <= g.nblocks; j += blockDim.x)
+        pos[j] = start[j] + offs[size_t(blockIdx.x) * (g.nblocks + 1) + j];
+    Guess q[3];
+    stage_guesses(rg, nr, tg, nt, pg, np, sgrid, tab, q);
+    const int64_t lo = int64_t(blockIdx.x) * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    constexpr int kAhead = 4;   // points per thread whose coordinates load together
+    for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += kAhead * int64_t(blockDim.x)) {
+        double c[kAhead][3];
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const int64_t i = i0 + u * int64_t(blockDim.x);
+            const double *p = pts + (i < hi ? i : i0) * 3;
+            c[u][0] = p[0];
+            c[u][1] = p[1];
+            c[u][2] = p[2];
+        }
+#pragma unroll
+        for (int u = 0; u < kAhead; ++u) {
+            const int64_t i = i0 + u * int64_t(blockDim.x);
+            unsigned cell, blk;
+            locate_point(q, g, c[u][0], c[u][1], c[u][2], cell, blk);
+            if (i < hi) {
+                const unsigned slot = atomicAdd(&pos[blk], 1u);
+                sorted[slot] = SortedPoint{c[u][0], c[u][1], c[u][2], unsigned(i), cell};
+                inv[i] = slot;
+            }
+        }
     }
 }
 
-// One lane per point, the points in cell order; the weights repeat bracket_linear_d / bracket_phi_d on the
-// cell the sort found, and the corners load straight from the brick.  A wave's 64 x 11 results pass
-// through LDS so that the stores go out point by point: consecutive lanes write consecutive channels of one
-// point, each point's 88 bytes in one piece, instead of every lane scattering 11 doubles.
-__global__ __launch_bounds__(256) void sorted_gather_kernel(const double *__restrict__ coeffs,
-                                                            const double *__restrict__ rg, int nr,
-                                                            const double *__restrict__ tg, int nt,
-                                                            const double *__restrict__ pg, int np,
-                                                            const SortedPoint *__restrict__ sorted, int64_t n,
-                                                            double *__restrict__ out) {
-    __shared__ double sgrid[kGatherGrid];
-    __shared__ double res_lds[4][64 * kCoeffs];
-    __shared__ unsigned idx_lds[4][64];
-    const bool staged = nr + nt + np <= kGatherGrid;
-    if (staged) {
-        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
-            sgrid[j] = j < nr ? rg[j] : (j < nr + nt ? tg[j - nr] : pg[j - nr - nt]);
+// Last pass: out[i] = res[inv[i]], the 11 results of each point read from where the gather wrote them in
+// block order; lanes on consecutive doubles of the output, so the stores are whole lines and the random
+// accesses are reads (random 88-byte reads cost a little less than random 88-byte writes here, 0.78 vs
+// ~0.85 ms per 2^24 points, and leave the gather's stores whole lines).
+__global__ __launch_bounds__(256) void unpermute_kernel(const double *__restrict__ res, const unsigned *__restrict__ inv,
+                                                        int64_t n, double *__restrict__ out) {
+    const int64_t total = n * kCoeffs;
+    for (int64_t f = int64_t(blockIdx.x) * 256 + threadIdx.x; f < total; f += int64_t(gridDim.x) * 256) {
+        const int64_t i = f / kCoeffs;
+        out[f] = res[size_t(inv[i]) * kCoeffs + size_t(f - i * kCoeffs)];
+    }
+}
+
+// The gather.  Workgroup w < nblocks * split serves the (w % split)-th part of block w / split: it copies
+// the block's corner nodes ([r][theta][phi][11] doubles, phi wrapping at the seam) and grid values into LDS,
+// then each lane takes one point of the segment at a time, forms the weights from the point's cell as
+// sample_aos_d does and interpolates the 11 channels from LDS.  The last kOutsideGroups workgroups write the
+// NaN rows of the points outside the r/theta range.  Results are written in block order (res[j] for the
+// point in sorted[j]); a wave's 64 x 11 values pass through LDS so that its stores cover whole lines.
+__global__ __launch_bounds__(kBlockThreads) void block_gather_kernel(const double *__restrict__ coeffs,
+                                                                     const double *__restrict__ rg, int nr,
+                                                                     const double *__restrict__ tg, int nt,
+                                                                     const double *__restrict__ pg, int np,
+                                                                     const SortedPoint *__restrict__ sorted,
+                                                                     const unsigned *__restrict__ start,
+                                                                     int64_t n, BlockGeom g,
+                                                                     double *__restrict__ out) {
+    extern __shared__ double lds[];   // 4 waves x 64 x 11 results, then the block's corner nodes
+    __shared__ double gloc[64];       // r nodes | theta nodes | phi nodes of the block
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double *res = lds + wave * 64 * kCoeffs;
+    double *node = lds + (kBlockThreads / 64) * 64 * kCoeffs;
+    const int groups = g.nblocks * g.split;
+    const bool inside = int(blockIdx.x) < groups;
+    const int b = inside ? int(blockIdx.x) / g.split : g.nblocks;
+    const int part = inside ? int(blockIdx.x) % g.split : int(blockIdx.x) - groups;
+    const int parts = inside ? g.split : kOutsideGroups;
+    const int64_t seg_lo = start[b];
+    const int64_t seg_hi = b < g.nblocks ? int64_t(start[b + 1]) : n;
+    const int64_t len = seg_hi - seg_lo;
+    const int64_t lo = seg_lo + len * part / parts, hi = seg_lo + len * (part + 1) / parts;
+    if (lo >= hi) return;   // workgroup-uniform
+    int r_lo = 0, t_lo = 0, p_lo = 0, nrr = 0, ntt = 0, npp = 0;
+    double r_last = 0.0, t_last = 0.0, p_first = 0.0;
+    if (inside) {
+        const int pb = b % g.nbp, tb = (b / g.nbp) % g.nbt, rb = b / (g.nbp * g.nbt);
+        r_lo = rb * g.br;
+        t_lo = tb * g.bt;
+        p_lo = pb * g.bp;
+        nrr = (g.br < nr - 1 - r_lo ? g.br : nr - 1 - r_lo) + 1;
+        ntt = (g.bt < nt - 1 - t_lo ? g.bt : nt - 1 - t_lo) + 1;
+        npp = (g.bp < np - p_lo ? g.bp : np - p_lo) + 1;
+        const int vals = nrr * ntt * npp * kCoeffs;
+        for (int e = threadIdx.x; e < vals; e += blockDim.x) {
+            const int ch = e % kCoeffs, q = e / kCoeffs;
+            const int k = q % npp, j = (q / npp) % ntt, i = q / (npp * ntt);
+            int pk = p_lo + k;
+            pk = pk >= np ? pk - np : pk;
+            node[e] = coeffs[((size_t(r_lo + i) * nt + (t_lo + j)) * np + pk) * kCoeffs + ch];
+        }
+        for (int e = threadIdx.x; e < nrr + ntt + npp; e += blockDim.x) {
+            double v;
+            if (e < nrr) v = rg[r_lo + e];
+            else if (e < nrr + ntt) v = tg[t_lo + e - nrr];
+            else {
+                int pk = p_lo + e - nrr - ntt;
+                v = pk >= np ? pg[0] + kTwoPi : pg[pk];   // the seam cell's upper node, as bracket_phi_d's hi
+            }
+            gloc[e] = v;
+        }
+        r_last = rg[nr - 1];
+        t_last = tg[nt - 1];
+        p_first = pg[0];
         __syncthreads();
     }
-    const double *r_ = staged ? sgrid : rg;
-    const double *t_ = staged ? sgrid + nr : tg;
-    const double *p_ = staged ? sgrid + nr + nt : pg;
-    const unsigned n_cells = unsigned(nr - 1) * unsigned(nt - 1) * unsigned(np);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double *res = res_lds[wave];
-    unsigned *idx = idx_lds[wave];
-    // the wave's 64 points per round: a wave-uniform loop, so every lane takes part in the LDS exchange
-    for (int64_t base = (int64_t(blockIdx.x) * 4 + wave) * 64; base < n; base += int64_t(gridDim.x) * 256) {
+    const double *rl = gloc, *tl = gloc + nrr, *pl = gloc + nrr + ntt;
+    const int sr = ntt * npp * kCoeffs, st = npp * kCoeffs;
+    // wave-uniform rounds of 64 points, so that every lane takes part in the LDS exchange
+    for (int64_t base = lo + wave * 64; base < hi; base += kBlockThreads) {
         const int64_t j = base + lane;
-        unsigned my_idx = ~0u;
-        if (j < n) {
+        if (j < hi) {
             const SortedPoint sp = sorted[j];
-            my_idx = sp.idx;
-            if (sp.cell >= n_cells) {
+            if (!inside) {
 #pragma unroll
                 for (int k = 0; k < kCoeffs; ++k) res[lane * kCoeffs + k] = __builtin_nan("");
             } else {
-                const int p0 = int(sp.cell % unsigned(np));
-                const unsigned rt = sp.cell / unsigned(np);
-                const int t0 = int(rt % unsigned(nt - 1)), r0 = int(rt / unsigned(nt - 1));
+                const int ri = int(sp.cell & 255u), ti = int((sp.cell >> 8) & 255u), pi = int(sp.cell >> 16);
                 double wr, wt;
-                if (sp.r == r_[nr - 1]) {
+                if (sp.r == r_last) {
                     wr = 1.0;
                 } else {
-                    double den = r_[r0 + 1] - r_[r0];
+                    double den = rl[ri + 1] - rl[ri];
                     den = (1.0e-300 > den) ? 1.0e-300 : den;
-                    wr = (sp.r - r_[r0]) / den;
+                    wr = (sp.r - rl[ri]) / den;
                 }
-                if (sp.th == t_[nt - 1]) {
+                if (sp.th == t_last) {
                     wt = 1.0;
                 } else {
-                    double den = t_[t0 + 1] - t_[t0];
+                    double den = tl[ti + 1] - tl[ti];
                     den = (1.0e-300 > den) ? 1.0e-300 : den;
-                    wt = (sp.th - t_[t0]) / den;
+                    wt = (sp.th - tl[ti]) / den;
                 }
-                const double p = wrap_phi_d(sp.ph, p_[0]);
-                const int p1 = (p0 + 1) % np;
-                const double hi = (p1 > p0) ? p_[p1] : p_[0] + kTwoPi;
-                const double pp = (p >= p_[p0]) ? p : p + kTwoPi;
-                double den = hi - p_[p0];
+                const double p = wrap_phi_d(sp.ph, p_first);
+                const double hi_p = pl[pi + 1];   // p_[p1], or p_[0] + 2 pi across the seam
+                const double pp = (p >= pl[pi]) ? p : p + kTwoPi;
+                double den = hi_p - pl[pi];
                 den = (1.0e-300 > den) ? 1.0e-300 : den;
-                const double wp = (pp - p_[p0]) / den;
-                const size_t row00 = (size_t(r0) * nt + t0) * np, row01 = row00 + np;
-                const size_t row10 = row00 + size_t(nt) * np, row11 = row10 + np;
-                const double *c000 = coeffs + (row00 + p0) * kCoeffs, *c001 = coeffs + (row00 + p1) * kCoeffs;
-                const double *c010 = coeffs + (row01 + p0) * kCoeffs, *c011 = coeffs + (row01 + p1) * kCoeffs;
-                const double *c100 = coeffs + (row10 + p0) * kCoeffs, *c101 = coeffs + (row10 + p1) * kCoeffs;
-                const double *c110 = coeffs + (row11 + p0) * kCoeffs, *c111 = coeffs + (row11 + p1) * kCoeffs;
+                const double wp = (pp - pl[pi]) / den;
+                const double *c000 = node + ri * sr + ti * st + pi * kCoeffs;
+                const double *c001 = c000 + kCoeffs, *c010 = c000 + st, *c011 = c010 + kCoeffs;
+                const double *c100 = c000 + sr, *c101 = c100 + kCoeffs, *c110 = c100 + st, *c111 = c110 + kCoeffs;
 #pragma unroll
                 for (int k = 0; k < kCoeffs; ++k)
                     res[lane * kCoeffs + k] =
                         trilerp_d(c000[k], c001[k], c010[k], c011[k], c100[k], c101[k], c110[k], c111[k], wr, wt, wp);
             }
         }
-        idx[lane] = my_idx;
         __builtin_amdgcn_wave_barrier();
+        const int64_t lim = (hi - base) * kCoeffs;   // values of the round's points that exist
 #pragma unroll
         for (int q = 0; q < kCoeffs; ++q) {   // value f of the wave's 64 x 11: point f / 11, channel f % 11
             const int f = q * 64 + lane;
-            const int pt = f / kCoeffs;
-            const unsigned ii = idx[pt];
-            if (ii != ~0u) out[size_t(ii) * kCoeffs + (f - pt * kCoeffs)] = res[f];
+            if (f < lim) out[size_t(base) * kCoeffs + f] = res[f];
         }
         __builtin_amdgcn_wave_barrier();   // the next round rewrites this wave's LDS
     }
@@ -525,18 +701,51 @@ struct Stage {
 
 static size_t rounded(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
-// The bucketed order pays for its sort when the points outnumber the cells (each record then serves
-// several points) and there are enough of them to fill the chip.
-constexpr int64_t kBucketMinPoints = int64_t(1) << 16;
+// Blocks of the partition: the first of these shapes with at most kMaxBlocks blocks whose corner nodes fit
+// kMaxRegionBytes; false when none does (the call then gathers unsorted).
+static bool choose_blocks(int nr, int nt, int np, int64_t n, BlockGeom &g) {
+    static const int shapes[][3] = {{4, 4, 8}, {4, 8, 8}, {8, 8, 8}};
+    for (const auto &s : shapes) {
+        g.br = s[0];
+        g.bt = s[1];
+        g.bp = s[2];
+        g.lbr = __builtin_ctz(unsigned(g.br));
+        g.lbt = __builtin_ctz(unsigned(g.bt));
+        g.lbp = __builtin_ctz(unsigned(g.bp));
+        g.nbr = (nr - 1 + g.br - 1) / g.br;
+        g.nbt = (nt - 1 + g.bt - 1) / g.bt;
+        g.nbp = (np + g.bp - 1) / g.bp;
+        const int64_t nb = int64_t(g.nbr) * g.nbt * g.nbp;
+        const int64_t region = int64_t(g.br + 1) * (g.bt + 1) * (g.bp + 1) * kCoeffs * 8;
+        if (nb <= kMaxBlocks && region <= kMaxRegionBytes) {
+            g.nblocks = int(nb);
+            // a workgroup per ~2048 points of a block on average
+            g.split = int(std::min<int64_t>(64, std::max<int64_t>(1, n / nb / 2048)));
+            return true;
+        }
+    }
+    return false;
+}
 
-// Bucketed large-N sample into device buffers; the workspace (cell counts and offsets, chunk sums, key and
-// rank of every point, the points in cell order) is the context's, grown on demand.
-static int sample_bucketed(bh_ctx *ctx, const double *d_coeffs, const double *d_r, int nr, const double *d_t, int nt,
-                           const double *d_p, int np, const double *d_points, int64_t n, double *d_out) {
-    const size_t n_cells = size_t(nr - 1) * size_t(nt - 1) * size_t(np) + 1;   // + the cell of outside points
-    const size_t nb = (n_cells + kScanBlock - 1) / kScanBlock;
-    const size_t need = rounded(n_cells * 4) + rounded(nb * 4) + 2 * rounded(size_t(n) * 4) +
-                        rounded(size_t(n) * sizeof(SortedPoint));
+// The partition pays when the points outnumber the blocks well (a block's corner nodes, ~20 KB, are read
+// once for all its points) and there are enough of them to fill the chip.
+constexpr int64_t kBucketMinPoints = int64_t(1) << 16;
+constexpr int64_t kMinPointsPerBlock = 32;
+
+// Partitioned large-N sample into device buffers; the workspace (histogram and offsets per block and
+// chunk, segment starts, the points and their results in block order, each point's place in that order) is
+// the context's, grown on demand.
+static int sample_partitioned(bh_ctx *ctx, const BlockGeom &g, const double *d_coeffs, const double *d_r, int nr,
+                              const double *d_t, int nt, const double *d_p, int np, const double *d_points,
+                              int64_t n, double *d_out) {
+    const int cus = ctx->cus > 0 ? ctx->cus : 256;
+    // two workgroups of the partition passes per CU (their LDS, 55 KB each, allows it)
+    const int chunks = int(std::max<int64_t>(1, std::min<int64_t>(2 * cus, (n + 16383) / 16384)));
+    const int64_t chunk = (n + chunks - 1) / chunks;
+    const int nb1 = g.nblocks + 1;
+    const size_t total = size_t(nb1) * size_t(chunks);
+    const size_t need = rounded(total * 4) + rounded(size_t(nb1) * 4) + rounded(size_t(n) * sizeof(SortedPoint)) +
+                        rounded(size_t(n) * 4) + rounded(size_t(n) * kCoeffs * 8);
     if (need > ctx->sampler_work_cap) {
         BH_HIP(hipStreamSynchronize(ctx->stream));
         if (ctx->d_sampler_work) (void)hipFree(ctx->d_sampler_work);
@@ -547,25 +756,24 @@ static int sample_bucketed(bh_ctx *ctx, const double *d_coeffs, const double *d_
         ctx->sampler_work_cap = need;
     }
     char *w = static_cast<char *>(ctx->d_sampler_work);
-    unsigned *count = reinterpret_cast<unsigned *>(w);
-    unsigned *sums = reinterpret_cast<unsigned *>(w + rounded(n_cells * 4));
-    unsigned *key = reinterpret_cast<unsigned *>(w + rounded(n_cells * 4) + rounded(nb * 4));
-    unsigned *rank = key + rounded(size_t(n) * 4) / 4;
-    SortedPoint *sorted = reinterpret_cast<SortedPoint *>(rank + rounded(size_t(n) * 4) / 4);
-    int st;
-    if ((st = fill_u32(ctx, count, 0u, int(n_cells)))) return st;
-    const int64_t cap = int64_t(ctx->cus > 0 ? ctx->cus : 256) * 16;
-    const unsigned pblocks = unsigned(std::min<int64_t>((n + 255) / 256, cap));
-    hipLaunchKernelGGL(bucket_count_kernel, dim3(pblocks), dim3(256), 0, ctx->stream, d_r, nr, d_t, nt, d_p, np, d_points, n,
-                       key, rank, count);
-    hipLaunchKernelGGL(scan_chunks_kernel, dim3(unsigned(nb)), dim3(kScanBlock), 0, ctx->stream, count, unsigned(n_cells),
-                       sums);
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, ctx->stream, sums, unsigned(nb));
-    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(nb)), dim3(kScanBlock), 0, ctx->stream, count, unsigned(n_cells), sums);
-    hipLaunchKernelGGL(bucket_scatter_kernel, dim3(pblocks), dim3(256), 0, ctx->stream, d_points, key, rank, count, n,
-                       sorted);
-    hipLaunchKernelGGL(sorted_gather_kernel, dim3(pblocks), dim3(256), 0, ctx->stream, d_coeffs, d_r, nr, d_t, nt, d_p, np,
-                       sorted, n, d_out);
+    unsigned *hist = reinterpret_cast<unsigned *>(w);
+    unsigned *start = reinterpret_cast<unsigned *>(w + rounded(total * 4));
+    SortedPoint *sorted = reinterpret_cast<SortedPoint *>(w + rounded(total * 4) + rounded(size_t(nb1) * 4));
+    unsigned *inv = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(sorted) + rounded(size_t(n) * sizeof(SortedPoint)));
+    double *res = reinterpret_cast<double *>(reinterpret_cast<char *>(inv) + rounded(size_t(n) * 4));
+    hipLaunchKernelGGL(part_count_kernel, dim3(unsigned(chunks)), dim3(kPartThreads), 0, ctx->stream, d_r, nr, d_t, nt,
+                       d_p, np, d_points, n, chunk, g, hist);
+    hipLaunchKernelGGL(column_scan_kernel, dim3(unsigned((nb1 + 63) / 64)), dim3(1024), 0, ctx->stream, hist, chunks, nb1,
+                       start);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanBlock), 0, ctx->stream, start, unsigned(nb1));
+    hipLaunchKernelGGL(part_scatter_kernel, dim3(unsigned(chunks)), dim3(kPartThreads), 0, ctx->stream, d_r, nr, d_t, nt,
+                       d_p, np, d_points, n, chunk, g, hist, start, sorted, inv);
+    const size_t region = size_t(g.br + 1) * (g.bt + 1) * (g.bp + 1) * kCoeffs * 8;
+    const size_t shmem = size_t(kBlockThreads / 64) * 64 * kCoeffs * 8 + region;
+    const unsigned groups = unsigned(g.nblocks * g.split + kOutsideGroups);
+    hipLaunchKernelGGL(block_gather_kernel, dim3(groups), dim3(kBlockThreads), shmem, ctx->stream, d_coeffs, d_r, nr, d_t,
+                       nt, d_p, np, sorted, start, n, g, res);
+    hipLaunchKernelGGL(unpermute_kernel, dim3(unsigned(cus * 16)), dim3(256), 0, ctx->stream, res, inv, n, d_out);
     BH_HIP(hipGetLastError());
     return BH_OK;
 }
@@ -733,12 +941,14 @@ int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs, const 
     if (nr < 2 || nt < 2 || np < 2) return set_error(BH_ERR_VALUE, "grids must contain at least 2 values");
     if (n < 0) return set_error(BH_ERR_VALUE, "n must be non-negative");
     if (n == 0) return BH_OK;
-    const int64_t n_cells = int64_t(nr - 1) * (nt - 1) * np + 1;
-    const bool bucketed = n >= kBucketMinPoints && n >= n_cells && n < (int64_t(1) << 32) &&
-                          n_cells < (int64_t(1) << 31);
+    const int64_t n_cells = int64_t(nr - 1) * (nt - 1) * np;
+    BlockGeom g{};
+    const bool partitioned = n >= kBucketMinPoints && n < (int64_t(1) << 32) && n_cells < (int64_t(1) << 31) &&
+                             choose_blocks(nr, nt, np, n, g) && n >= kMinPointsPerBlock * g.nblocks;
     begin_timing(ctx);
-    if (bucketed) {
-        if ((st = sample_bucketed(ctx, d_coeffs, d_r, nr, d_theta, nt, d_phi, np, d_points, n, d_out))) return st;
+    if (partitioned) {
+        if ((st = sample_partitioned(ctx, g, d_coeffs, d_r, nr, d_theta, nt, d_phi, np, d_points, n, d_out)))
+            return st;
     } else {
         hipLaunchKernelGGL(sample_gather_kernel, dim3(gather_blocks(ctx, n)), dim3(256), 0, ctx->stream, d_coeffs, d_r,
                            int(nr), d_theta, int(nt), d_phi, int(np), d_points, n, d_out);

@@ -152,8 +152,8 @@ def _device_sample(renderer, coeffs, grids, pts):
 
 @pytest.mark.parametrize("n", [1 << 16, 300_001])
 def test_bucketed_sampler_equals_direct_gather(renderer, n):
-    """From 2^16 points (and at least one per cell) the device sampler sorts the points by brick cell before
-    gathering; each point's arithmetic is unchanged, so the result equals the unsorted gather (calls below
+    """From 2^16 points (and at least 32 per block of brick cells) the device sampler partitions the points by
+    block before gathering; each point's arithmetic is unchanged, so the result equals the unsorted gather (calls below
     the threshold) bit for bit, NaN rows outside the r/theta range included, and the oracle within the
     reference's 1e-12 gate."""
     print(f"bucketed sampler n={n}", flush=True)
@@ -173,5 +173,34 @@ def test_bucketed_sampler_equals_direct_gather(renderer, n):
     outside = np.isnan(got[:, 0])
     assert 0 < outside.mean() < 0.2
     pick = rng.choice(n, 5000, replace=False)
+    ref, _ = micro.sample_vec(coeffs, *grids, pts[pick, 0], pts[pick, 1], pts[pick, 2])
+    np.testing.assert_allclose(got[pick], ref, rtol=1e-12, atol=1e-12, equal_nan=True)
+
+
+def test_partitioned_sampler_on_the_scale_brick(renderer):
+    """The 128x64x128 brick (8192 blocks of 4x4x8 cells) at 2^20 points: grid values on every block face,
+    the last r and theta nodes, phi on the seam and beyond 2 pi, 5 % outside; equal to the unsorted gather bit
+    for bit and to the oracle within the reference's gate."""
+    n = 1 << 20
+    print(f"partitioned sampler, scale brick, n={n}", flush=True)
+    rng = np.random.default_rng(77)
+    br = scenes.scale_brick()
+    coeffs = br.coeffs.astype(np.float64)
+    grids = [g.astype(np.float64) for g in (br.r, br.theta, br.phi)]
+    pts = np.empty((n, 3))
+    span_r, span_t = grids[0][-1] - grids[0][0], grids[1][-1] - grids[1][0]
+    pts[:, 0] = rng.uniform(grids[0][0] - 0.025 * span_r, grids[0][-1] + 0.025 * span_r, n)
+    pts[:, 1] = rng.uniform(grids[1][0] - 0.025 * span_t, grids[1][-1] + 0.025 * span_t, n)
+    pts[:, 2] = rng.uniform(-2.0 * np.pi, 4.0 * np.pi, n)
+    k = 4096
+    pts[:k, 0] = grids[0][rng.integers(0, grids[0].size, k)]
+    pts[k:2 * k, 1] = grids[1][rng.integers(0, grids[1].size, k)]
+    pts[2 * k:3 * k, 2] = grids[2][rng.integers(0, grids[2].size, k)] + 2.0 * np.pi * rng.integers(-1, 2, k)
+    pts[3 * k:3 * k + 64, 2] = grids[2][-1]
+    got = _device_sample(renderer, coeffs, grids, pts)
+    direct = np.concatenate([_device_sample(renderer, coeffs, grids, pts[a:a + 60000]) for a in range(0, n, 60000)])
+    np.testing.assert_array_equal(got, direct)
+    assert 0.02 < np.isnan(got[:, 0]).mean() < 0.1
+    pick = np.concatenate([np.arange(3 * k + 64), rng.choice(n, 4000, replace=False)])
     ref, _ = micro.sample_vec(coeffs, *grids, pts[pick, 0], pts[pick, 1], pts[pick, 2])
     np.testing.assert_allclose(got[pick], ref, rtol=1e-12, atol=1e-12, equal_nan=True)
