@@ -76,6 +76,36 @@ bh_ctx *default_ctx(int *status) {
     return *status == BH_OK ? g_default : nullptr;
 }
 
+// Pinned, coherent host memory mapped into the device's address space: kernels of small host-buffer calls
+// read their inputs from it and write their outputs to it directly, so such a call is one launch and one
+// wait, with no copy-engine transfer.  Grow-only, like the device staging buffer.
+static void *g_host_staging = nullptr;
+static void *g_host_staging_dev = nullptr;
+static size_t g_host_staging_bytes = 0;
+
+int host_staging(size_t bytes, void **host, void **dev) {
+    if (bytes > g_host_staging_bytes) {
+        if (g_host_staging) (void)hipHostFree(g_host_staging);
+        g_host_staging = g_host_staging_dev = nullptr;
+        g_host_staging_bytes = 0;
+        size_t want = bytes < (size_t(1) << 18) ? (size_t(1) << 18) : bytes;
+        if (hipHostMalloc(&g_host_staging, want, hipHostMallocCoherent | hipHostMallocPortable | hipHostMallocMapped) !=
+            hipSuccess) {
+            g_host_staging = nullptr;
+            return set_error(BH_ERR_MEMORY, "pinned host staging allocation of %zu bytes failed", want);
+        }
+        if (hipHostGetDevicePointer(&g_host_staging_dev, g_host_staging, 0) != hipSuccess) {
+            (void)hipHostFree(g_host_staging);
+            g_host_staging = g_host_staging_dev = nullptr;
+            return set_error(BH_ERR_DEVICE, "pinned host staging is not mapped for the device");
+        }
+        g_host_staging_bytes = want;
+    }
+    *host = g_host_staging;
+    *dev = g_host_staging_dev;
+    return BH_OK;
+}
+
 int staging(size_t bytes, void **ptr) {
     if (bytes > g_staging_bytes) {
         if (g_staging) (void)hipFree(g_staging);

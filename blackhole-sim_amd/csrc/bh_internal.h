@@ -105,6 +105,8 @@ bh_ctx *default_ctx(int *status);
 std::mutex &default_ctx_mutex();
 // Grow-only device staging buffer owned by the default context.
 int staging(size_t bytes, void **ptr);
+// Grow-only pinned, coherent host buffer that kernels address directly (host and device addresses).
+int host_staging(size_t bytes, void **host, void **dev);
 // MLP inference enqueued on the MLP's context stream, without timing events:
 // d_x [rows][n_in] -> d_y [rows][n_out] or [n_out][rows] (transposed).
 int mlp_forward_enqueue(bh_mlp *m, const float *d_x, int64_t rows, float *d_y, int transposed);

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace bh {
 
@@ -597,11 +598,24 @@ __global__ __launch_bounds__(kBlockThreads) void block_gather_kernel(const doubl
     }
 }
 
-__global__ void sample_step_kernel(const double *__restrict__ coeffs, const double *__restrict__ rg, int nr,
-                                   const double *__restrict__ tg, int nt, const double *__restrict__ pg, int np,
-                                   const double *__restrict__ pts, int64_t n, double ds,
-                                   const double *__restrict__ init, int broadcast, int policy,
-                                   double *__restrict__ out) {
+// One lane per point; grids of up to kGatherGrid values in all are searched in LDS (the searches' dependent
+// reads then stay on the CU, also when the arrays are in pinned host memory).
+__global__ __launch_bounds__(256) void sample_step_kernel(const double *__restrict__ coeffs,
+                                                          const double *__restrict__ rg_in, int nr,
+                                                          const double *__restrict__ tg_in, int nt,
+                                                          const double *__restrict__ pg_in, int np,
+                                                          const double *__restrict__ pts, int64_t n, double ds,
+                                                          const double *__restrict__ init, int broadcast, int policy,
+                                                          double *__restrict__ out) {
+    __shared__ double sgrid[kGatherGrid];
+    const bool staged = nr + nt + np <= kGatherGrid;
+    if (staged) {
+        for (int j = threadIdx.x; j < nr + nt + np; j += blockDim.x)
+            sgrid[j] = j < nr ? rg_in[j] : (j < nr + nt ? tg_in[j - nr] : pg_in[j - nr - nt]);
+        __syncthreads();
+    }
+    const double *rg = staged ? sgrid : rg_in, *tg = staged ? sgrid + nr : tg_in;
+    const double *pg = staged ? sgrid + nr + nt : pg_in;
     int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *p = pts + i * 3;
@@ -682,24 +696,67 @@ static inline unsigned gather_blocks(bh_ctx *ctx, int64_t n) {
     return unsigned(want < cap ? (want > 0 ? want : 1) : cap);
 }
 
-// Copies host arrays into one staging allocation; returns device pointers in order.
+static size_t rounded(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+// Host-buffer calls whose arrays take up to this many bytes in all (the reference's micro-benchmark sizes:
+// an 8x6x8 brick and 64 points are 37 KB) run zero-copy: their kernels read the inputs from, and write
+// the outputs to, pinned host memory, so the call is one launch and one wait.  Larger calls copy through
+// device memory.
+constexpr size_t kZeroCopyBytes = size_t(256) << 10;
+
+// The host arrays of one call, laid out back to back (256-byte aligned) in a staging area: the pinned host
+// buffer (zero-copy, the arrays copied in by the CPU) or the device staging buffer (copied by the stream).
+// put() returns the device address of an array; fetch() brings an output back after the kernel and waits.
 struct Stage {
-    char *base = nullptr;
+    bh_ctx *ctx = nullptr;
+    char *dev = nullptr;    // device address of the area
+    char *host = nullptr;   // host address of the area when zero-copy, else null
     size_t off = 0;
     template <typename T>
-    T *put(bh_ctx *ctx, const T *src, size_t count, int *st) {
-        T *d = reinterpret_cast<T *>(base + off);
-        size_t bytes = count * sizeof(T);
-        off += (bytes + 255) & ~size_t(255);
+    T *put(const T *src, size_t count, int *st) {
+        T *d = reinterpret_cast<T *>(dev + off);
+        const size_t bytes = count * sizeof(T);
         if (count && src) {
-            hipError_t e = hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
-            if (e != hipSuccess) *st = set_error(BH_ERR_DEVICE, "hipMemcpyAsync failed: %s", hipGetErrorString(e));
+            if (host) {
+                memcpy(host + off, src, bytes);
+            } else {
+                hipError_t e = hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+                if (e != hipSuccess) *st = set_error(BH_ERR_DEVICE, "hipMemcpyAsync failed: %s", hipGetErrorString(e));
+            }
         }
+        off += rounded(bytes);
         return d;
+    }
+    int fetch(void *dst, const void *d, size_t bytes) {
+        if (host) {
+            BH_HIP(hipStreamSynchronize(ctx->stream));
+            memcpy(dst, host + (static_cast<const char *>(d) - dev), bytes);
+        } else {
+            BH_HIP(hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+            BH_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        return BH_OK;
     }
 };
 
-static size_t rounded(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+// Opens a staging area of `need` bytes for a call on the default context.
+static int stage_open(bh_ctx *ctx, size_t need, Stage &sg) {
+    sg.ctx = ctx;
+    sg.off = 0;
+    int st;
+    if (need <= kZeroCopyBytes) {
+        void *h = nullptr, *d = nullptr;
+        if ((st = host_staging(need, &h, &d))) return st;
+        sg.host = static_cast<char *>(h);
+        sg.dev = static_cast<char *>(d);
+        return BH_OK;
+    }
+    void *buf = nullptr;
+    if ((st = staging(need, &buf))) return st;
+    sg.host = nullptr;
+    sg.dev = static_cast<char *>(buf);
+    return BH_OK;
+}
 
 // Blocks of the partition: the first of these shapes with at most kMaxBlocks blocks whose corner nodes fit
 // kMaxRegionBytes; false when none does (the call then gathers unsorted).
@@ -802,23 +859,17 @@ int bh_stokes_rk2_brick(const double *coeffs, size_t n_coeffs, double ds_cm, con
     std::lock_guard<std::mutex> lock(default_ctx_mutex());
     bh_ctx *ctx = default_ctx(&st);
     if (!ctx) return st;
-    void *buf = nullptr;
-    size_t need = rounded(n_coeffs * 8) + rounded(n_initial * 8) + rounded(cells * kStokes * 8);
-    if ((st = staging(need, &buf))) return st;
-    Stage sg{static_cast<char *>(buf)};
-    st = BH_OK;
-    double *dc = sg.put(ctx, coeffs, n_coeffs, &st);
-    double *di = sg.put(ctx, initial, n_initial, &st);
-    double *dout = sg.put<double>(ctx, nullptr, cells * kStokes, &st);
+    const size_t need = rounded(n_coeffs * 8) + rounded(n_initial * 8) + rounded(cells * kStokes * 8);
+    Stage sg;
+    if ((st = stage_open(ctx, need, sg))) return st;
+    double *dc = sg.put(coeffs, n_coeffs, &st);
+    double *di = sg.put(initial, n_initial, &st);
+    double *dout = sg.put<double>(nullptr, cells * kStokes, &st);
     if (st) return st;
-    begin_timing(ctx);
     hipLaunchKernelGGL(stokes_rk2_kernel, dim3(blocks_for(int64_t(cells), 256)), dim3(256), 0, ctx->stream, dc,
                        int64_t(cells), ds_cm, di, int(bcast), dout);
     BH_HIP(hipGetLastError());
-    end_timing(ctx, 1);
-    BH_HIP(hipMemcpyAsync(out, dout, cells * kStokes * 8, hipMemcpyDeviceToHost, ctx->stream));
-    BH_HIP(hipStreamSynchronize(ctx->stream));
-    return BH_OK;
+    return sg.fetch(out, dout, cells * kStokes * 8);
 }
 
 int bh_sample_brick_trilinear(const double *coeffs, size_t n_coeffs, const double *r_grid, size_t nr,
@@ -833,26 +884,21 @@ int bh_sample_brick_trilinear(const double *coeffs, size_t n_coeffs, const doubl
     std::lock_guard<std::mutex> lock(default_ctx_mutex());
     bh_ctx *ctx = default_ctx(&st);
     if (!ctx) return st;
-    void *buf = nullptr;
-    size_t need = rounded(n_coeffs * 8) + rounded(nr * 8) + rounded(nt * 8) + rounded(np * 8) +
-                  rounded(n_points_flat * 8) + rounded(n * kCoeffs * 8);
-    if ((st = staging(need, &buf))) return st;
-    Stage sg{static_cast<char *>(buf)};
-    double *dc = sg.put(ctx, coeffs, n_coeffs, &st);
-    double *dr = sg.put(ctx, r_grid, nr, &st);
-    double *dt = sg.put(ctx, theta_grid, nt, &st);
-    double *dp = sg.put(ctx, phi_grid, np, &st);
-    double *dpts = sg.put(ctx, points, n_points_flat, &st);
-    double *dout = sg.put<double>(ctx, nullptr, n * kCoeffs, &st);
+    const size_t need = rounded(n_coeffs * 8) + rounded(nr * 8) + rounded(nt * 8) + rounded(np * 8) +
+                        rounded(n_points_flat * 8) + rounded(n * kCoeffs * 8);
+    Stage sg;
+    if ((st = stage_open(ctx, need, sg))) return st;
+    double *dc = sg.put(coeffs, n_coeffs, &st);
+    double *dr = sg.put(r_grid, nr, &st);
+    double *dt = sg.put(theta_grid, nt, &st);
+    double *dp = sg.put(phi_grid, np, &st);
+    double *dpts = sg.put(points, n_points_flat, &st);
+    double *dout = sg.put<double>(nullptr, n * kCoeffs, &st);
     if (st) return st;
-    begin_timing(ctx);
     hipLaunchKernelGGL(sample_gather_kernel, dim3(gather_blocks(ctx, int64_t(n))), dim3(256), 0, ctx->stream, dc, dr, int(nr),
                        dt, int(nt), dp, int(np), dpts, int64_t(n), dout);
     BH_HIP(hipGetLastError());
-    end_timing(ctx, 1);
-    BH_HIP(hipMemcpyAsync(out, dout, n * kCoeffs * 8, hipMemcpyDeviceToHost, ctx->stream));
-    BH_HIP(hipStreamSynchronize(ctx->stream));
-    return BH_OK;
+    return sg.fetch(out, dout, n * kCoeffs * 8);
 }
 
 int bh_sample_and_step_stokes(const double *coeffs, size_t n_coeffs, const double *r_grid, size_t nr,
@@ -877,28 +923,23 @@ int bh_sample_and_step_stokes(const double *coeffs, size_t n_coeffs, const doubl
     std::lock_guard<std::mutex> lock(default_ctx_mutex());
     bh_ctx *ctx = default_ctx(&st);
     if (!ctx) return st;
-    void *buf = nullptr;
-    size_t need = rounded(n_coeffs * 8) + rounded(nr * 8) + rounded(nt * 8) + rounded(np * 8) +
-                  rounded(n_points_flat * 8) + rounded(n_initial * 8) + rounded(n * kStokes * 8);
-    if ((st = staging(need, &buf))) return st;
-    Stage sg{static_cast<char *>(buf)};
-    double *dc = sg.put(ctx, coeffs, n_coeffs, &st);
-    double *dr = sg.put(ctx, r_grid, nr, &st);
-    double *dt = sg.put(ctx, theta_grid, nt, &st);
-    double *dp = sg.put(ctx, phi_grid, np, &st);
-    double *dpts = sg.put(ctx, points, n_points_flat, &st);
-    double *di = sg.put(ctx, initial, n_initial, &st);
-    double *dout = sg.put<double>(ctx, nullptr, n * kStokes, &st);
+    const size_t need = rounded(n_coeffs * 8) + rounded(nr * 8) + rounded(nt * 8) + rounded(np * 8) +
+                        rounded(n_points_flat * 8) + rounded(n_initial * 8) + rounded(n * kStokes * 8);
+    Stage sg;
+    if ((st = stage_open(ctx, need, sg))) return st;
+    double *dc = sg.put(coeffs, n_coeffs, &st);
+    double *dr = sg.put(r_grid, nr, &st);
+    double *dt = sg.put(theta_grid, nt, &st);
+    double *dp = sg.put(phi_grid, np, &st);
+    double *dpts = sg.put(points, n_points_flat, &st);
+    double *di = sg.put(initial, n_initial, &st);
+    double *dout = sg.put<double>(nullptr, n * kStokes, &st);
     if (st) return st;
-    begin_timing(ctx);
     hipLaunchKernelGGL(sample_step_kernel, dim3(blocks_for(int64_t(n), 256)), dim3(256), 0, ctx->stream, dc, dr,
                        int(nr), dt, int(nt), dp, int(np), dpts, int64_t(n), ds_cm, di, int(bcast), invalid_policy,
                        dout);
     BH_HIP(hipGetLastError());
-    end_timing(ctx, 1);
-    BH_HIP(hipMemcpyAsync(out, dout, n * kStokes * 8, hipMemcpyDeviceToHost, ctx->stream));
-    BH_HIP(hipStreamSynchronize(ctx->stream));
-    return BH_OK;
+    return sg.fetch(out, dout, n * kStokes * 8);
 }
 
 int bh_sample_brick_valid_mask(const double *r_grid, size_t nr, const double *theta_grid, size_t nt,
@@ -916,20 +957,15 @@ int bh_sample_brick_valid_mask(const double *r_grid, size_t nr, const double *th
     std::lock_guard<std::mutex> lock(default_ctx_mutex());
     bh_ctx *ctx = default_ctx(&st);
     if (!ctx) return st;
-    void *buf = nullptr;
-    if ((st = staging(rounded(n_points_flat * 8) + rounded(n), &buf))) return st;
-    Stage sg{static_cast<char *>(buf)};
-    double *dpts = sg.put(ctx, points, n_points_flat, &st);
-    uint8_t *dout = sg.put<uint8_t>(ctx, nullptr, n, &st);
+    Stage sg;
+    if ((st = stage_open(ctx, rounded(n_points_flat * 8) + rounded(n), sg))) return st;
+    double *dpts = sg.put(points, n_points_flat, &st);
+    uint8_t *dout = sg.put<uint8_t>(nullptr, n, &st);
     if (st) return st;
-    begin_timing(ctx);
     hipLaunchKernelGGL(valid_mask_kernel, dim3(blocks_for(int64_t(n), 256)), dim3(256), 0, ctx->stream, r_grid[0],
                        r_grid[nr - 1], theta_grid[0], theta_grid[nt - 1], dpts, int64_t(n), dout);
     BH_HIP(hipGetLastError());
-    end_timing(ctx, 1);
-    BH_HIP(hipMemcpyAsync(out, dout, n, hipMemcpyDeviceToHost, ctx->stream));
-    BH_HIP(hipStreamSynchronize(ctx->stream));
-    return BH_OK;
+    return sg.fetch(out, dout, n);
 }
 
 int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs, const double *d_r, int32_t nr,
