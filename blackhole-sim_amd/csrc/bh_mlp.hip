@@ -21,10 +21,14 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 64;     // output tile edge per workgroup
+constexpr int kMaxLayersAdam = 20;   // at least depth + 1 for the largest depth bh_mlp_create accepts
+// batch rows per split-K slice of a weight-gradient GEMM (measured at batch 8192: 256 -> 252 us per Adam step,
+// 512 -> 262, 1024 -> 330; fewer slices leave CUs idle)
+constexpr int kDwRows = 256;
 constexpr int kKStep = 32;    // K per MFMA pass
 constexpr int kLdsK = kKStep + 8;
 
-enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3, EPI_ATOMIC = 4 };
+enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3, EPI_ATOMIC = 4, EPI_MSE = 5 };
 
 struct GemmArgs {
     const __bf16 *A;   // op(A): M x K; stored [M][K] (A_KC) or [K][M]
@@ -40,6 +44,13 @@ struct GemmArgs {
     int ldc;
     int c_trans;       // Cf stored [n][m] (ld = ldc) instead of [m][n]
     int k_split;       // K range of one grid.z slice (multiple of kKStep)
+    float *colpart;    // EPI_DSILU, EPI_MSE: each row tile's column sums of the result, [blockIdx.y][N]
+    // EPI_MSE: weighted squared error against t [M][n_real] with per-output weights w [n_real] and per-row
+    // weights rw [M] (null: 1) on the first `masked` outputs; the result is dL/dy, and each workgroup's
+    // share of the loss goes to loss_part[blockIdx.y gridDim.x + blockIdx.x]
+    const float *t, *w, *rw;
+    int n_real, masked;
+    float *loss_part;
 };
 
 // SiLU and its derivative with the hardware reciprocal (1 ulp) instead of a full-precision divide.
@@ -102,6 +113,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
             for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
+    float csum[2] = {0.f, 0.f};   // EPI_DSILU / EPI_MSE: this lane's part of its two columns' sums
+    float lsum = 0.f;             // EPI_MSE: this lane's part of the loss
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -120,6 +133,18 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                     v += g.bias[n];
                 } else if constexpr (EPI == EPI_DSILU) {
                     v *= dsilu(g.Z[size_t(m) * g.ldz + n]);
+                    csum[j] += v;
+                } else if constexpr (EPI == EPI_MSE) {
+                    const float y = v + g.bias[n];
+                    float gr = 0.f;
+                    if (n < g.n_real) {
+                        const float wo = (g.rw && n < g.masked) ? g.w[n] * g.rw[m] : g.w[n];
+                        const float d = y - g.t[size_t(m) * g.n_real + n];
+                        gr = wo == 0.f ? 0.f : 2.0f * wo * d / float(g.M);
+                        lsum += wo == 0.f ? 0.f : wo * d * d / float(g.M);
+                    }
+                    v = gr;
+                    csum[j] += v;
                 }
                 const size_t ci = g.c_trans ? size_t(n) * g.ldc + m : size_t(m) * g.ldc + n;
                 if constexpr (EPI == EPI_ATOMIC) {
@@ -129,6 +154,82 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                     if (g.Cb) g.Cb[ci] = __bf16(v);
                 }
             }
+    if constexpr (EPI == EPI_DSILU || EPI == EPI_MSE) {
+        // the four lanes of a column (lane / 16) hold its 4 x 2 x 4 values of this wave's 32 rows; the two
+        // waves of a column range (wm = 0, 32) meet in LDS, and the tile's column sums are stored (no atomics,
+        // so that the bias gradient, reduced over the tiles in a fixed order, is the same on every run)
+        __shared__ float cpart[2][32];
+        __shared__ float lpart[4];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            csum[j] += __shfl_xor(csum[j], 16);
+            csum[j] += __shfl_xor(csum[j], 32);
+        }
+        if (wm != 0 && lane < 16) {
+            cpart[wave & 1][rl] = csum[0];
+            cpart[wave & 1][16 + rl] = csum[1];
+        }
+        if constexpr (EPI == EPI_MSE) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+            if (lane == 0) lpart[wave] = lsum;
+        }
+        __syncthreads();
+        if (wm == 0 && lane < 16) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int n = n0 + wn + j * 16 + rl;
+                if (n < g.n_store) g.colpart[size_t(blockIdx.y) * g.N + n] = csum[j] + cpart[wave & 1][j * 16 + rl];
+            }
+        }
+        if constexpr (EPI == EPI_MSE) {
+            if (tid == 0) g.loss_part[blockIdx.y * gridDim.x + blockIdx.x] = lpart[0] + lpart[1] + lpart[2] + lpart[3];
+        }
+    }
+}
+
+// Where each layer's bias gradient lies: b_off[l] in the parameter vector, fo[l] entries, and its row tiles'
+// column sums at part_off[l] in the partial-sum buffer ([tiles][fo[l]]).
+struct BiasMap {
+    int layers, tiles;
+    long long b_off[kMaxLayersAdam], part_off[kMaxLayersAdam];
+    int fo[kMaxLayersAdam];
+};
+
+// Bias gradients: grad[b_off[l] + n] = the sum of the row tiles' column sums, always added in the same order.
+// A workgroup takes 64 biases; its 16 waves sum every 16th tile (tiles w, w + 16, ...), and wave 0 adds the
+// 16 partial sums in order.
+__global__ __launch_bounds__(1024) void bias_reduce_kernel(const float *part, BiasMap B, float *grad) {
+    __shared__ float acc[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int i = int(blockIdx.x) * 64 + lane, l = 0;
+    while (l < B.layers && i >= B.fo[l]) i -= B.fo[l++];
+    float s = 0.f;
+    if (l < B.layers) {
+        const float *p = part + B.part_off[l] + i;
+#pragma unroll 4
+        for (int t = w; t < B.tiles; t += 16) s += p[size_t(t) * B.fo[l]];
+    }
+    acc[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && l < B.layers) {
+        float tot = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) tot += acc[k][lane];
+        grad[B.b_off[l] + i] = tot;
+    }
+}
+
+// Sum of n floats into *out, by one workgroup (the loss from the output GEMM's per-workgroup parts).
+__global__ __launch_bounds__(256) void sum_kernel(const float *x, int n, float *out) {
+    __shared__ float part[4];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = part[0] + part[1] + part[2] + part[3];
 }
 
 // FP32 [rows][n] -> BF16 [rows][n_pad], zero padded
@@ -153,53 +254,16 @@ __global__ void transpose_bf16_kernel(const float *w, int K, int N, __bf16 *wt) 
     wt[size_t(n) * K + k] = __bf16(w[i]);
 }
 
-// Column sums of an [M][N] FP32 matrix into out (zeroed beforehand): block (x, y) sums columns
-// x*256 .. +255 over rows y*kColRows .. +kColRows-1, one atomic per column per block.
-constexpr int kColRows = 32;
-__global__ void colsum_kernel(const float *x, int M, int N, float *out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const int r0 = blockIdx.y * kColRows, r1 = min(M, r0 + kColRows);
-    float s = 0.f;
-    for (int m = r0; m < r1; ++m) s += x[size_t(m) * N + n];
-    atomicAdd(&out[n], s);
-}
+// Where the weight matrices lie in the parameter vector: layer l's W [fi][fo] at w_off[l] (biases follow it).
+struct LayerMap {
+    int layers;
+    long long w_off[kMaxLayersAdam];
+    int fi[kMaxLayersAdam], fo[kMaxLayersAdam];
+};
 
-// Weighted squared error: L = sum_{i,o} w_io (y_io - t_io)^2 / rows,  dY = 2 w_io (y - t) / rows, with
-// w_io = w_o * rw_i for o < masked (rw null: 1) and w_o otherwise.  y is [rows][n_pad]; t is [rows][n];
-// padded columns get zero gradient.  Block (x, r) covers columns x*256 .. of rows r*kMseRows ..; the
-// loss is reduced in the block and added with one atomic per block.
-constexpr int kMseRows = 16;
-__global__ __launch_bounds__(256) void mse_grad_kernel(const float *y, const float *t, const float *w, const float *rw,
-                                                       int masked, int rows, int n, int n_pad, float *dy, __bf16 *dyb,
-                                                       float *loss) {
-    __shared__ float part[4];
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    float l = 0.f;
-    if (o < n_pad) {
-        const float wo_col = o < n ? w[o] : 0.f;
-        const int r0 = blockIdx.y * kMseRows, r1 = min(rows, r0 + kMseRows);
-        for (int r = r0; r < r1; ++r) {
-            const size_t i = size_t(r) * n_pad + o;
-            float gr = 0.f;
-            if (o < n) {
-                const float wo = (rw && o < masked) ? wo_col * rw[r] : wo_col;
-                const float d = y[i] - t[size_t(r) * n + o];
-                gr = wo == 0.f ? 0.f : 2.0f * wo * d / float(rows);
-                l += wo == 0.f ? 0.f : wo * d * d / float(rows);
-            }
-            dy[i] = gr;
-            dyb[i] = __bf16(gr);
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss, part[0] + part[1] + part[2] + part[3]);
-}
-
+// Adam update of the FP32 masters, with the BF16 copy and, for a weight, its BF16 W^T entry.
 __global__ void adam_kernel(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2,
-                            float eps, float c1, float c2, __bf16 *wb) {
+                            float eps, float c1, float c2, __bf16 *wb, __bf16 *wtb, LayerMap L) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -210,6 +274,13 @@ __global__ void adam_kernel(float *w, const float *g, float *m, float *v, size_t
     const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     w[i] = wi;
     wb[i] = __bf16(wi);
+    for (int l = 0; l < L.layers; ++l) {
+        const long long e = (long long)i - L.w_off[l];
+        if (e >= 0 && e < (long long)L.fi[l] * L.fo[l]) {
+            const int k = int(e / L.fo[l]), c = int(e % L.fo[l]);
+            wtb[L.w_off[l] + (long long)c * L.fi[l] + k] = __bf16(wi);
+        }
+    }
 }
 
 // ---- fused inference: all layers of a 128-row tile in one workgroup ------------------------------
@@ -466,10 +537,14 @@ struct bh_mlp {
     __bf16 *xb = nullptr;                      // [cap][n_in_pad]
     std::vector<__bf16 *> act;                 // per hidden layer [cap][width]
     std::vector<float *> zpre;                 // per hidden layer pre-activation [cap][width]
-    float *y = nullptr, *dy = nullptr;         // [cap][n_out_pad]
-    __bf16 *dyb = nullptr, *dab = nullptr, *dab2 = nullptr;
-    float *dz = nullptr;
+    __bf16 *dyb = nullptr;                     // [cap][n_out_pad] loss gradient of the outputs
+    __bf16 *dab = nullptr, *dab2 = nullptr;    // [cap][widest] pre-activation gradients, alternating
     float *loss = nullptr;
+    float *loss_part = nullptr;                // per-workgroup loss of the output GEMM (one per 64 x 64 tile)
+    int loss_parts = 0;
+    float *bias_part = nullptr;                // per row tile column sums of each layer's gradient [tiles][fo]
+    int n_bias = 0;                            // biases in all (sum of fo)
+    std::vector<int> fo_prefix;                // biases of the layers before l
     bool fused = true;                         // inference through mlp_fused_kernel when the shape allows
 };
 
@@ -494,12 +569,13 @@ static void free_acts(bh_mlp *m) {
     for (auto p : m->act) (void)hipFree(p);
     for (auto p : m->zpre)
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)m->y, (void *)m->dy, (void *)m->dyb, (void *)m->dab, (void *)m->dab2, (void *)m->dz})
+    for (void *p : {(void *)m->dyb, (void *)m->dab, (void *)m->dab2, (void *)m->loss_part, (void *)m->bias_part})
         if (p) (void)hipFree(p);
+    m->loss_part = m->bias_part = nullptr;
+    m->loss_parts = 0;
     m->xb = nullptr;
     m->act.clear();
     m->zpre.clear();
-    m->y = m->dy = m->dz = nullptr;
     m->dyb = m->dab = m->dab2 = nullptr;
     m->cap = 0;
     m->cap_train = false;
@@ -522,11 +598,13 @@ static int ensure_cap(bh_mlp *m, int rows, bool train) {
         m->act.push_back(a);
         m->zpre.push_back(z);
     }
-    ok = ok && hipMalloc(&m->y, R * m->n_out_pad * 4) == hipSuccess;
+    const int parts = int(((R + kTile - 1) / kTile) * ((m->n_out_pad + kTile - 1) / kTile));
     if (ok && train)
-        ok = hipMalloc(&m->dy, R * m->n_out_pad * 4) == hipSuccess && hipMalloc(&m->dyb, R * m->n_out_pad * 2) == hipSuccess &&
-             hipMalloc(&m->dab, R * widest * 2) == hipSuccess && hipMalloc(&m->dab2, R * widest * 2) == hipSuccess &&
-             hipMalloc(&m->dz, R * widest * 4) == hipSuccess;
+        ok = hipMalloc(&m->dyb, R * m->n_out_pad * 2) == hipSuccess && hipMalloc(&m->dab, R * widest * 2) == hipSuccess &&
+             hipMalloc(&m->dab2, R * widest * 2) == hipSuccess &&
+             hipMalloc(&m->loss_part, size_t(parts) * 4) == hipSuccess &&
+             hipMalloc(&m->bias_part, ((R + kTile - 1) / kTile) * size_t(m->n_bias) * 4) == hipSuccess;
+    if (ok && train) m->loss_parts = parts;
     if (!ok) {
         free_acts(m);
         return set_error(BH_ERR_MEMORY, "MLP activation allocation for %d rows failed", rows);
@@ -551,16 +629,16 @@ static int refresh_bf16(bh_mlp *m) {
     return BH_OK;
 }
 
-// Forward of `rows` rows of x (FP32 [rows][n_in]).  Output: FP32 into m->y (ld n_out_pad) when
-// y_out is null, else the n_out real columns into y_out ([rows][ld_out], or [n_out][ld_out] when
-// trans).  train: keep the pre-activations for backward.
+// Forward of `rows` rows of x (FP32 [rows][n_in]): the n_out real columns into y_out ([rows][ld_out], or
+// [n_out][ld_out] when trans).  train: the hidden layers only, keeping the pre-activations; the output
+// layer then runs in backward(), whose epilogue turns its result into the loss gradient.
 static int forward(bh_mlp *m, const float *d_x, int rows, bool train, float *y_out, int trans, int ld_out) {
     bh_ctx *ctx = m->ctx;
     hipLaunchKernelGGL(pack_input_kernel, dim3(blocks_for(size_t(rows) * m->n_in_pad)), dim3(256), 0, ctx->stream, d_x,
                        rows, m->n_in, m->n_in_pad, m->xb);
     const __bf16 *in = m->xb;
     int st;
-    for (int l = 0; l <= m->depth; ++l) {
+    for (int l = 0; l <= (train ? m->depth - 1 : m->depth); ++l) {
         GemmArgs g{};
         g.A = in;
         g.lda = m->fi[l];
@@ -577,15 +655,11 @@ static int forward(bh_mlp *m, const float *d_x, int rows, bool train, float *y_o
             g.ldc = g.N;
             st = run_gemm<true, true, EPI_BIAS_SILU>(ctx, g);
             in = m->act[l];
-        } else if (y_out) {
+        } else {
             g.Cf = y_out;
             g.n_store = m->n_out;
             g.c_trans = trans;
             g.ldc = ld_out;
-            st = run_gemm<true, true, EPI_BIAS>(ctx, g);
-        } else {
-            g.Cf = m->y;
-            g.ldc = g.N;
             st = run_gemm<true, true, EPI_BIAS>(ctx, g);
         }
         if (st) return st;
@@ -593,17 +667,51 @@ static int forward(bh_mlp *m, const float *d_x, int rows, bool train, float *y_o
     return BH_OK;
 }
 
-// Backward of the weighted MSE loss at the activations of the last forward; gradients into m->grad.
+// Start of layer l's row-tile column sums in m->bias_part ([cap tiles][fo[l]] per layer, layers in order).
+static size_t bias_part_base(const bh_mlp *m, int l) {
+    return size_t((m->cap + kTile - 1) / kTile) * size_t(m->fo_prefix[l]);
+}
+
+// The batch loss of the last backward() into m->loss (the sum of the output GEMM's per-workgroup parts).
+static int loss_total(bh_mlp *m, int rows) {
+    const int parts = ((rows + kTile - 1) / kTile) * ((m->n_out_pad + kTile - 1) / kTile);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, m->ctx->stream, m->loss_part, parts, m->loss);
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
+// Output layer and backward of the weighted MSE loss at the hidden activations of the last training
+// forward; gradients into m->grad, the loss in parts into m->loss_part (loss_total() adds them up).  Bias
+// gradients are the column sums of each
+// layer's pre-activation gradient, formed in the epilogue of the GEMM that produces it.
 static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, const float *d_rw, int masked) {
     bh_ctx *ctx = m->ctx;
     BH_HIP(hipMemsetAsync(m->grad, 0, m->n_params * 4, ctx->stream));
-    BH_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), ctx->stream));
-    hipLaunchKernelGGL(mse_grad_kernel, dim3((m->n_out_pad + 255) / 256, (rows + kMseRows - 1) / kMseRows), dim3(256), 0,
-                       ctx->stream, m->y, d_t, d_w, d_rw, masked, rows, m->n_out, m->n_out_pad, m->dy, m->dyb, m->loss);
     int st;
+    {   // y = act W + b, then dL/dy (BF16), its column sums (output bias gradient) and the loss
+        const int l = m->depth;
+        GemmArgs g{};
+        g.A = l == 0 ? m->xb : m->act[l - 1];
+        g.lda = m->fi[l];
+        g.B = m->wtb + m->w_off[l];
+        g.ldb = m->fi[l];
+        g.M = rows;
+        g.K = m->fi[l];
+        g.N = m->fo[l];
+        g.bias = m->w32 + m->b_off[l];
+        g.Cb = m->dyb;
+        g.ldc = g.N;
+        g.colpart = m->bias_part + bias_part_base(m, l);
+        g.t = d_t;
+        g.w = d_w;
+        g.rw = d_rw;
+        g.n_real = m->n_out;
+        g.masked = masked;
+        g.loss_part = m->loss_part;
+        if ((st = run_gemm<true, true, EPI_MSE>(ctx, g))) return st;
+    }
     const __bf16 *dcur = m->dyb;   // gradient w.r.t. this layer's pre-activation, BF16
-    const float *dcurf = m->dy;    // the same in FP32 (bias gradient)
-    const int splits = rows >= 1024 ? rows / 256 : 1;
+    const int splits = rows >= 1024 ? rows / kDwRows : 1;
     for (int l = m->depth; l >= 0; --l) {
         const int K = m->fi[l], N = m->fo[l];
         const __bf16 *xin = l == 0 ? m->xb : m->act[l - 1];
@@ -619,8 +727,6 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         gw.Cf = m->grad + m->w_off[l];
         gw.ldc = N;
         if ((st = run_gemm<false, false, EPI_ATOMIC>(ctx, gw, splits))) return st;
-        hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256, (rows + kColRows - 1) / kColRows), dim3(256), 0,
-                           ctx->stream, dcurf, rows, N, m->grad + m->b_off[l]);
         if (l == 0) break;
         // dZ_prev = (dZ W^T) * silu'(z_prev)
         __bf16 *dnext = (dcur == m->dab) ? m->dab2 : m->dab;
@@ -634,13 +740,22 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         gd.K = N;
         gd.Z = m->zpre[l - 1];
         gd.ldz = K;
-        gd.Cf = m->dz;
         gd.Cb = dnext;
         gd.ldc = K;
+        gd.colpart = m->bias_part + bias_part_base(m, l - 1);
         if ((st = run_gemm<true, true, EPI_DSILU>(ctx, gd))) return st;
         dcur = dnext;
-        dcurf = m->dz;
     }
+    BiasMap B{};
+    B.layers = m->depth + 1;
+    B.tiles = (rows + kTile - 1) / kTile;
+    for (int l = 0; l <= m->depth; ++l) {
+        B.b_off[l] = (long long)m->b_off[l];
+        B.part_off[l] = (long long)bias_part_base(m, l);
+        B.fo[l] = m->fo[l];
+    }
+    hipLaunchKernelGGL(bias_reduce_kernel, dim3(unsigned((m->n_bias + 63) / 64)), dim3(1024), 0, ctx->stream, m->bias_part,
+                       B, m->grad);
     BH_HIP(hipGetLastError());
     return BH_OK;
 }
@@ -744,6 +859,8 @@ int bh_mlp_create(bh_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32
         off += size_t(a) * b;
         m->b_off.push_back(off);
         off += b;
+        m->fo_prefix.push_back(m->n_bias);
+        m->n_bias += b;
     }
     m->n_params = off;
     bool ok = hipMalloc(&m->w32, off * 4) == hipSuccess && hipMalloc(&m->wb, off * 2) == hipSuccess &&
@@ -883,7 +1000,10 @@ int bh_mlp_gradient(bh_mlp *m, const float *d_x, const float *d_t, const float *
     if (masked_cols < 0 || masked_cols > m->n_out) return set_error(BH_ERR_VALUE, "masked_cols must lie in [0, n_out]");
     if ((st = backward(m, rows, d_t, d_w, d_row_w, masked_cols))) return st;
     if (grad_host) BH_HIP(hipMemcpyAsync(grad_host, m->grad, m->n_params * 4, hipMemcpyDeviceToHost, m->ctx->stream));
-    if (loss_host) BH_HIP(hipMemcpyAsync(loss_host, m->loss, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    if (loss_host) {
+        if ((st = loss_total(m, rows))) return st;
+        BH_HIP(hipMemcpyAsync(loss_host, m->loss, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    }
     BH_HIP(hipStreamSynchronize(m->ctx->stream));
     return BH_OK;
 }
@@ -904,12 +1024,19 @@ int bh_mlp_train_step(bh_mlp *m, const float *d_x, const float *d_t, const float
     m->adam_t += 1;
     const double b1 = 0.9, b2 = 0.999;
     const float c1 = float(1.0 - std::pow(b1, double(m->adam_t))), c2 = float(1.0 - std::pow(b2, double(m->adam_t)));
+    LayerMap L{};
+    L.layers = m->depth + 1;
+    for (int l = 0; l <= m->depth; ++l) {
+        L.w_off[l] = (long long)m->w_off[l];
+        L.fi[l] = m->fi[l];
+        L.fo[l] = m->fo[l];
+    }
     hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(m->n_params)), dim3(256), 0, m->ctx->stream, m->w32, m->grad,
-                       m->adam_m, m->adam_v, m->n_params, lr, float(b1), float(b2), 1e-8f, c1, c2, m->wb);
-    refresh_transposed(m);
+                       m->adam_m, m->adam_v, m->n_params, lr, float(b1), float(b2), 1e-8f, c1, c2, m->wb, m->wtb, L);
     BH_HIP(hipGetLastError());
     end_timing(m->ctx, 1);
     if (loss_host) {
+        if ((st = loss_total(m, rows))) return st;
         BH_HIP(hipMemcpyAsync(loss_host, m->loss, 4, hipMemcpyDeviceToHost, m->ctx->stream));
         BH_HIP(hipStreamSynchronize(m->ctx->stream));
     }
