@@ -140,7 +140,7 @@ struct Mth<float> {
     static __device__ __forceinline__ float pi_minus_eps() { return float(3.14159265358979323846 - kEpsTheta); }
     static __device__ __forceinline__ float wrap(float phi, float base) {
         float p = phi - base;
-        p = p - two_pi() * floorf(p * 0.159154943091895336f);
+        p = __builtin_fmaf(-two_pi(), floorf(p * 0.159154943091895336f), p);
         if (p < 0.f) p += two_pi();
         if (p >= two_pi()) p -= two_pi();
         return p + base;
@@ -232,8 +232,12 @@ __device__ __forceinline__ double island_rcp(double x) {
 
 // d/dlambda (r, theta, phi, p_r, p_theta) of the Kerr null Hamiltonian.  ISLAND: FP64 inside the islands
 // of a mixed kernel, with the island math above.
+// The arithmetic of the right-hand side and of the steps below is written with explicit fused
+// multiply-adds and compiled with contraction off, so that a step rounds the same in every code path that
+// takes it (the ray's own lane, a lone ray's batch, the helper of a split lone ray): left to the
+// compiler, the fusion of a * b + c depends on the code the step is inlined into.
 template <typename R, bool ISLAND = false>
-__device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
+__device__ __forceinline__ void kerr_rhs_free(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
                                          R &dpth) {
     const R th = clamp_theta(th_in);
     R s, c;
@@ -276,6 +280,60 @@ __device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R
     dpth = R(-0.5) * inv_sig * (dNth + R(2) * k.a2 * sc * N * inv_sig);
 }
 
+template <typename R, bool ISLAND = false>
+__device__ __forceinline__ void kerr_rhs_pinned(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
+                                         R &dpth) {
+#pragma clang fp contract(off)
+    const R th = clamp_theta(th_in);
+    R s, c;
+    if constexpr (ISLAND) island_sincos(th, s, c);
+    else Mth<R>::sincos(th, s, c);
+    R s2 = s * s;
+    s2 = (s2 < R(kEpsTheta * kEpsTheta)) ? R(kEpsTheta * kEpsTheta) : s2;   // kerr.py:93 floor
+    const R r2 = r * r;
+    const R r2a2 = r2 + k.a2;
+    const R sig = fma_hd(k.a2, c * c, r2);
+    const R dlt = fma_hd(R(-2), r, r2a2);
+    const R P = fma_hd(r2a2, k.E, -k.aL);
+    R inv_s2, inv_sig, inv_dlt;
+    if constexpr (sizeof(R) == 4 || ISLAND) {
+        // one reciprocal of Sigma * Delta * sin^2 serves all three (|product| stays well inside FP32 range)
+        const R sd = sig * dlt;
+        R inv;
+        if constexpr (ISLAND) inv = island_rcp(sd * s2);
+        else inv = Mth<R>::rcp(sd * s2);
+        inv_s2 = sd * inv;
+        inv_sig = (dlt * s2) * inv;
+        inv_dlt = (sig * s2) * inv;
+    } else {
+        inv_s2 = Mth<R>::rcp(s2);
+        inv_sig = Mth<R>::rcp(sig);
+        inv_dlt = Mth<R>::rcp(dlt);
+    }
+    const R Wd = (P * P) * inv_dlt;
+    const R T = fma_hd(k.a2E2, s2, fma_hd(k.L2, inv_s2, -k.twoaEL));
+    const R pr2 = pr * pr;
+    const R N = fma_hd(dlt, pr2, fma_hd(pth, pth, T)) - Wd;   // 2 Sigma H
+    dr = (dlt * pr) * inv_sig;
+    dth = pth * inv_sig;
+    dph = fma_hd(k.a * P, inv_dlt, fma_hd(k.L, inv_s2, -k.aE)) * inv_sig;
+    const R dlt_p = fma_hd(R(2), r, R(-2));
+    const R dNr = fma_hd(Wd * dlt_p, inv_dlt, fma_hd(dlt_p, pr2, -((k.fourE * r) * P) * inv_dlt));
+    dpr = (R(-0.5) * inv_sig) * fma_hd(-((R(2) * r) * N), inv_sig, dNr);
+    const R sc = s * c;
+    const R dNth = (R(2) * sc) * fma_hd(-(k.L2 * inv_s2), inv_s2, k.a2E2);
+    dpth = (R(-0.5) * inv_sig) * fma_hd(((R(2) * k.a2) * sc) * N, inv_sig, dNth);
+}
+
+// The FP32 and island steps use the pinned form; the pure FP64 kernels, which take every step in the ray's own
+// lane, keep the compiler's fusion (the form their parity with the oracle was measured on).
+template <typename R, bool ISLAND = false>
+__device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
+                                         R &dpth) {
+    if constexpr (sizeof(R) == 8 && !ISLAND) kerr_rhs_free<R, ISLAND>(k, r, th_in, pr, pth, dr, dth, dph, dpr, dpth);
+    else kerr_rhs_pinned<R, ISLAND>(k, r, th_in, pr, pth, dr, dth, dph, dpr, dpth);
+}
+
 // ---- brick sampler -----------------------------------------------------------
 
 // Storage codes inside the kernels: BH_STORE_* in bits 0-1, plus kStoreWide for corner bricks of 2 GiB and
@@ -295,20 +353,6 @@ __device__ __forceinline__ float load_coeff(__amdgpu_buffer_rsrc_t rs, unsigned 
     } else {
         unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(rs, int(voff), int(soff), 0);
         return float(__builtin_bit_cast(_Float16, h));
-    }
-}
-
-// Element idx of a corner record (STORE: f32, bf16 or f16) as FP32, through a 64-bit address (wide
-// bricks).
-template <int STORE>
-__device__ __forceinline__ float load_coeff_at(const char *rec, unsigned idx) {
-    if constexpr (store_of(STORE) == BH_STORE_F32) {
-        return reinterpret_cast<const float *>(rec)[idx];
-    } else if constexpr (store_of(STORE) == BH_STORE_BF16) {
-        const unsigned h = reinterpret_cast<const unsigned short *>(rec)[idx];
-        return __builtin_bit_cast(float, h << 16);
-    } else {
-        return float(reinterpret_cast<const _Float16 *>(rec)[idx]);
     }
 }
 
@@ -444,10 +488,7 @@ __device__ __forceinline__ float half_hi(unsigned w, int store) {
                                   : float(__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)));
 }
 
-// COOP (FP32, corner layout): every lane of the wave holds the same ray; lane k (k < 11) loads the 8 corner
-// values of channel k and evaluates that channel's weighted sum in the same order as the per-lane
-// form, so the eleven results are bit-identical to it, and each result is read back from its lane.
-template <typename R, int STORE, int LAYOUT, bool COOP = false>
+template <typename R, int STORE, int LAYOUT>
 __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
                                              const R *gp, R x, R y, R z, R out[kCoeffs]) {
     CellPos<R> cp;
@@ -455,33 +496,6 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
     const int r0 = cp.r0, t0 = cp.t0, p0 = cp.p0, p1 = cp.p1;
     const R wr = cp.wr, wt = cp.wt, wp = cp.wp;
     constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
-    if constexpr (COOP) {
-        static_assert(sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER, "the cooperative sample is FP32, corner layout");
-        (void)p1;
-        const unsigned k = min(unsigned(threadIdx.x & 63), unsigned(kCoeffs - 1));   // lanes >= 11 repeat channel 10
-        const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
-        float v[8];
-        if constexpr (wide_of(STORE)) {
-            const char *rec = corner_ptr<STORE>(A, cell);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = load_coeff_at<STORE>(rec, unsigned(q * kCoeffs) + k);
-        } else {
-            const unsigned base = A.brick_base + cell * unsigned(8 * kCoeffs * esz);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = load_coeff<STORE>(rs, base + (unsigned(q * kCoeffs) + k) * esz, 0);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
-        const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
-        const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
-        R acc = w8[0] * R(v[0]);
-#pragma unroll
-        for (int q = 1; q < 8; ++q) acc = acc + w8[q] * R(v[q]);
-#pragma unroll
-        for (int j = 0; j < kCoeffs; ++j)
-            out[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, acc), j));
-        return true;
-    }
     if constexpr (LAYOUT == BH_LAYOUT_CORNER) {
         // one cell record: corners (dr, dt, dp) in order dr*4 + dt*2 + dp, 11 channels each
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
@@ -516,7 +530,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
                 float v;
                 if constexpr (store_of(STORE) == BH_STORE_F32) v = __builtin_bit_cast(float, w[e]);
                 else v = (e & 1) ? half_hi(w[e >> 1], store_of(STORE)) : half_lo(w[e >> 1], store_of(STORE));
-                acc = (q == 0) ? w8[0] * R(v) : acc + w8[q] * R(v);
+                acc = (q == 0) ? w8[0] * R(v) : fma_hd(w8[q], R(v), acc);
             }
             out[k] = acc;
         }
@@ -563,7 +577,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         for (int k = 0; k < kCoeffs; ++k) {
             R acc = w8[0] * raw[k][0];
 #pragma unroll
-            for (int q = 1; q < 8; ++q) acc += w8[q] * raw[k][q];
+            for (int q = 1; q < 8; ++q) acc = fma_hd(w8[q], raw[k][q], acc);
             out[k] = acc;
         }
     }
@@ -628,21 +642,24 @@ __device__ __forceinline__ bool sample_brick_numpy32(const RenderArgs &A, __amdg
 // lib.rs:37-59 and 210-225.
 template <typename R>
 __device__ __forceinline__ void stokes_rhs(const R s[4], const R c[kCoeffs], R o[4]) {
-    o[0] = c[0] - (c[4] * s[0] + c[5] * s[1] + c[6] * s[2] + c[7] * s[3]);
-    o[1] = c[1] - (c[5] * s[0] + c[4] * s[1] + c[8] * s[2] - c[10] * s[3]);
-    o[2] = c[2] - (c[6] * s[0] - c[8] * s[1] + c[4] * s[2] + c[9] * s[3]);
-    o[3] = c[3] - (c[7] * s[0] + c[10] * s[1] - c[9] * s[2] + c[4] * s[3]);
+#pragma clang fp contract(off)
+    o[0] = c[0] - dot4_hd(c[4], s[0], c[5], s[1], c[6], s[2], c[7], s[3], R(0));
+    o[1] = c[1] - dot4_hd(c[5], s[0], c[4], s[1], c[8], s[2], -c[10], s[3], R(0));
+    o[2] = c[2] - dot4_hd(c[6], s[0], -c[8], s[1], c[4], s[2], c[9], s[3], R(0));
+    o[3] = c[3] - dot4_hd(c[7], s[0], c[10], s[1], -c[9], s[2], c[4], s[3], R(0));
 }
 
 template <typename R>
 __device__ __forceinline__ void stokes_rk2(R s[4], const R c[kCoeffs], R ds) {
+#pragma clang fp contract(off)
     R k1[4], mid[4], k2[4];
     stokes_rhs(s, c, k1);
+    const R hh = R(0.5) * ds;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) mid[i] = s[i] + R(0.5) * ds * k1[i];
+    for (int i = 0; i < 4; ++i) mid[i] = fma_hd(hh, k1[i], s[i]);
     stokes_rhs(mid, c, k2);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s[i] = s[i] + ds * k2[i];
+    for (int i = 0; i < 4; ++i) s[i] = fma_hd(ds, k2[i], s[i]);
 }
 
 // ---- camera (kerr.py:188-233), FP64 per ray --------------------------------
@@ -739,14 +756,14 @@ __device__ __forceinline__ RayK<G> make_k(double a, double E, double L) {
 // RK2 midpoint step of the geodesic in precision G: k1 = f(y), k2 = f(y + h/2 k1), y + h k2.
 // Returns the raw (unclamped) end theta in nt_raw and the midpoint theta in mt.
 template <typename G, bool ISLAND = false>
-__device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
+__device__ __forceinline__ void geo_step_free(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
                                          G &npr, G &npt, G &mt) {
     G k1r, k1t, k1p, k1pr, k1pt;
-    kerr_rhs<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    kerr_rhs_free<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
     const G hh = G(0.5) * h;
     mt = th + hh * k1t;
     G k2r, k2t, k2p, k2pr, k2pt;
-    kerr_rhs<G, ISLAND>(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    kerr_rhs_free<G, ISLAND>(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
     nr = r + h * k2r;
     nt_raw = th + h * k2t;
     np = ph + h * k2p;
@@ -754,24 +771,49 @@ __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr
     npt = pth + h * k2pt;
 }
 
+template <typename G, bool ISLAND = false>
+__device__ __forceinline__ void geo_step_pinned(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
+                                         G &npr, G &npt, G &mt) {
+#pragma clang fp contract(off)
+    G k1r, k1t, k1p, k1pr, k1pt;
+    kerr_rhs_pinned<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    const G hh = G(0.5) * h;
+    mt = fma_hd(hh, k1t, th);
+    G k2r, k2t, k2p, k2pr, k2pt;
+    kerr_rhs_pinned<G, ISLAND>(k, fma_hd(hh, k1r, r), mt, fma_hd(hh, k1pr, pr), fma_hd(hh, k1pt, pth), k2r, k2t, k2p, k2pr,
+                        k2pt);
+    nr = fma_hd(h, k2r, r);
+    nt_raw = fma_hd(h, k2t, th);
+    np = fma_hd(h, k2p, ph);
+    npr = fma_hd(h, k2pr, pr);
+    npt = fma_hd(h, k2pt, pth);
+}
+
+template <typename G, bool ISLAND = false>
+__device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
+                                         G &npr, G &npt, G &mt) {
+    if constexpr (sizeof(G) == 8 && !ISLAND) geo_step_free<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
+    else geo_step_pinned<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
+}
+
 // Classical RK4 step of the reference tracer (kerr.py:306-313): k1 = f(y), k2 = f(y + h/2 k1),
 // k3 = f(y + h/2 k2), k4 = f(y + h k3), y + (h/6)(k1 + 2 k2 + 2 k3 + k4), summed in that order.
 // Returns the raw end theta in nt_raw and the lowest and highest theta of the three inner stages.
 template <typename G, bool ISLAND = false>
-__device__ __forceinline__ void geo_step4(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
+__device__ __forceinline__ void geo_step4_free(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
                                           G &np, G &npr, G &npt, G &mlo, G &mhi) {
     G k1r, k1t, k1p, k1pr, k1pt;
-    kerr_rhs<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    kerr_rhs_free<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
     const G hh = G(0.5) * h;
     const G t2 = th + hh * k1t;
     G k2r, k2t, k2p, k2pr, k2pt;
-    kerr_rhs<G, ISLAND>(k, r + hh * k1r, t2, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    kerr_rhs_free<G, ISLAND>(k, r + hh * k1r, t2, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
     const G t3 = th + hh * k2t;
     G k3r, k3t, k3p, k3pr, k3pt;
-    kerr_rhs<G, ISLAND>(k, r + hh * k2r, t3, pr + hh * k2pr, pth + hh * k2pt, k3r, k3t, k3p, k3pr, k3pt);
+    kerr_rhs_free<G, ISLAND>(k, r + hh * k2r, t3, pr + hh * k2pr, pth + hh * k2pt, k3r, k3t, k3p, k3pr, k3pt);
     const G t4 = th + h * k3t;
     G k4r, k4t, k4p, k4pr, k4pt;
-    kerr_rhs<G, ISLAND>(k, r + h * k3r, t4, pr + h * k3pr, pth + h * k3pt, k4r, k4t, k4p, k4pr, k4pt);
+    kerr_rhs_free<G, ISLAND>(k, r + h * k3r, t4, pr + h * k3pr, pth + h * k3pt, k4r, k4t, k4p, k4pr, k4pt);
     const G w = h / G(6);
     nr = r + w * (k1r + G(2) * k2r + G(2) * k3r + k4r);
     nt_raw = th + w * (k1t + G(2) * k2t + G(2) * k3t + k4t);
@@ -780,6 +822,41 @@ __device__ __forceinline__ void geo_step4(const RayK<G> &k, G r, G th, G ph, G p
     npt = pth + w * (k1pt + G(2) * k2pt + G(2) * k3pt + k4pt);
     mlo = t2 < t3 ? (t2 < t4 ? t2 : t4) : (t3 < t4 ? t3 : t4);
     mhi = t2 > t3 ? (t2 > t4 ? t2 : t4) : (t3 > t4 ? t3 : t4);
+}
+
+template <typename G, bool ISLAND = false>
+__device__ __forceinline__ void geo_step4_pinned(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
+                                          G &np, G &npr, G &npt, G &mlo, G &mhi) {
+#pragma clang fp contract(off)
+    G k1r, k1t, k1p, k1pr, k1pt;
+    kerr_rhs_pinned<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    const G hh = G(0.5) * h;
+    // the stages and the update are rounded as NumPy rounds them (kerr.py:306-313): a product, then a sum
+    const G t2 = th + hh * k1t;
+    G k2r, k2t, k2p, k2pr, k2pt;
+    kerr_rhs_pinned<G, ISLAND>(k, r + hh * k1r, t2, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    const G t3 = th + hh * k2t;
+    G k3r, k3t, k3p, k3pr, k3pt;
+    kerr_rhs_pinned<G, ISLAND>(k, r + hh * k2r, t3, pr + hh * k2pr, pth + hh * k2pt, k3r, k3t, k3p, k3pr, k3pt);
+    const G t4 = th + h * k3t;
+    G k4r, k4t, k4p, k4pr, k4pt;
+    kerr_rhs_pinned<G, ISLAND>(k, r + h * k3r, t4, pr + h * k3pr, pth + h * k3pt, k4r, k4t, k4p, k4pr, k4pt);
+    const G w = h / G(6);
+    nr = r + w * (k1r + G(2) * k2r + G(2) * k3r + k4r);
+    nt_raw = th + w * (k1t + G(2) * k2t + G(2) * k3t + k4t);
+    np = ph + w * (k1p + G(2) * k2p + G(2) * k3p + k4p);
+    npr = pr + w * (k1pr + G(2) * k2pr + G(2) * k3pr + k4pr);
+    npt = pth + w * (k1pt + G(2) * k2pt + G(2) * k3pt + k4pt);
+    mlo = t2 < t3 ? (t2 < t4 ? t2 : t4) : (t3 < t4 ? t3 : t4);
+    mhi = t2 > t3 ? (t2 > t4 ? t2 : t4) : (t3 > t4 ? t3 : t4);
+}
+
+template <typename G, bool ISLAND = false>
+__device__ __forceinline__ void geo_step4(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
+                                          G &np, G &npr, G &npt, G &mlo, G &mhi) {
+    if constexpr (sizeof(G) == 8 && !ISLAND)
+        geo_step4_free<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo, mhi);
+    else geo_step4_pinned<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo, mhi);
 }
 
 // The end point of one geodesic step: the state after it, whether it met the pole pathology (the
@@ -842,12 +919,21 @@ __device__ __forceinline__ GeoPoint<R> geo_next(const RenderArgs &A, const HotPa
     return b;
 }
 
-// ds of the segment from the ray's position to b (accelerated_renderer.py:119-121).
+// ds of the segment from (r, th, ph) to b (accelerated_renderer.py:119-121).  The sum of squares is
+// written as explicit fused multiply-adds with contraction off, so every code path that takes a step
+// (the ray's own lane, a lone ray's batch, the helper of a split lone ray) rounds ds the same way.
+template <typename R>
+__device__ __forceinline__ R segment_ds_at(const HotParams<R> &P, R r, R th, R ph, const GeoPoint<R> &b) {
+#pragma clang fp contract(off)
+    const R dx = b.r - r, dy = b.th - th, dz = b.ph - ph;
+    const R dlam = Mth<R>::sqrt(fma_hd(dz, dz, fma_hd(dy, dy, dx * dx)));
+    return pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
+}
+
+// ds of the segment from the ray's position to b.
 template <typename R>
 __device__ __forceinline__ R segment_ds(const HotParams<R> &P, const RayState<R> &s, const GeoPoint<R> &b) {
-    const R dx = b.r - s.r, dy = b.th - s.th, dz = b.ph - s.ph;
-    const R dlam = Mth<R>::sqrt(dx * dx + dy * dy + dz * dz);
-    return pymax(P.L * pymax(dlam, R(1.0e-14)), R(0));
+    return segment_ds_at(P, s.r, s.th, s.ph, b);
 }
 
 // Transfer over a segment of length ds with the coefficients c sampled at its midpoint
@@ -856,7 +942,7 @@ template <typename R, bool EXACT>
 __device__ __forceinline__ void segment_transfer(RayState<R> &s, R ds, const R c[kCoeffs]) {
     if constexpr (EXACT) stokes_exact(s.S, c, ds);
     else stokes_rk2(s.S, c, ds);
-    s.tau = s.tau + pymax(c[4], R(0)) * ds;
+    s.tau = fma_hd(pymax(c[4], R(0)), ds, s.tau);
 }
 
 // The geodesic tests of a step that ends at b from radius r_prev: captured, escaped (MIXED: in FP64).
@@ -885,7 +971,7 @@ __device__ __forceinline__ int end_code(const HotParams<R> &P, const RayState<R>
 }
 
 // One RK2 step plus its transfer segment.  Returns the status code when the ray ends, -1 otherwise.
-template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4, bool COOP = false>
+template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                         const R *gr, const R *gt, const R *gp, RayState<R> &s, unsigned &n_samples) {
     const GeoPoint<R> b = geo_next<R, MIXED, RK4>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
@@ -896,10 +982,10 @@ __device__ __forceinline__ int ray_step(const RenderArgs &A, const HotParams<R> 
         if (A.lerp32)
             inside = sample_brick_numpy32(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th), midpoint(s.ph, b.ph), c);
         else
-            inside = sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
+            inside = sample_brick<R, STORE, LAYOUT>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
                                                           midpoint(s.ph, b.ph), c);
     } else {
-        inside = sample_brick<R, STORE, LAYOUT, COOP>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
+        inside = sample_brick<R, STORE, LAYOUT>(A, rs, gr, gt, gp, midpoint(s.r, b.r), midpoint(s.th, b.th),
                                                       midpoint(s.ph, b.ph), c);
     }
     if (inside) {
@@ -969,19 +1055,154 @@ __device__ __forceinline__ void ray_store(const RenderArgs &A, const RayState<R>
     if (A.status) A.status[s.pix] = uint8_t(code | (s.clamped ? 8 : 0));
 }
 
+// ---- a lone ray in batches of steps ------------------------------------------------------
+// A wave left with one ray (every lane holding a copy) takes its steps in batches: the geodesic runs
+// up to kLoneBatch steps ahead, lane j keeping step j's segment (midpoint, ds, end flags); then every
+// lane samples its own segment's midpoint, so the batch's brick loads are in flight together (one
+// memory round trip per batch instead of one per step); then the transfer and the end tests run over
+// the batch's steps in order, and the first step that ends the ray ends it (geodesic steps computed
+// past it are dropped).  Every value goes through the functions the per-lane step uses, in the same
+// order, so the image is the per-lane image bit for bit.  Batches grow from 8 steps to kLoneBatch as the
+// ray goes on, so a ray that ends soon wastes few geodesic steps.
+constexpr int kLoneBatch = 64;
+constexpr int kLoneStride = 12;   // floats per step in the wave's LDS buffer (11 coefficients, 16-byte rows)
+constexpr int kLoneCheck = 8;     // geodesic steps between looks at the end flags
+
+// One step of a batch, as the lane that keeps it holds it.
+struct AheadStep {
+    float x, y, z, ds;           // midpoint and ds of the segment
+    unsigned flags;              // bit 0 captured, 1 escaped, 2 finite, 3 pole pathology
+    float r, th, ph, pr, pth;    // the ray after the step (KEEP_END)
+};
+
+// Up to nb geodesic steps from (r, th, ph, pr, pth); lane j keeps step j in `mine`.  The end flags are
+// looked at every kLoneCheck steps: a step past the ray's end only computes values that are dropped.
+// Returns the steps taken; (r, th, ph, pr, pth) become the ray after the last one, `ended` whether a
+// step ended the geodesic (captured, escaped or not finite).
+template <bool MIXED, bool RK4, bool KEEP_END>
+__device__ __forceinline__ int geo_ahead(const RenderArgs &A, const HotParams<float> &P, const RayState<float> &s,
+                                         int nb, int lane, float &r, float &th, float &ph, float &pr, float &pth,
+                                         AheadStep &mine, bool &ended) {
+    int n = 0;
+    bool end = false;
+    while (n < nb) {
+        const int stop = n + kLoneCheck < nb ? n + kLoneCheck : nb;
+        for (; n < stop; ++n) {
+            const GeoPoint<float> b = geo_next<float, MIXED, RK4>(A, P, s, r, th, ph, pr, pth);
+            bool captured, escaped;
+            end_tests<float, MIXED>(A, P, r, b.r, captured, escaped);
+            const float x = midpoint(r, b.r), y = midpoint(th, b.th), z = midpoint(ph, b.ph);
+            const float ds = segment_ds_at(P, r, th, ph, b);
+            const unsigned f = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u);
+            if (lane == n) {
+                mine.x = x;
+                mine.y = y;
+                mine.z = z;
+                mine.ds = ds;
+                mine.flags = f;
+                if constexpr (KEEP_END) {
+                    mine.r = b.r;
+                    mine.th = b.th;
+                    mine.ph = b.ph;
+                    mine.pr = b.pr;
+                    mine.pth = b.pth;
+                }
+            }
+            end = end || captured || escaped || !b.fin;
+            r = b.r;
+            th = b.th;
+            ph = b.ph;
+            pr = b.pr;
+            pth = b.pth;
+        }
+        if (__builtin_amdgcn_readfirstlane(int(end))) break;
+    }
+    ended = __builtin_amdgcn_readfirstlane(int(end)) != 0;
+    return n;
+}
+
+// The transfer and the end tests of one step, in the per-lane step's order (flags as AheadStep's, plus
+// bit 4 when the midpoint is in the brick and c holds its coefficients).  Returns the end code, -1 when
+// the ray goes on.
+template <bool EXACT>
+__device__ __forceinline__ int take_step(const HotParams<float> &P, RayState<float> &s, unsigned f, float ds,
+                                         const float c[kCoeffs], unsigned &steps, unsigned &samples) {
+    if (f & 16u) {
+        segment_transfer<float, EXACT>(s, ds, c);
+        ++samples;
+    }
+    s.clamped = s.clamped || (f & 8u);
+    s.steps += 1;
+    ++steps;
+    return end_code(P, s, (f & 1u) != 0, (f & 2u) != 0, (f & 4u) != 0);
+}
+
+// The lone ray for up to `budget` steps; returns its end code, or -1 when the budget ran out first (s then
+// holds the ray after the last step).  cb: this wave's LDS buffer, kLoneBatch x kLoneStride floats.
+template <int STORE, bool MIXED, bool EXACT, bool RK4>
+__device__ __forceinline__ int lone_batched(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
+                                            const float *gr, const float *gt, const float *gp, RayState<float> &s,
+                                            int lane, float *cb, unsigned budget, unsigned &steps, unsigned &samples) {
+    int batch = 8;
+    while (budget > 0) {
+        float r = s.r, th = s.th, ph = s.ph, pr = s.pr, pth = s.pth;
+        AheadStep mine{};
+        const int room = __builtin_amdgcn_readfirstlane(P.max_steps - s.steps);
+        int nb = batch < room ? batch : room;
+        nb = unsigned(nb) < budget ? nb : int(budget);
+        bool ended;
+        const int n = geo_ahead<MIXED, RK4, false>(A, P, s, nb, lane, r, th, ph, pr, pth, mine, ended);
+        // the batch's samples, one per lane
+        float c[kCoeffs];
+        bool inside = false;
+        if (lane < n) inside = sample_brick<float, STORE, BH_LAYOUT_CORNER>(A, rs, gr, gt, gp, mine.x, mine.y, mine.z, c);
+        if (inside) {
+#pragma unroll
+            for (int k = 0; k < kCoeffs; ++k) cb[lane * kLoneStride + k] = c[k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const unsigned mf = mine.flags | (inside ? 16u : 0u);
+        // transfer and end tests in step order; step j + 1's coefficients are read while step j integrates
+        float cn[kCoeffs];
+#pragma unroll
+        for (int k = 0; k < kCoeffs; ++k) cn[k] = cb[k];
+        for (int j = 0; j < n; ++j) {
+            float cj[kCoeffs];
+#pragma unroll
+            for (int k = 0; k < kCoeffs; ++k) cj[k] = cn[k];
+            if (j + 1 < n) {
+#pragma unroll
+                for (int k = 0; k < kCoeffs; ++k) cn[k] = cb[(j + 1) * kLoneStride + k];
+            }
+            const int code = take_step<EXACT>(P, s, __builtin_amdgcn_readlane(mf, j),
+                                              __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                                            __builtin_bit_cast(int, mine.ds), j)),
+                                              cj, steps, samples);
+            if (code >= 0) return code;
+        }
+        __builtin_amdgcn_wave_barrier();   // the next batch rewrites this wave's LDS buffer
+        s.r = r;
+        s.th = th;
+        s.ph = ph;
+        s.pr = pr;
+        s.pth = pth;
+        budget -= unsigned(n);
+        batch = batch * 2 < kLoneBatch ? batch * 2 : kLoneBatch;
+    }
+    return -1;
+}
+
 // ---- a lone ray split over two waves -----------------------------------------------------
 // A ray that stays alone in its wave for coop_steps steps asks for help: a wave of the same
-// workgroup that has no rays of its own takes the geodesic (the RK2 step, the end tests, ds, and the
-// midpoint with its wrapped phi and index guesses) and hands each step to the lone wave through a
-// ring in LDS; the lone wave does the rest: it checks the guesses against the grids, forms the
-// weights, samples the brick, updates Stokes and the optical depth, and decides where the ray ends.
-// It starts the next step's brick loads before it integrates the current step, so the loads are in
-// flight while it works.  A lone step is bound by the issue of its dependent instructions, and the
-// two waves each issue about half of them.  With the exact Stokes step, whose matrix exponential
-// lengthens the lone wave's share, the helper does the whole cell lookup instead.  Every wait is
-// bounded: a helper that stops answering makes the lone wave finish the ray alone from the last step
-// it took (counted in work[4]).
-constexpr int kCoopRing = 32;
+// workgroup that has no rays of its own takes the geodesic and the sampling, in the batches above
+// (geodesic ahead, then every lane sampling its own step), and writes each step into a ring in LDS:
+// the coefficients at the midpoint, ds, the end flags and the ray after the step.  The lone wave takes
+// the steps in order and does the transfer and the end tests, the same functions in the same order as
+// the per-lane step, so the image is the per-lane image bit for bit.  The two waves work at the same
+// time, the helper a batch or two ahead.  Every wait is bounded: a helper that stops answering makes the
+// lone wave finish the ray alone from the last step it took (counted in work[4]).
+constexpr int kCoopRing = 2 * kLoneBatch;
+constexpr int kCoopFirstBatch = 16;
 constexpr unsigned kCoopSpin = 1u << 21;   // ~0.1 s of s_sleep(1) polls
 // The hand-over is one word, phase | requesting wave << 8, so every transition is a single
 // compare-and-swap on the exact value read: Idle -> Asked (lone wave), Asked -> Accepted (helper),
@@ -990,17 +1211,12 @@ constexpr unsigned kCoopSpin = 1u << 21;   // ~0.1 s of s_sleep(1) polls
 enum : int { kCoopIdle = 0, kCoopAsked = 1, kCoopAccepted = 2, kCoopGo = 3, kCoopCancel = 4, kCoopFree = 5 };
 __device__ __forceinline__ int coop_word(int phase, int wave) { return phase | (wave << 8); }
 
-struct alignas(16) CoopSlot {
-    float r, th, ph, pr, pth;   // the ray after the step (restart point if the helper stops answering)
-    unsigned flags;             // bit 0 captured, 1 escaped, 2 finite, 3 pole pathology, 4 midpoint in r/theta range
+struct alignas(16) CoopSample {
+    float c[kCoeffs];            // the coefficients at the step's midpoint (when flags bit 4 is set)
     float ds;
-    // RK2 transfer: the midpoint's r and theta, its wrapped phi and index guesses (with bit 4)
-    float x, y;
-    CellGuess<float> g;
-    // exact transfer, whose Stokes step costs the lone wave more: the whole lookup, done by the helper
-    unsigned cell;              // corner_record of the midpoint's cell, ~0u outside the brick
-    float wr, wt, wp;
-    unsigned pad[3];
+    unsigned flags;              // AheadStep's, plus bit 4: the midpoint is in the brick
+    float r, th, ph, pr, pth;    // the ray after the step (restart point if the helper stops answering)
+    unsigned pad[2];
 };
 
 struct CoopLds {
@@ -1010,28 +1226,12 @@ struct CoopLds {
     unsigned tail;   // steps taken by the lone wave
     int stop;        // the lone wave is done: the helper stops
     RayState<float> ray;
-    CoopSlot ring[kCoopRing];
+    CoopSample ring[kCoopRing];
 };
 
-// A cell's record in the corner layout (8 corners x 11 channels, corner-major): its byte offset in the
-// coefficient buffer, or for wide bricks (whose offsets may pass 32 bits) its index in the brick.
-template <int STORE>
-__device__ __forceinline__ unsigned corner_record(const RenderArgs &A, int r0, int t0, int p0) {
-    const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
-    if constexpr (wide_of(STORE)) return cell;
-    constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
-    return A.brick_base + cell * unsigned(8 * kCoeffs * esz);
-}
-// corner_record of the brick's first cell
-template <int STORE>
-__device__ __forceinline__ unsigned corner_first(const RenderArgs &A) {
-    if constexpr (wide_of(STORE)) return 0u;
-    return A.brick_base;
-}
-
 // Ring indices: a wave's LDS operations execute in order, so a relaxed store of the index after the
-// slot's stores (kept in order by a compiler fence) publishes the slot without waiting for the
-// stores to complete, and a reader that sees the index sees the slot.
+// slots' stores (kept in order by a compiler fence) publishes the slots without waiting for the
+// stores to complete, and a reader that sees the index sees the slots.
 template <typename T>
 __device__ __forceinline__ T lds_peek(T *p) {
     const T v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1092,258 +1292,135 @@ __device__ __forceinline__ T lds_wait(T *p, F pred, bool &timed_out) {
     return v;
 }
 
-// Helper side: the geodesic of the handed-over ray, one ring slot per step, until the geodesic ends the
-// ray, the lone wave says stop, or the ring stays full past the spin bound.
-template <int STORE, bool MIXED, bool EXACT, bool RK4>
-__device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotParams<float> &P, const float *gr,
-                                                 const float *gt, const float *gp, CoopLds *c, int lane) {
+// Helper side: the handed-over ray's geodesic and samples in batches, written to the ring, until the
+// geodesic ends the ray, the step limit, the lone wave says stop, or the ring stays full past the spin
+// bound.  Returns the steps written.
+template <int STORE, bool MIXED, bool RK4>
+__device__ __forceinline__ unsigned coop_produce(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
+                                                 const float *gr, const float *gt, const float *gp, CoopLds *c,
+                                                 int lane) {
     RayState<float> s = c->ray;
-    __builtin_amdgcn_s_setprio(2);   // the lone wave waits on every step of this wave
+    __builtin_amdgcn_s_setprio(2);   // the lone wave waits on every batch of this wave
     unsigned head = 0, tail = 0;
+    int batch = kCoopFirstBatch;
     for (;;) {
-        if (head - tail >= unsigned(kCoopRing)) {   // ring full: wait for the lone wave
+        const int room = __builtin_amdgcn_readfirstlane(P.max_steps - s.steps);
+        const int nb = batch < room ? batch : room;
+        if (nb <= 0) break;
+        if (head + unsigned(nb) - tail > unsigned(kCoopRing)) {   // no room for the batch: wait for the lone wave
             bool timed_out = false;
-            tail = lds_wait_relaxed(&c->tail, [&](unsigned v) { return head - v < unsigned(kCoopRing) || lds_peek(&c->stop); },
-                                    timed_out);
-            if (timed_out || head - tail >= unsigned(kCoopRing)) break;
+            tail = lds_wait_relaxed(
+                &c->tail, [&](unsigned v) { return head + unsigned(nb) - v <= unsigned(kCoopRing) || lds_peek(&c->stop); },
+                timed_out);
+            if (timed_out || head + unsigned(nb) - tail > unsigned(kCoopRing)) break;
         }
-        if ((head & 7u) == 0 && lds_peek(&c->stop)) break;
-        const GeoPoint<float> b = geo_next<float, MIXED, RK4>(A, P, s, s.r, s.th, s.ph, s.pr, s.pth);
-        bool captured, escaped;
-        end_tests<float, MIXED>(A, P, s.r, b.r, captured, escaped);
-        const float x = midpoint(s.r, b.r), y = midpoint(s.th, b.th), z = midpoint(s.ph, b.ph);
-        CellGuess<float> g;
-        CellPos<float> cp;
-        bool in;
-        if constexpr (EXACT) in = locate(A, gr, gt, gp, x, y, z, cp);
-        else in = locate_guess(A, x, y, z, g);
-        const float ds = segment_ds(P, s, b);
-        s.r = b.r;
-        s.th = b.th;
-        s.ph = b.ph;
-        s.pr = b.pr;
-        s.pth = b.pth;
-        s.steps += 1;
-        if (lane == 0) {
-            CoopSlot &sl = c->ring[head & unsigned(kCoopRing - 1)];
-            sl.r = b.r;
-            sl.th = b.th;
-            sl.ph = b.ph;
-            sl.pr = b.pr;
-            sl.pth = b.pth;
-            sl.flags = (captured ? 1u : 0u) | (escaped ? 2u : 0u) | (b.fin ? 4u : 0u) | (b.hit ? 8u : 0u) | (in ? 16u : 0u);
-            sl.ds = ds;
-            if constexpr (EXACT) {
-                sl.cell = in ? corner_record<STORE>(A, cp.r0, cp.t0, cp.p0) : ~0u;
-                sl.wr = cp.wr;
-                sl.wt = cp.wt;
-                sl.wp = cp.wp;
-            } else {
-                sl.x = x;
-                sl.y = y;
-                sl.g = g;
+        if (lds_peek(&c->stop)) break;
+        float r = s.r, th = s.th, ph = s.ph, pr = s.pr, pth = s.pth;
+        AheadStep mine{};
+        bool ended;
+        const int n = geo_ahead<MIXED, RK4, true>(A, P, s, nb, lane, r, th, ph, pr, pth, mine, ended);
+        float cf[kCoeffs];
+        bool inside = false;
+        if (lane < n) inside = sample_brick<float, STORE, BH_LAYOUT_CORNER>(A, rs, gr, gt, gp, mine.x, mine.y, mine.z, cf);
+        if (lane < n) {
+            CoopSample &sl = c->ring[(head + unsigned(lane)) & unsigned(kCoopRing - 1)];
+            if (inside) {
+#pragma unroll
+                for (int k = 0; k < kCoeffs; ++k) sl.c[k] = cf[k];
             }
-            lds_publish(&c->head, head + 1);
+            sl.ds = mine.ds;
+            sl.flags = mine.flags | (inside ? 16u : 0u);
+            sl.r = mine.r;
+            sl.th = mine.th;
+            sl.ph = mine.ph;
+            sl.pr = mine.pr;
+            sl.pth = mine.pth;
         }
-        ++head;
-        if (captured || escaped || !b.fin || s.steps >= P.max_steps) break;   // the geodesic ends the ray here
+        head += unsigned(n);
+        if (lane == 0) lds_publish(&c->head, head);
+        s.r = r;
+        s.th = th;
+        s.ph = ph;
+        s.pr = pr;
+        s.pth = pth;
+        s.steps += n;
+        if (ended) break;   // the geodesic ends the ray within the steps written
+        batch = batch * 2 < kLoneBatch ? batch * 2 : kLoneBatch;
     }
     return head;
 }
 
-// One step taken from the ring by the lone wave: its end point and flags, the midpoint's cell record
-// (corner_record, ~0u outside the brick) and weights, ds, and channel k's 8 corner values.
-struct CoopStep {
-    float r, th, ph, pr, pth;
-    unsigned flags, cell;
-    float wr, wt, wp, ds;
-    float v[8];
-};
-
-// Waits for ring slot `next`, then finishes the lookup of the step's midpoint cell.  False after a
-// time-out.
-template <int STORE, bool EXACT>
-__device__ __forceinline__ bool coop_locate(const RenderArgs &A, const float *gr, const float *gt, const float *gp,
-                                            CoopLds *c, unsigned next, CoopStep &st) {
-    // the slot is read with the index, not after it: a wave's LDS reads execute in order and the helper
-    // writes the slot before the index, so a ready index means the slot read saw the step
-    CoopSlot sl;
-    const unsigned h = lds_peek(&c->head);
-    sl = c->ring[next & unsigned(kCoopRing - 1)];
-    // keeps the compiler from sinking the slot read below the test of the index
-    asm volatile("" ::"v"(sl.r), "v"(sl.th), "v"(sl.ph), "v"(sl.pr), "v"(sl.pth), "v"(sl.flags), "v"(sl.ds));
-    if constexpr (EXACT) asm volatile("" ::"v"(sl.cell), "v"(sl.wr), "v"(sl.wt), "v"(sl.wp));
-    else asm volatile("" ::"v"(sl.x), "v"(sl.y), "v"(sl.g.p), "v"(sl.g.ir), "v"(sl.g.it), "v"(sl.g.ip));
-    if (h == next) {
-        bool timed_out = false;
-        lds_wait_relaxed(&c->head, [next](unsigned v) { return v != next; }, timed_out);
-        if (timed_out) return false;
-        sl = c->ring[next & unsigned(kCoopRing - 1)];
-    }
-    st.r = sl.r;
-    st.th = sl.th;
-    st.ph = sl.ph;
-    st.pr = sl.pr;
-    st.pth = sl.pth;
-    st.flags = sl.flags;
-    st.ds = sl.ds;
-    if constexpr (EXACT) {
-        st.cell = sl.cell;
-        st.wr = sl.wr;
-        st.wt = sl.wt;
-        st.wp = sl.wp;
-        return true;
-    }
-    st.cell = ~0u;
-    if (sl.flags & 16u) {
-        CellPos<float> cp;
-        locate_finish(A, gr, gt, gp, sl.x, sl.y, sl.g, cp);
-        st.cell = corner_record<STORE>(A, cp.r0, cp.t0, cp.p0);
-        st.wr = cp.wr;
-        st.wt = cp.wt;
-        st.wp = cp.wp;
-    }
-    return true;
-}
-
-// Channel k's 8 corner values of cell record `rec` (corner_record).  Every step issues these 8 loads, from
-// the brick's first record when it has no cell to read, so the count of loads in flight is the same on
-// every path and the wait before a step's arithmetic leaves the next step's loads outstanding.
-template <int STORE>
-__device__ __forceinline__ void coop_loads(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, unsigned rec, unsigned k,
-                                           CoopStep &st) {
-    if constexpr (wide_of(STORE)) {
-        const char *p = corner_ptr<STORE>(A, rec);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) st.v[q] = load_coeff_at<STORE>(p, unsigned(q * kCoeffs) + k);
-    } else {
-        constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) st.v[q] = load_coeff<STORE>(rs, rec + (unsigned(q * kCoeffs) + k) * esz, 0);
-    }
-}
-
-// One step of the lone wave: start the next step (slot `next`: lookup and loads), then integrate the
-// current one.  True when the ray ended (code >= 0) or the helper stopped answering (code -1).
-template <int STORE, bool EXACT>
-__device__ __forceinline__ bool coop_step(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
-                                          const float *gr, const float *gt, const float *gp, RayState<float> &s,
-                                          CoopLds *c, int lane, unsigned k, unsigned &next, const CoopStep &cur,
-                                          CoopStep &nxt, unsigned &steps, unsigned &samples, int &code) {
-    if (lane == 0) lds_publish(&c->tail, next);   // slot contents are in registers: the helper may reuse them
-    // the helper writes no slot past the step that ends the geodesic
-    const bool geo_end = (cur.flags & 3u) != 0 || (cur.flags & 4u) == 0 || s.steps + 1u >= P.max_steps;
-    bool have_next = false;
-    if (!geo_end) have_next = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, nxt);
-    coop_loads<STORE>(A, rs, (have_next && nxt.cell != ~0u) ? nxt.cell : corner_first<STORE>(A), k, nxt);
-    s.clamped = s.clamped || (cur.flags & 8u);
-    if (cur.cell != ~0u) {
-        float w8[8];
-        {
-            const float ar = one_minus(cur.wr), at = one_minus(cur.wt), ap = one_minus(cur.wp);
-            const float q00 = ar * at, q01 = ar * cur.wt, q10 = cur.wr * at, q11 = cur.wr * cur.wt;
-            w8[0] = q00 * ap;
-            w8[1] = q00 * cur.wp;
-            w8[2] = q01 * ap;
-            w8[3] = q01 * cur.wp;
-            w8[4] = q10 * ap;
-            w8[5] = q10 * cur.wp;
-            w8[6] = q11 * ap;
-            w8[7] = q11 * cur.wp;
-        }
-        float acc = w8[0] * cur.v[0];
-#pragma unroll
-        for (int q = 1; q < 8; ++q) acc = acc + w8[q] * cur.v[q];
-        float cf[kCoeffs];
-#pragma unroll
-        for (int j = 0; j < kCoeffs; ++j)
-            cf[j] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, acc), j));
-        segment_transfer<float, EXACT>(s, cur.ds, cf);
-        ++samples;
-    }
-    s.r = cur.r;
-    s.th = cur.th;
-    s.ph = cur.ph;
-    s.pr = cur.pr;
-    s.pth = cur.pth;
-    s.steps += 1;
-    ++steps;
-    code = end_code(P, s, (cur.flags & 1u) != 0, (cur.flags & 2u) != 0, (cur.flags & 4u) != 0);
-    if (code >= 0 || !have_next) {   // the ray ended, or the helper stopped answering
-        if (lane == 0) lds_store(&c->stop, 1);
-        return true;
-    }
-    ++next;
-    return false;
-}
-
-// Lone-wave side: take the helper's steps until the ray ends.  Returns the end code, or -1 after a
-// time-out (s then holds the ray after the last step taken, and the lone loop continues alone).  Two
-// step buffers take turns as current and next, so no step's values are copied.
-template <int STORE, bool EXACT>
-__device__ __forceinline__ int coop_consume(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
-                                            const float *gr, const float *gt, const float *gp, RayState<float> &s,
-                                            CoopLds *c, int lane, unsigned &steps, unsigned &samples) {
-    const unsigned k = min(unsigned(lane), unsigned(kCoeffs - 1));
-    unsigned next = 0;   // ring slots taken
-    CoopStep a, b;
-    const bool ok = coop_locate<STORE, EXACT>(A, gr, gt, gp, c, next, a);
-    coop_loads<STORE>(A, rs, (ok && a.cell != ~0u) ? a.cell : corner_first<STORE>(A), k, a);
-    if (!ok) {
-        if (lane == 0) lds_store(&c->stop, 1);
-        return -1;
-    }
-    ++next;
-    int code = -1;
+// Lone-wave side: take the helper's steps in order until the ray ends.  Returns the end code, or -1 after
+// a time-out (s then holds the ray after the last step taken, and the lone wave goes on alone).
+template <bool EXACT>
+__device__ __forceinline__ int coop_consume(const HotParams<float> &P, RayState<float> &s, CoopLds *c, int lane,
+                                            unsigned &steps, unsigned &samples) {
+    unsigned next = 0;   // steps taken
     for (;;) {
-        if (coop_step<STORE, EXACT>(A, P, rs, gr, gt, gp, s, c, lane, k, next, a, b, steps, samples, code)) return code;
-        if (coop_step<STORE, EXACT>(A, P, rs, gr, gt, gp, s, c, lane, k, next, b, a, steps, samples, code)) return code;
+        bool timed_out = false;
+        const unsigned h = lds_wait_relaxed(&c->head, [next](unsigned v) { return v != next; }, timed_out);
+        if (timed_out) {
+            if (lane == 0) lds_store(&c->stop, 1);
+            return -1;
+        }
+        for (; next != h; ++next) {
+            const CoopSample &sl = c->ring[next & unsigned(kCoopRing - 1)];
+            float cf[kCoeffs];
+#pragma unroll
+            for (int k = 0; k < kCoeffs; ++k) cf[k] = sl.c[k];
+            const int code = take_step<EXACT>(P, s, sl.flags, sl.ds, cf, steps, samples);
+            s.r = sl.r;
+            s.th = sl.th;
+            s.ph = sl.ph;
+            s.pr = sl.pr;
+            s.pth = sl.pth;
+            if (code >= 0) {
+                if (lane == 0) lds_store(&c->stop, 1);
+                return code;
+            }
+        }
+        if (lane == 0) lds_publish(&c->tail, next);   // the slots taken may be rewritten
     }
 }
 
-// A wave left with one live ray (FP32 kernels, corner layout): every lane takes a copy of it, the brick sample of
-// each step is split over the lanes (COOP), and the ray runs to its end; then its owner lane writes
-// the result.  The copies stay identical, so the wave leaves the loop together.  With coop != null and
-// coop_steps > 0, a ray still alone after coop_steps steps asks an idle wave of the workgroup to take its
-// geodesic (see above).
+// A wave left with one live ray (FP32 kernels, corner layout): every lane takes a copy of it and the ray runs
+// to its end in batches; then its owner lane writes the result.  With coop != null and coop_steps > 0, a ray
+// still alone after coop_steps steps asks an idle wave of the workgroup to take its geodesic and samples
+// (see above).
 template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4>
 __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R> &P, __amdgpu_buffer_rsrc_t rs,
                                          const R *gr, const R *gt, const R *gp, RayState<R> &s, int lane, int owner,
                                          unsigned &n_steps, unsigned &n_samples, unsigned long long &iters,
-                                         CoopLds *coop, int wave) {
+                                         CoopLds *coop, int wave, float *cb) {
+    static_assert(sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER, "lone rays are FP32, corner layout");
     broadcast_ray(s, owner);
     __builtin_amdgcn_s_setprio(2);
     unsigned steps = 0, samples = 0;
-    int code;
     bool asked = false, helped = false, timed_out = false;
-    bool coop_on = false;
-    if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) coop_on = coop != nullptr && A.coop_steps > 0;
-    for (;;) {
-        // the lone loop proper; with cooperation on, it pauses every few steps for the bookkeeping below
-        const unsigned pause = coop_on ? (asked ? steps + 4u : max(steps + 8u, unsigned(A.coop_steps))) : ~0u;
-        do {
-            code = ray_step<R, STORE, LAYOUT, MIXED, EXACT, RK4, true>(A, P, rs, gr, gt, gp, s, samples);
-            ++steps;
-        } while (code < 0 && steps < pause);
-        if (code >= 0) break;
-        if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
-            if (!asked) {
-                asked = wave_cas(&coop->state, kCoopIdle, coop_word(kCoopAsked, wave), lane);
-            } else if (lds_load(&coop->state) == coop_word(kCoopAccepted, wave)) {
-                if (lane == 0) {
-                    coop->ray = s;
-                    coop->head = 0u;
-                    coop->tail = 0u;
-                    coop->stop = 0;
-                    lds_store(&coop->state, coop_word(kCoopGo, wave));
-                }
-                helped = true;
-                coop_on = false;
-                code = coop_consume<STORE, EXACT>(A, P, rs, gr, gt, gp, s, coop, lane, steps, samples);
-                if (code >= 0) break;
-                timed_out = true;   // finish alone from the last step received
+    const bool coop_on = coop != nullptr && A.coop_steps > 0;
+    // alone until coop_steps steps; then, while the request waits, a few steps at a time
+    int code = lone_batched<STORE, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, lane, cb,
+                                                      coop_on ? unsigned(A.coop_steps) : ~0u, steps, samples);
+    unsigned wait = 8;   // steps between looks at the request, doubling while nobody answers
+    while (code < 0) {
+        if (!asked) {
+            asked = wave_cas(&coop->state, kCoopIdle, coop_word(kCoopAsked, wave), lane);
+        } else if (!timed_out && lds_load(&coop->state) == coop_word(kCoopAccepted, wave)) {
+            if (lane == 0) {
+                coop->ray = s;
+                coop->head = 0u;
+                coop->tail = 0u;
+                coop->stop = 0;
+                lds_store(&coop->state, coop_word(kCoopGo, wave));
             }
+            helped = true;
+            code = coop_consume<EXACT>(P, s, coop, lane, steps, samples);
+            if (code >= 0) break;
+            timed_out = true;   // finish alone from the last step received
         }
+        code = lone_batched<STORE, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, lane, cb, timed_out ? ~0u : wait, steps,
+                                                      samples);
+        wait = wait * 2 < unsigned(kLoneBatch) ? wait * 2 : unsigned(kLoneBatch);
     }
     if (asked && lane == 0) {
         // withdraw the request (nobody accepted it), or cancel it / wait for the helper to leave
@@ -1368,10 +1445,10 @@ __device__ __forceinline__ void lone_ray(const RenderArgs &A, const HotParams<R>
 
 // Helper side of the handshake, for a wave with no rays of its own.  Returns true when it helped (or
 // took a request that was then withdrawn).
-template <int STORE, bool MIXED, bool EXACT, bool RK4>
-__device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<float> &P, const float *gr,
-                                          const float *gt, const float *gp, CoopLds *c, int lane, int wave,
-                                          unsigned long long &iters) {
+template <int STORE, bool MIXED, bool RK4>
+__device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<float> &P, __amdgpu_buffer_rsrc_t rs,
+                                          const float *gr, const float *gt, const float *gp, CoopLds *c, int lane,
+                                          int wave, unsigned long long &iters) {
     const int word = lds_load(&c->state);
     const int requester = word >> 8;
     if ((word & 0xff) != kCoopAsked || requester == wave) return false;
@@ -1379,7 +1456,7 @@ __device__ __forceinline__ bool coop_help(const RenderArgs &A, const HotParams<f
     bool timed_out = false;
     const int go = coop_word(kCoopGo, requester), cancel = coop_word(kCoopCancel, requester);
     const int st = lds_wait(&c->state, [go, cancel](int v) { return v == go || v == cancel; }, timed_out);
-    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED, EXACT, RK4>(A, P, gr, gt, gp, c, lane);
+    if (!timed_out && st == go) iters += coop_produce<STORE, MIXED, RK4>(A, P, rs, gr, gt, gp, c, lane);
     // leave: Go / Cancel -> Free (after a time-out the lone wave sees no Free and stops using the slot)
     if (lane == 0 && !timed_out) lds_store(&c->state, coop_word(kCoopFree, requester));
     return true;
@@ -1401,6 +1478,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
     for (int i = threadIdx.x; i < ng; i += blockDim.x) s_grid[i] = R(A.grids[i]);
     if (threadIdx.x == 0) s_grid[ng] = R(__builtin_huge_valf());   // sentinel g[np] of the phi grid
     __shared__ CoopLds coop;
+    // per wave: a lone ray's batch of sampled coefficients (FP32 kernels)
+    __shared__ __align__(16) float lone_c[sizeof(R) == 4 ? 4 * kLoneBatch * kLoneStride : 1];
     if (threadIdx.x == 0) {
         coop.state = kCoopIdle;
         coop.tracing = int((blockDim.x + 63) / 64);
@@ -1433,7 +1512,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         const int n_idle = __popcll(idle);
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             // a wave with no rays first serves a lone ray of its workgroup that asks for help
-            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED, EXACT, RK4>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+            if (n_idle == 64 && A.coop_steps > 0 && coop_help<STORE, MIXED, RK4>(A, P, rs, gr, gt, gp, &coop, lane, wave, iters)) continue;
         }
         if (exhausted != all_q && (n_idle >= A.refill_min || n_idle == 64)) {
             for (int t = 0; t < A.queues && idle; ++t) {
@@ -1481,7 +1560,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
                     if (A.coop_steps > 0) {
                         unsigned spins = 0;
                         while (lds_peek(&coop.tracing) > 0 && spins < kCoopSpin) {
-                            if (coop_help<STORE, MIXED, EXACT, RK4>(A, P, gr, gt, gp, &coop, lane, wave, iters)) continue;
+                            if (coop_help<STORE, MIXED, RK4>(A, P, rs, gr, gt, gp, &coop, lane, wave, iters)) continue;
                             __builtin_amdgcn_s_sleep(2);
                             ++spins;
                         }
@@ -1494,7 +1573,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {   // no refill happened above: the wave serves its last ray with all lanes
                 lone_ray<R, STORE, LAYOUT, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
-                                                  n_steps, n_samples, iters, &coop, wave);
+                                                  n_steps, n_samples, iters, &coop, wave,
+                                                  lone_c + (sizeof(R) == 4 ? wave * kLoneBatch * kLoneStride : 0));
                 active = false;
                 continue;
             }
@@ -1557,6 +1637,8 @@ template <typename R, int STORE, int LAYOUT, bool MIXED, bool EXACT, bool RK4, b
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_WAVES_PER_EU))) void batch_kernel(
     BatchArgs B) {
     __shared__ R s_grid[kMaxGrid + 8];
+    // per wave: a lone ray's batch of sampled coefficients (FP32 kernels)
+    __shared__ __align__(16) float lone_c[sizeof(R) == 4 ? 4 * kLoneBatch * kLoneStride : 1];
     for (int i = threadIdx.x; i < B.ng; i += blockDim.x) s_grid[i] = R(B.grids[i]);
     __syncthreads();
     const __amdgpu_buffer_rsrc_t rs =
@@ -1638,7 +1720,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BH_RENDER_W
         if constexpr (sizeof(R) == 4 && LAYOUT == BH_LAYOUT_CORNER) {
             if (live == 1) {
                 lone_ray<R, STORE, LAYOUT, MIXED, EXACT, RK4>(A, P, rs, gr, gt, gp, s, lane, int(__builtin_ctzll(live_mask)),
-                                                  n_steps, n_samples, iters, nullptr, 0);
+                                                  n_steps, n_samples, iters, nullptr, 0,
+                                                  lone_c + (sizeof(R) == 4 ? int(threadIdx.x >> 6) * kLoneBatch * kLoneStride : 0));
                 active = false;
                 continue;
             }

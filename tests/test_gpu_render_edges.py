@@ -27,8 +27,7 @@ from oracle import render as orender
 
 pytestmark = pytest.mark.gpu
 
-# coop_steps=0: the lone-ray path without the two-wave split (the split is checked against it below,
-# to rounding, because its brick weights reach the sample through LDS)
+# coop_steps=0: the lone-ray path without the two-wave split (the split is checked against it below)
 TUNINGS = [
     dict(block_threads=64, blocks_per_cu=1, steps_per_sync=1, refill_min=1, queues=1, order=0, coop_steps=0),
     dict(block_threads=256, blocks_per_cu=0, steps_per_sync=32, refill_min=64, queues=8, order=1, coop_steps=0),
@@ -63,9 +62,9 @@ def test_tuning_does_not_change_the_image(renderer, storage, layout, precision):
                                                          ("bf16", PREC_MIXED, 1024, False), ("f32", PREC_FP32, 1024, True),
                                                          ("f16", PREC_MIXED, 512, True)])
 def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size, exact):
-    """A lone ray whose geodesic an idle wave takes over (coop_steps > 0) ends on the same step with the same
-    status as the lone-wave path, and its Stokes vector agrees to rounding.  With the exact Stokes step the
-    helper also does the whole cell lookup."""
+    """A lone ray whose geodesic and samples an idle wave takes over (coop_steps > 0) gives the image of the
+    lone-wave path bit for bit: statuses, steps and IQUV equal on every pixel (both paths run the same step
+    arithmetic, which rounds the same wherever it is inlined)."""
     sc, br = scenes.scale_plain_scene(size, size), scenes.scale_brick()
     sc = replace(sc, exact_step=exact)
     db = renderer.upload(br, storage, "corner")
@@ -78,12 +77,11 @@ def test_split_lone_rays_match_lone_rays(renderer, storage, precision, size, exa
             renderer.set_tuning(precision, coop_steps=cs)
             out = renderer.render(sc, db, precision)
             split, timeouts = renderer.last_coop()
-            d = np.abs(out["iquv"].astype(np.float64) - base["iquv"]) / np.abs(base["iquv"][..., 0]).max()
-            print(f"{storage} {size} exact={exact}: coop_steps {cs}: {split} split rays, {timeouts} time-outs, max |dIQUV| / max I {d.max():.1e}")
+            n_diff = int((out["iquv"] != base["iquv"]).any(axis=-1).sum())
+            print(f"{storage} {size} exact={exact}: coop_steps {cs}: {split} split rays, {timeouts} time-outs, "
+                  f"{n_diff} pixels differ")
             assert split > 0 and timeouts == 0
-            np.testing.assert_array_equal(out["status"], base["status"])
-            np.testing.assert_array_equal(out["steps"], base["steps"])
-            assert d.max() <= 1e-6
+            _same(out, base)
     finally:
         renderer.set_tuning(precision, **saved)
         renderer.release(db)
@@ -105,10 +103,7 @@ def test_split_with_small_workgroups(renderer, block_threads):
         print(f"block_threads {block_threads}: {split} split rays, {timeouts} time-outs")
         assert timeouts == 0
         assert split == 0 if block_threads == 64 else split > 0
-        np.testing.assert_array_equal(out["status"], base["status"])
-        np.testing.assert_array_equal(out["steps"], base["steps"])
-        d = np.abs(out["iquv"].astype(np.float64) - base["iquv"]) / np.abs(base["iquv"][..., 0]).max()
-        assert d.max() <= 1e-6
+        _same(out, base)
     finally:
         renderer.set_tuning(PREC_FP32, **saved)
         renderer.release(db)
