@@ -42,7 +42,7 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed); scalar issue caps at
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the plain FP32 kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled x2.0 / x1.0 by a 256 MiB copy); profiles/render_kerr_stokes_1024.md
-TRAFFIC_PLAIN_FP32 = 1.818e9 + 28.0e6
+TRAFFIC_PLAIN_FP32 = 1.833e9 + 27.7e6
 
 
 def log(msg: str) -> None:
