@@ -488,6 +488,18 @@ __device__ __forceinline__ float half_hi(unsigned w, int store) {
                                   : float(__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)));
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 weights as bf16 pairs, a head and a tail: hi = bf16(w) (round to nearest even) and
+// lo = bf16(w - hi), where w - hi is exact, so hi + lo holds w to ~2^-17 relative.
+__device__ __forceinline__ void split_bf16_pair(float a, float b, bf16x2 &hi, bf16x2 &lo) {
+    hi = __builtin_convertvector((f32x2){a, b}, bf16x2);
+    const unsigned u = __builtin_bit_cast(unsigned, hi);
+    lo = __builtin_convertvector((f32x2){a - __builtin_bit_cast(float, u << 16), b - __builtin_bit_cast(float, u & 0xffff0000u)},
+                                 bf16x2);
+}
+
 template <typename R, int STORE, int LAYOUT>
 __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffer_rsrc_t rs, const R *gr, const R *gt,
                                              const R *gp, R x, R y, R z, R out[kCoeffs]) {
@@ -497,7 +509,7 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
     const R wr = cp.wr, wt = cp.wt, wp = cp.wp;
     constexpr unsigned esz = store_of(STORE) == BH_STORE_F32 ? 4u : 2u;
     if constexpr (LAYOUT == BH_LAYOUT_CORNER) {
-        // one cell record: corners (dr, dt, dp) in order dr*4 + dt*2 + dp, 11 channels each
+        // one cell record: corners (dr, dt, dp) in order dr*4 + dt*2 + dp, 11 channels each (bf16: channel-major)
         const unsigned cell = (unsigned(r0) * unsigned(A.nt - 1) + unsigned(t0)) * unsigned(A.np) + unsigned(p0);
         constexpr int kWords = 8 * kCoeffs * int(esz) / 4;   // 88 (f32) or 44 (16-bit)
         constexpr int kVec = kWords / 4;                      // 22 or 11 loads of 16 bytes
@@ -521,6 +533,26 @@ __device__ __forceinline__ bool sample_brick(const RenderArgs &A, __amdgpu_buffe
         const R ar = one_minus(wr), at = one_minus(wt), ap = one_minus(wp);
         const R q00 = ar * at, q01 = ar * wt, q10 = wr * at, q11 = wr * wt;
         const R w8[8] = {q00 * ap, q00 * wp, q01 * ap, q01 * wp, q10 * ap, q10 * wp, q11 * ap, q11 * wp};
+        if constexpr (store_of(STORE) == BH_STORE_BF16 && sizeof(R) == 4) {
+            // bf16 records are channel-major: load k holds channel k's 8 corners, corners 2m and 2m + 1 in
+            // word m, so one v_dot2_f32_bf16 takes a corner pair against a pair of split weights (tails
+            // first, then heads): 8 dot products per channel instead of 8 unpacks and 8 FMAs
+            bf16x2 hi[4], lo[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) split_bf16_pair(w8[2 * m], w8[2 * m + 1], hi[m], lo[m]);
+#pragma unroll
+            for (int k = 0; k < kCoeffs; ++k) {
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w[4 * k + m]), lo[m], acc, false);
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w[4 * k + m]), hi[m], acc, false);
+                out[k] = acc;
+            }
+            return true;
+        }
 #pragma unroll
         for (int k = 0; k < kCoeffs; ++k) {
             R acc = R(0);
@@ -1949,17 +1981,18 @@ static void transpose_soa(const float *aos, int nr, int nt, int np, T *soa, T (*
         for (int k = 0; k < kCoeffs; ++k) soa[size_t(k) * plane + e] = cvt(aos[e * kCoeffs + k]);
 }
 
-// Corner records: cell (ir, it, ip) holds corners (ir+dr, it+dt, (ip+dp) mod np) in order dr*4 + dt*2 + dp.
+// Corner records: cell (ir, it, ip) holds corners (ir+dr, it+dt, (ip+dp) mod np), corner q = dr*4 + dt*2 + dp,
+// corner-major (value (q, k) at q * 11 + k) or, for bf16 records, channel-major (at k * 8 + q).
 template <typename T>
-static void pack_corners(const float *aos, int nr, int nt, int np, T *out, T (*cvt)(float)) {
+static void pack_corners(const float *aos, int nr, int nt, int np, T *out, T (*cvt)(float), bool channel_major) {
     size_t o = 0;
     for (int ir = 0; ir < nr - 1; ++ir)
         for (int it = 0; it < nt - 1; ++it)
-            for (int ip = 0; ip < np; ++ip)
+            for (int ip = 0; ip < np; ++ip, o += 8 * kCoeffs)
                 for (int q = 0; q < 8; ++q) {
                     const int jr = ir + (q >> 2), jt = it + ((q >> 1) & 1), jp = (ip + (q & 1)) % np;
                     const float *src = aos + ((size_t(jr) * nt + jt) * np + jp) * kCoeffs;
-                    for (int k = 0; k < kCoeffs; ++k) out[o++] = cvt(src[k]);
+                    for (int k = 0; k < kCoeffs; ++k) out[o + (channel_major ? k * 8 + q : q * kCoeffs + k)] = cvt(src[k]);
                 }
 }
 
@@ -2243,9 +2276,9 @@ int bh_brick_upload(bh_ctx *ctx, const float *coeffs_aos, const float *r_grid, i
         else if (storage == BH_STORE_BF16) transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16);
         else transpose_soa<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16);
     } else {
-        if (storage == BH_STORE_F32) pack_corners<float>(coeffs_aos, nr, nt, np, reinterpret_cast<float *>(host.data()), cvt_f32);
-        else if (storage == BH_STORE_BF16) pack_corners<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16);
-        else pack_corners<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16);
+        if (storage == BH_STORE_F32) pack_corners<float>(coeffs_aos, nr, nt, np, reinterpret_cast<float *>(host.data()), cvt_f32, false);
+        else if (storage == BH_STORE_BF16) pack_corners<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_bf16, true);
+        else pack_corners<uint16_t>(coeffs_aos, nr, nt, np, reinterpret_cast<uint16_t *>(host.data()), cvt_f16, false);
     }
     BrickDev b;
     b.storage = storage;

@@ -119,7 +119,8 @@ int bh_sample_brick_trilinear_device(bh_ctx *ctx, const double *d_coeffs,
  *  BH_LAYOUT_SOA     channel planes [11][nr][nt][np];
  *  BH_LAYOUT_CORNER  one record per trilinear cell [nr-1][nt-1][np] holding the
  *                    cell's 8 corners x 11 channels contiguously (phi seam cell
- *                    included): 8x the bytes, one contiguous read per sample. */
+ *                    included; corner-major, channel-major for BH_STORE_BF16):
+ *                    8x the bytes, one contiguous read per sample. */
 #define BH_LAYOUT_SOA     0
 #define BH_LAYOUT_CORNER  1
 
