@@ -22,13 +22,13 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 64;     // output tile edge per workgroup
 constexpr int kMaxLayersAdam = 20;   // at least depth + 1 for the largest depth bh_mlp_create accepts
-// batch rows per split-K slice of a weight-gradient GEMM (measured at batch 8192: 256 -> 252 us per Adam step,
-// 512 -> 262, 1024 -> 330; fewer slices leave CUs idle)
+// batch rows per split-K slice of a weight-gradient GEMM (fewer slices leave CUs idle), and at most this many
+// slices (their partial results are summed by weight_reduce_kernel)
 constexpr int kDwRows = 256;
+constexpr int kDwMaxSplits = 32;
 constexpr int kKStep = 32;    // K per MFMA pass
-constexpr int kLdsK = kKStep + 8;
 
-enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3, EPI_ATOMIC = 4, EPI_MSE = 5 };
+enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3, EPI_MSE = 5 };
 
 struct GemmArgs {
     const __bf16 *A;   // op(A): M x K; stored [M][K] (A_KC) or [K][M]
@@ -42,6 +42,7 @@ struct GemmArgs {
     float *Cf;         // FP32 result (optional)
     __bf16 *Cb;        // BF16 result (optional)
     int ldc;
+    long long z_stride;   // elements between the results of successive grid.z slices (split K) in Cf
     int c_trans;       // Cf stored [n][m] (ld = ldc) instead of [m][n]
     int k_split;       // K range of one grid.z slice (multiple of kKStep)
     float *colpart;    // EPI_DSILU, EPI_MSE: each row tile's column sums of the result, [blockIdx.y][N]
@@ -60,32 +61,52 @@ __device__ __forceinline__ float dsilu(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
-// Stage a 64 x 32 (rows x k) operand tile into LDS as [row][k].
+// A 64 x kStageK (rows x k) operand tile moves through registers into LDS as [row][k]: every thread loads
+// kStageLoads 16-byte pieces (all in flight together), then stores them.  Zeros past the operand's rows
+// and past the slice's K range.
+constexpr int kStageK = 128;
+constexpr int kLdsStage = kStageK + 8;
+constexpr int kStageLoads = kStageK / kKStep;
+
 template <bool KC>
-__device__ __forceinline__ void stage(const __bf16 *src, int ld, int rows, int K, int r0, int k0,
-                                      __bf16 (*dst)[kLdsK], int tid) {
-    if constexpr (KC) {   // stored [row][k]
-        const int r = tid >> 2, kk = (tid & 3) * 8;
-        uint4 v = {0u, 0u, 0u, 0u};
-        if (r0 + r < rows && k0 + kk < K) v = *reinterpret_cast<const uint4 *>(src + size_t(r0 + r) * ld + k0 + kk);
-        *reinterpret_cast<uint4 *>(&dst[r][kk]) = v;
-    } else {              // stored [k][row]
-        const int k = tid >> 3, rr = (tid & 7) * 8;
-        uint4 v = {0u, 0u, 0u, 0u};
-        if (k0 + k < K && r0 + rr < rows) v = *reinterpret_cast<const uint4 *>(src + size_t(k0 + k) * ld + r0 + rr);
-        const unsigned short *e = reinterpret_cast<const unsigned short *>(&v);
+__device__ __forceinline__ void load_stage(const __bf16 *src, int ld, int rows, int K, int r0, int k0, uint4 (&v)[kStageLoads],
+                                           int tid) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) *reinterpret_cast<unsigned short *>(&dst[rr + i][k]) = e[i];
+    for (int s = 0; s < kStageLoads; ++s) {
+        v[s] = uint4{0u, 0u, 0u, 0u};
+        if constexpr (KC) {   // stored [row][k]
+            const int r = tid >> 2, kk = s * kKStep + (tid & 3) * 8;
+            if (r0 + r < rows && k0 + kk < K) v[s] = *reinterpret_cast<const uint4 *>(src + size_t(r0 + r) * ld + k0 + kk);
+        } else {              // stored [k][row]
+            const int k = s * kKStep + (tid >> 3), rr = (tid & 7) * 8;
+            if (k0 + k < K && r0 + rr < rows) v[s] = *reinterpret_cast<const uint4 *>(src + size_t(k0 + k) * ld + r0 + rr);
+        }
     }
 }
 
-// C tile (64 x 64) = op(A) op(B) over this slice's K range.  Four waves, each a 32 x 32 quadrant of
-// 2 x 2 MFMA 16x16x32 tiles.  Lane l of an MFMA holds A[l%16][(l/16)*8 .. +7] and B[(l/16)*8 .. +7][l%16]
-// and receives D[(l/16)*4 + q][l%16], q = 0..3.
+template <bool KC>
+__device__ __forceinline__ void store_stage(const uint4 (&v)[kStageLoads], __bf16 (*dst)[kLdsStage], int tid) {
+#pragma unroll
+    for (int s = 0; s < kStageLoads; ++s) {
+        if constexpr (KC) {
+            *reinterpret_cast<uint4 *>(&dst[tid >> 2][s * kKStep + (tid & 3) * 8]) = v[s];
+        } else {
+            const int k = s * kKStep + (tid >> 3), rr = (tid & 7) * 8;
+            const unsigned short *e = reinterpret_cast<const unsigned short *>(&v[s]);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) *reinterpret_cast<unsigned short *>(&dst[rr + i][k]) = e[i];
+        }
+    }
+}
+
+// C tile (64 x 64) = op(A) op(B) over this slice's K range, in stages of kStageK: the next stage's loads
+// are in flight while the current stage's MFMAs run.  Four waves, each a 32 x 32 quadrant of 2 x 2 MFMA
+// 16x16x32 tiles, the k-steps in order (so the sums do not depend on the staging).  Lane l of an MFMA
+// holds A[l%16][(l/16)*8 .. +7] and B[(l/16)*8 .. +7][l%16] and receives D[(l/16)*4 + q][l%16], q = 0..3.
 template <bool A_KC, bool B_KC, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    __shared__ __align__(16) __bf16 sA[kTile][kLdsK];
-    __shared__ __align__(16) __bf16 sB[kTile][kLdsK];
+    __shared__ __align__(16) __bf16 sA[kTile][kLdsStage];
+    __shared__ __align__(16) __bf16 sB[kTile][kLdsStage];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
@@ -98,19 +119,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int kb = (lane >> 4) * 8, rl = lane & 15;
-    for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
-        stage<A_KC>(g.A, g.lda, g.M, kend, m0, k0, sA, tid);
-        stage<B_KC>(g.B, g.ldb, g.N, kend, n0, k0, sB, tid);
+    uint4 va[kStageLoads], vb[kStageLoads];
+    load_stage<A_KC>(g.A, g.lda, g.M, kend, m0, kbeg, va, tid);
+    load_stage<B_KC>(g.B, g.ldb, g.N, kend, n0, kbeg, vb, tid);
+    for (int k0 = kbeg; k0 < kend; k0 += kStageK) {
+        store_stage<A_KC>(va, sA, tid);
+        store_stage<B_KC>(vb, sB, tid);
         __syncthreads();
-        bf16x8 a[2], b[2];
+        if (k0 + kStageK < kend) {
+            load_stage<A_KC>(g.A, g.lda, g.M, kend, m0, k0 + kStageK, va, tid);
+            load_stage<B_KC>(g.B, g.ldb, g.N, kend, n0, k0 + kStageK, vb, tid);
+        }
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + rl][kb]);
+        for (int ks = 0; ks < kStageLoads; ++ks) {
+            if (k0 + ks * kKStep >= kend) break;
+            bf16x8 a[2], b[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8 *>(&sB[wn + j * 16 + rl][kb]);
+            for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + rl][ks * kKStep + kb]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8 *>(&sB[wn + j * 16 + rl][ks * kKStep + kb]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
     float csum[2] = {0.f, 0.f};   // EPI_DSILU / EPI_MSE: this lane's part of its two columns' sums
@@ -147,12 +180,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                     csum[j] += v;
                 }
                 const size_t ci = g.c_trans ? size_t(n) * g.ldc + m : size_t(m) * g.ldc + n;
-                if constexpr (EPI == EPI_ATOMIC) {
-                    atomicAdd(&g.Cf[ci], v);
-                } else {
-                    if (g.Cf) g.Cf[ci] = v;
-                    if (g.Cb) g.Cb[ci] = __bf16(v);
-                }
+                if (g.Cf) g.Cf[size_t(blockIdx.z) * size_t(g.z_stride) + ci] = v;
+                if (g.Cb) g.Cb[ci] = __bf16(v);
             }
     if constexpr (EPI == EPI_DSILU || EPI == EPI_MSE) {
         // the four lanes of a column (lane / 16) hold its 4 x 2 x 4 values of this wave's 32 rows; the two
@@ -218,6 +247,26 @@ __global__ __launch_bounds__(1024) void bias_reduce_kernel(const float *part, Bi
         for (int k = 0; k < 16; ++k) tot += acc[k][lane];
         grad[B.b_off[l] + i] = tot;
     }
+}
+
+// Where each layer's weight gradient lies: w_off[l] in the parameter vector, n[l] = fi * fo entries, and its
+// split-K slices' partial results at part_off[l] ([splits][n[l]]).
+struct WeightMap {
+    int layers, splits;
+    long long w_off[kMaxLayersAdam], part_off[kMaxLayersAdam], n[kMaxLayersAdam];
+};
+
+// Weight gradients: grad[w_off[l] + i] = the sum of the slices' partial results, added in slice order, so the
+// gradient is the same on every run (split-K with atomics would add them in arrival order).
+__global__ __launch_bounds__(256) void weight_reduce_kernel(const float *part, WeightMap W, float *grad) {
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    int l = 0;
+    while (l < W.layers && i >= W.n[l]) i -= W.n[l++];
+    if (l >= W.layers) return;
+    const float *p = part + W.part_off[l] + i;
+    float s = p[0];
+    for (int z = 1; z < W.splits; ++z) s += p[size_t(z) * size_t(W.n[l])];
+    grad[W.w_off[l] + i] = s;
 }
 
 // Sum of n floats into *out, by one workgroup (the loss from the output GEMM's per-workgroup parts).
@@ -543,6 +592,8 @@ struct bh_mlp {
     float *loss_part = nullptr;                // per-workgroup loss of the output GEMM (one per 64 x 64 tile)
     int loss_parts = 0;
     float *bias_part = nullptr;                // per row tile column sums of each layer's gradient [tiles][fo]
+    float *dw_part = nullptr;                  // split-K partial weight gradients, per layer [splits][fi * fo]
+    int dw_splits = 0;                         // slices the buffer holds
     int n_bias = 0;                            // biases in all (sum of fo)
     std::vector<int> fo_prefix;                // biases of the layers before l
     bool fused = true;                         // inference through mlp_fused_kernel when the shape allows
@@ -564,14 +615,28 @@ static int run_gemm(bh_ctx *ctx, const GemmArgs &g0, int splits = 1) {
 
 static unsigned blocks_for(size_t n) { return unsigned((n + 255) / 256); }
 
+// Split-K slices of the weight gradients for a batch of `rows` (monotonic in rows).
+static int dw_splits_for(int rows) {
+    if (rows < 1024) return 1;
+    return rows / kDwRows < kDwMaxSplits ? rows / kDwRows : kDwMaxSplits;
+}
+
+static size_t weight_count(const bh_mlp *m) {
+    size_t n = 0;
+    for (int l = 0; l <= m->depth; ++l) n += size_t(m->fi[l]) * m->fo[l];
+    return n;
+}
+
 static void free_acts(bh_mlp *m) {
     if (m->xb) (void)hipFree(m->xb);
     for (auto p : m->act) (void)hipFree(p);
     for (auto p : m->zpre)
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)m->dyb, (void *)m->dab, (void *)m->dab2, (void *)m->loss_part, (void *)m->bias_part})
+    for (void *p : {(void *)m->dyb, (void *)m->dab, (void *)m->dab2, (void *)m->loss_part, (void *)m->bias_part,
+                    (void *)m->dw_part})
         if (p) (void)hipFree(p);
-    m->loss_part = m->bias_part = nullptr;
+    m->loss_part = m->bias_part = m->dw_part = nullptr;
+    m->dw_splits = 0;
     m->loss_parts = 0;
     m->xb = nullptr;
     m->act.clear();
@@ -603,8 +668,12 @@ static int ensure_cap(bh_mlp *m, int rows, bool train) {
         ok = hipMalloc(&m->dyb, R * m->n_out_pad * 2) == hipSuccess && hipMalloc(&m->dab, R * widest * 2) == hipSuccess &&
              hipMalloc(&m->dab2, R * widest * 2) == hipSuccess &&
              hipMalloc(&m->loss_part, size_t(parts) * 4) == hipSuccess &&
-             hipMalloc(&m->bias_part, ((R + kTile - 1) / kTile) * size_t(m->n_bias) * 4) == hipSuccess;
-    if (ok && train) m->loss_parts = parts;
+             hipMalloc(&m->bias_part, ((R + kTile - 1) / kTile) * size_t(m->n_bias) * 4) == hipSuccess &&
+             hipMalloc(&m->dw_part, size_t(dw_splits_for(rows)) * weight_count(m) * 4) == hipSuccess;
+    if (ok && train) {
+        m->loss_parts = parts;
+        m->dw_splits = dw_splits_for(rows);
+    }
     if (!ok) {
         free_acts(m);
         return set_error(BH_ERR_MEMORY, "MLP activation allocation for %d rows failed", rows);
@@ -686,7 +755,6 @@ static int loss_total(bh_mlp *m, int rows) {
 // layer's pre-activation gradient, formed in the epilogue of the GEMM that produces it.
 static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, const float *d_rw, int masked) {
     bh_ctx *ctx = m->ctx;
-    BH_HIP(hipMemsetAsync(m->grad, 0, m->n_params * 4, ctx->stream));
     int st;
     {   // y = act W + b, then dL/dy (BF16), its column sums (output bias gradient) and the loss
         const int l = m->depth;
@@ -711,7 +779,20 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         if ((st = run_gemm<true, true, EPI_MSE>(ctx, g))) return st;
     }
     const __bf16 *dcur = m->dyb;   // gradient w.r.t. this layer's pre-activation, BF16
-    const int splits = rows >= 1024 ? rows / kDwRows : 1;
+    // weight gradients in split-K slices of the batch rows (as run_gemm cuts them), summed afterwards
+    const int splits = dw_splits_for(rows);
+    const int k_split = ((rows + splits - 1) / splits + kKStep - 1) / kKStep * kKStep;
+    WeightMap W{};
+    W.layers = m->depth + 1;
+    W.splits = (rows + k_split - 1) / k_split;
+    if (W.splits > m->dw_splits) return set_error(BH_ERR_STATE, "weight-gradient slices exceed their buffer");
+    long long part_off = 0;
+    for (int l = 0; l <= m->depth; ++l) {
+        W.w_off[l] = (long long)m->w_off[l];
+        W.n[l] = (long long)m->fi[l] * m->fo[l];
+        W.part_off[l] = part_off;
+        part_off += W.n[l] * m->dw_splits;
+    }
     for (int l = m->depth; l >= 0; --l) {
         const int K = m->fi[l], N = m->fo[l];
         const __bf16 *xin = l == 0 ? m->xb : m->act[l - 1];
@@ -724,9 +805,10 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         gw.M = K;
         gw.N = N;
         gw.K = rows;
-        gw.Cf = m->grad + m->w_off[l];
+        gw.Cf = m->dw_part + W.part_off[l];
         gw.ldc = N;
-        if ((st = run_gemm<false, false, EPI_ATOMIC>(ctx, gw, splits))) return st;
+        gw.z_stride = W.n[l];
+        if ((st = run_gemm<false, false, EPI_STORE>(ctx, gw, splits))) return st;
         if (l == 0) break;
         // dZ_prev = (dZ W^T) * silu'(z_prev)
         __bf16 *dnext = (dcur == m->dab) ? m->dab2 : m->dab;
@@ -746,6 +828,8 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         if ((st = run_gemm<true, true, EPI_DSILU>(ctx, gd))) return st;
         dcur = dnext;
     }
+    hipLaunchKernelGGL(weight_reduce_kernel, dim3(blocks_for(weight_count(m))), dim3(256), 0, ctx->stream, m->dw_part, W,
+                       m->grad);
     BiasMap B{};
     B.layers = m->depth + 1;
     B.tiles = (rows + kTile - 1) / kTile;
