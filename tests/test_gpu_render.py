@@ -207,31 +207,33 @@ def test_corner_layout_matches_soa(renderer, storage, precision):
     assert rel_l1(ob["iquv"][..., 0][ok].astype(np.float64), oa["iquv"][..., 0][ok].astype(np.float64)) <= 1e-5
 
 
-def test_corner_brick_beyond_4_gib():
-    """Corner records are read through 64-bit addresses: a 256 x 192 x 256 FP32 brick (4.39 GB of corner
-    records, past what a 32-bit buffer offset reaches) renders like its SoA copy (0.55 GB).  Rays from
-    the camera enter the brick through its outermost radial cells, whose records lie beyond 4 GiB."""
+@pytest.mark.parametrize("storage,precision", [("f32", PREC_FP32), ("bf16", PREC_MIXED)])
+def test_corner_brick_beyond_4_gib(storage, precision):
+    """Corner records of 2 GiB and more are read through 64-bit addresses: a 256 x 192 x 256 brick (4.39 GB
+    of FP32 corner records, past what a 32-bit buffer offset reaches; 2.19 GB of bf16 records, channel-major)
+    renders like its SoA copy.  Rays from the camera enter the brick through its outermost radial cells,
+    whose records lie at the end of the buffer."""
     from bhgrrt.render import Renderer
 
-    print("corner brick > 4 GiB: building 256x192x256", flush=True)
+    print(f"corner brick beyond 2 GiB ({storage}): building 256x192x256", flush=True)
     br = scenes.make_brick(256, 192, 256, spin=0.94, seed=7, disk=True)
     cells = 255 * 191 * 256
-    assert cells * 8 * 11 * 4 > (1 << 32)
+    assert cells * 8 * 11 * (4 if storage == "f32" else 2) > (1 << 31)
     sc = scenes.scale_plain_scene(128, 128)
     rr = Renderer(0)
     try:
-        a = rr.upload(br, "f32", "soa")
-        b = rr.upload(br, "f32", "corner")
-        print("corner brick > 4 GiB: uploaded, rendering", flush=True)
-        oa = rr.render(sc, a, PREC_FP32)
-        ob = rr.render(sc, b, PREC_FP32)
+        a = rr.upload(br, storage, "soa")
+        b = rr.upload(br, storage, "corner")
+        print(f"corner brick beyond 2 GiB ({storage}): uploaded, rendering", flush=True)
+        oa = rr.render(sc, a, precision)
+        ob = rr.render(sc, b, precision)
         _, samples = rr.last_work()
     finally:
         rr.close()
     ok = ((oa["status"] | ob["status"]) & orender.ST_POLE_CLAMPED) == 0
     l1 = rel_l1(ob["iquv"][..., 0][ok].astype(np.float64), oa["iquv"][..., 0][ok].astype(np.float64))
-    print(f"corner brick > 4 GiB: {samples} samples, statuses equal on {np.mean(oa['status'] == ob['status']):.4f}, "
-          f"rel L1 I {l1:.2e}")
+    print(f"corner brick beyond 2 GiB ({storage}): {samples} samples, statuses equal on "
+          f"{np.mean(oa['status'] == ob['status']):.4f}, rel L1 I {l1:.2e}")
     assert samples > 0
     assert np.mean(oa["status"] == ob["status"]) >= 0.995
     assert l1 <= 1e-5
