@@ -193,7 +193,8 @@ typedef struct bh_tuning {
     int32_t queues;           /* 1, or 8 for one ray queue per XCD */
     int32_t order;            /* 0: row-major tiles; 1: tiles nearest the critical curve first */
     int32_t coop_steps;       /* render kernels: a ray alone in its wave for this many steps asks an idle
-                                 wave of its workgroup to take its geodesic (0: never; -1 in set: keep) */
+                                 wave of its workgroup to take its geodesic and brick samples (0: never;
+                                 -1 in set: keep) */
 } bh_tuning;
 
 int bh_ctx_set_tuning(bh_ctx *ctx, int32_t precision, const bh_tuning *t);
