@@ -230,19 +230,16 @@ __device__ __forceinline__ double island_rcp(double x) {
     return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
 }
 
-// d/dlambda (r, theta, phi, p_r, p_theta) of the Kerr null Hamiltonian.  ISLAND: FP64 inside the islands
-// of a mixed kernel, with the island math above.
-// The arithmetic of the right-hand side and of the steps below is written with explicit fused
-// multiply-adds and compiled with contraction off, so that a step rounds the same in every code path that
-// takes it (the ray's own lane, a lone ray's batch, the helper of a split lone ray): left to the
-// compiler, the fusion of a * b + c depends on the code the step is inlined into.
-template <typename R, bool ISLAND = false>
+// d/dlambda (r, theta, phi, p_r, p_theta) of the Kerr null Hamiltonian, as the FP64 reference kernels take it:
+// every step in the ray's own lane, library math, the compiler's multiply-add fusion (the form their parity
+// with the oracle is measured on).
+template <typename R>
 __device__ __forceinline__ void kerr_rhs_free(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
-                                         R &dpth) {
+                                              R &dpth) {
+    static_assert(sizeof(R) == 8, "the FP32 kernels and the FP64 islands take kerr_rhs_pinned");
     const R th = clamp_theta(th_in);
     R s, c;
-    if constexpr (ISLAND) island_sincos(th, s, c);
-    else Mth<R>::sincos(th, s, c);
+    Mth<R>::sincos(th, s, c);
     R s2 = s * s;
     s2 = (s2 < R(kEpsTheta * kEpsTheta)) ? R(kEpsTheta * kEpsTheta) : s2;   // kerr.py:93 floor
     const R r2 = r * r;
@@ -250,21 +247,7 @@ __device__ __forceinline__ void kerr_rhs_free(const RayK<R> &k, R r, R th_in, R 
     const R sig = r2 + k.a2 * (c * c);
     const R dlt = r2a2 - R(2) * r;
     const R P = r2a2 * k.E - k.aL;
-    R inv_s2, inv_sig, inv_dlt;
-    if constexpr (sizeof(R) == 4 || ISLAND) {
-        // one reciprocal of Sigma * Delta * sin^2 serves all three (|product| stays well inside FP32 range)
-        const R sd = sig * dlt;
-        R inv;
-        if constexpr (ISLAND) inv = island_rcp(sd * s2);
-        else inv = Mth<R>::rcp(sd * s2);
-        inv_s2 = sd * inv;
-        inv_sig = (dlt * s2) * inv;
-        inv_dlt = (sig * s2) * inv;
-    } else {
-        inv_s2 = Mth<R>::rcp(s2);
-        inv_sig = Mth<R>::rcp(sig);
-        inv_dlt = Mth<R>::rcp(dlt);
-    }
+    const R inv_s2 = Mth<R>::rcp(s2), inv_sig = Mth<R>::rcp(sig), inv_dlt = Mth<R>::rcp(dlt);
     const R Wd = (P * P) * inv_dlt;
     const R T = k.L2 * inv_s2 - k.twoaEL + k.a2E2 * s2;
     const R pr2 = pr * pr;
@@ -280,10 +263,16 @@ __device__ __forceinline__ void kerr_rhs_free(const RayK<R> &k, R r, R th_in, R 
     dpth = R(-0.5) * inv_sig * (dNth + R(2) * k.a2 * sc * N * inv_sig);
 }
 
+// The same right-hand side in the FP32 kernels and, ISLAND, in the FP64 islands of a mixed kernel (with the
+// island math above).  Its arithmetic, and that of the steps built on it, is written with explicit fused
+// multiply-adds and compiled with contraction off, so that a step rounds the same in every code path that
+// takes it (the ray's own lane, a lone ray's batch, the helper of a split lone ray): left to the compiler,
+// the fusion of a * b + c depends on the code the step is inlined into.
 template <typename R, bool ISLAND = false>
 __device__ __forceinline__ void kerr_rhs_pinned(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
-                                         R &dpth) {
+                                                R &dpth) {
 #pragma clang fp contract(off)
+    static_assert(sizeof(R) == 4 || ISLAND, "the FP64 reference kernels take kerr_rhs_free");
     const R th = clamp_theta(th_in);
     R s, c;
     if constexpr (ISLAND) island_sincos(th, s, c);
@@ -295,21 +284,12 @@ __device__ __forceinline__ void kerr_rhs_pinned(const RayK<R> &k, R r, R th_in, 
     const R sig = fma_hd(k.a2, c * c, r2);
     const R dlt = fma_hd(R(-2), r, r2a2);
     const R P = fma_hd(r2a2, k.E, -k.aL);
-    R inv_s2, inv_sig, inv_dlt;
-    if constexpr (sizeof(R) == 4 || ISLAND) {
-        // one reciprocal of Sigma * Delta * sin^2 serves all three (|product| stays well inside FP32 range)
-        const R sd = sig * dlt;
-        R inv;
-        if constexpr (ISLAND) inv = island_rcp(sd * s2);
-        else inv = Mth<R>::rcp(sd * s2);
-        inv_s2 = sd * inv;
-        inv_sig = (dlt * s2) * inv;
-        inv_dlt = (sig * s2) * inv;
-    } else {
-        inv_s2 = Mth<R>::rcp(s2);
-        inv_sig = Mth<R>::rcp(sig);
-        inv_dlt = Mth<R>::rcp(dlt);
-    }
+    // one reciprocal of Sigma * Delta * sin^2 serves all three (|product| stays well inside FP32 range)
+    const R sd = sig * dlt;
+    R inv;
+    if constexpr (ISLAND) inv = island_rcp(sd * s2);
+    else inv = Mth<R>::rcp(sd * s2);
+    const R inv_s2 = sd * inv, inv_sig = (dlt * s2) * inv, inv_dlt = (sig * s2) * inv;
     const R Wd = (P * P) * inv_dlt;
     const R T = fma_hd(k.a2E2, s2, fma_hd(k.L2, inv_s2, -k.twoaEL));
     const R pr2 = pr * pr;
@@ -325,14 +305,6 @@ __device__ __forceinline__ void kerr_rhs_pinned(const RayK<R> &k, R r, R th_in, 
     dpth = (R(-0.5) * inv_sig) * fma_hd(((R(2) * k.a2) * sc) * N, inv_sig, dNth);
 }
 
-// The FP32 and island steps use the pinned form; the pure FP64 kernels, which take every step in the ray's own
-// lane, keep the compiler's fusion (the form their parity with the oracle was measured on).
-template <typename R, bool ISLAND = false>
-__device__ __forceinline__ void kerr_rhs(const RayK<R> &k, R r, R th_in, R pr, R pth, R &dr, R &dth, R &dph, R &dpr,
-                                         R &dpth) {
-    if constexpr (sizeof(R) == 8 && !ISLAND) kerr_rhs_free<R, ISLAND>(k, r, th_in, pr, pth, dr, dth, dph, dpr, dpth);
-    else kerr_rhs_pinned<R, ISLAND>(k, r, th_in, pr, pth, dr, dth, dph, dpr, dpth);
-}
 
 // ---- brick sampler -----------------------------------------------------------
 
@@ -786,16 +758,17 @@ __device__ __forceinline__ RayK<G> make_k(double a, double E, double L) {
 }
 
 // RK2 midpoint step of the geodesic in precision G: k1 = f(y), k2 = f(y + h/2 k1), y + h k2.
-// Returns the raw (unclamped) end theta in nt_raw and the midpoint theta in mt.
-template <typename G, bool ISLAND = false>
+// Returns the raw (unclamped) end theta in nt_raw and the midpoint theta in mt.  _free: the FP64 reference
+// kernels (kerr_rhs_free); _pinned: the FP32 kernels and the islands, rounded alike in every code path.
+template <typename G>
 __device__ __forceinline__ void geo_step_free(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
-                                         G &npr, G &npt, G &mt) {
+                                              G &npr, G &npt, G &mt) {
     G k1r, k1t, k1p, k1pr, k1pt;
-    kerr_rhs_free<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    kerr_rhs_free<G>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
     const G hh = G(0.5) * h;
     mt = th + hh * k1t;
     G k2r, k2t, k2p, k2pr, k2pt;
-    kerr_rhs_free<G, ISLAND>(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    kerr_rhs_free<G>(k, r + hh * k1r, mt, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
     nr = r + h * k2r;
     nt_raw = th + h * k2t;
     np = ph + h * k2p;
@@ -805,7 +778,7 @@ __device__ __forceinline__ void geo_step_free(const RayK<G> &k, G r, G th, G ph,
 
 template <typename G, bool ISLAND = false>
 __device__ __forceinline__ void geo_step_pinned(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
-                                         G &npr, G &npt, G &mt) {
+                                                G &npr, G &npt, G &mt) {
 #pragma clang fp contract(off)
     G k1r, k1t, k1p, k1pr, k1pt;
     kerr_rhs_pinned<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
@@ -824,28 +797,28 @@ __device__ __forceinline__ void geo_step_pinned(const RayK<G> &k, G r, G th, G p
 template <typename G, bool ISLAND = false>
 __device__ __forceinline__ void geo_step(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw, G &np,
                                          G &npr, G &npt, G &mt) {
-    if constexpr (sizeof(G) == 8 && !ISLAND) geo_step_free<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
+    if constexpr (sizeof(G) == 8 && !ISLAND) geo_step_free<G>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
     else geo_step_pinned<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mt);
 }
 
 // Classical RK4 step of the reference tracer (kerr.py:306-313): k1 = f(y), k2 = f(y + h/2 k1),
 // k3 = f(y + h/2 k2), k4 = f(y + h k3), y + (h/6)(k1 + 2 k2 + 2 k3 + k4), summed in that order.
 // Returns the raw end theta in nt_raw and the lowest and highest theta of the three inner stages.
-template <typename G, bool ISLAND = false>
+template <typename G>
 __device__ __forceinline__ void geo_step4_free(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
-                                          G &np, G &npr, G &npt, G &mlo, G &mhi) {
+                                               G &np, G &npr, G &npt, G &mlo, G &mhi) {
     G k1r, k1t, k1p, k1pr, k1pt;
-    kerr_rhs_free<G, ISLAND>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
+    kerr_rhs_free<G>(k, r, th, pr, pth, k1r, k1t, k1p, k1pr, k1pt);
     const G hh = G(0.5) * h;
     const G t2 = th + hh * k1t;
     G k2r, k2t, k2p, k2pr, k2pt;
-    kerr_rhs_free<G, ISLAND>(k, r + hh * k1r, t2, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
+    kerr_rhs_free<G>(k, r + hh * k1r, t2, pr + hh * k1pr, pth + hh * k1pt, k2r, k2t, k2p, k2pr, k2pt);
     const G t3 = th + hh * k2t;
     G k3r, k3t, k3p, k3pr, k3pt;
-    kerr_rhs_free<G, ISLAND>(k, r + hh * k2r, t3, pr + hh * k2pr, pth + hh * k2pt, k3r, k3t, k3p, k3pr, k3pt);
+    kerr_rhs_free<G>(k, r + hh * k2r, t3, pr + hh * k2pr, pth + hh * k2pt, k3r, k3t, k3p, k3pr, k3pt);
     const G t4 = th + h * k3t;
     G k4r, k4t, k4p, k4pr, k4pt;
-    kerr_rhs_free<G, ISLAND>(k, r + h * k3r, t4, pr + h * k3pr, pth + h * k3pt, k4r, k4t, k4p, k4pr, k4pt);
+    kerr_rhs_free<G>(k, r + h * k3r, t4, pr + h * k3pr, pth + h * k3pt, k4r, k4t, k4p, k4pr, k4pt);
     const G w = h / G(6);
     nr = r + w * (k1r + G(2) * k2r + G(2) * k3r + k4r);
     nt_raw = th + w * (k1t + G(2) * k2t + G(2) * k3t + k4t);
@@ -887,7 +860,7 @@ template <typename G, bool ISLAND = false>
 __device__ __forceinline__ void geo_step4(const RayK<G> &k, G r, G th, G ph, G pr, G pth, G h, G &nr, G &nt_raw,
                                           G &np, G &npr, G &npt, G &mlo, G &mhi) {
     if constexpr (sizeof(G) == 8 && !ISLAND)
-        geo_step4_free<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo, mhi);
+        geo_step4_free<G>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo, mhi);
     else geo_step4_pinned<G, ISLAND>(k, r, th, ph, pr, pth, h, nr, nt_raw, np, npr, npt, mlo, mhi);
 }
 
