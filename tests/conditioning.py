@@ -2,7 +2,7 @@
 
 A ray whose image value moves by a large factor when its initial state moves by a rounding error cannot be
 held to a tight bar in any precision: two correct FP64 codes that round differently disagree there.  The
-configs[0] scene (64 fixed RK2 steps of h = 20 M from r = 1000) is such a case for most of its rays, whose
+small plain scene (64 fixed RK2 steps of h = 20 M from r = 1000) is such a case for most of its rays, whose
 steps jump across the coordinate axis near the hole.  ``well_conditioned`` renders the scene with the
 oracle at the camera radius and field of view scaled by 1 - delta and 1 + delta and returns the pixels
 whose status and step count stay the same and whose I moves by less than ``tol`` relative: on those, a

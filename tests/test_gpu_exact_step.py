@@ -2,8 +2,8 @@
 polarized_transfer.py:51-80) against the FP64 oracle, which takes scipy's expm of the augmented
 matrix like the reference.
 
-* C0 (32x18, fixed h = 20) and the scale scene at 48^2 in FP64: statuses and step counts equal; on C0
-  relative I error <= 1e-8 on every pixel the oracle shows to be well conditioned at FP64 rounding
+* The small plain scene (32x18, fixed h = 20) and the scale scene at 48^2 in FP64: statuses and step counts
+  equal; on the small scene relative I error <= 1e-8 on every pixel the oracle shows to be well conditioned at FP64 rounding
   (tests/conditioning.py), on the scale scene <= 1e-9 on 99 % of unflagged pixels.
 * FP32 and mixed-precision exact renders against the FP64 exact render: rel-L1(I) <= 1e-4 / 1e-3.
 * A batch of exact renders equals the single renders; the exact image differs from the RK2 one.
@@ -40,7 +40,7 @@ def test_small_scene_exact_fp64_matches_oracle(renderer):
     I, I0 = out["iquv"][..., 0], ref["iquv"][..., 0]
     rel = np.abs(I - I0) / np.maximum(np.abs(I0), 1e-300)
     cond = well_conditioned(sc, br, 1e-12, 1e-6, ref)
-    print(f"C0 exact fp64: {cond.mean():.3f} well conditioned, max rel there {rel[cond].max():.2e}; all pixels "
+    print(f"small scene exact fp64: {cond.mean():.3f} well conditioned, max rel there {rel[cond].max():.2e}; all pixels "
           f"median {np.median(rel):.2e} max {rel.max():.2e}")
     assert cond.mean() >= 0.5
     assert rel[cond].max() <= 1e-8

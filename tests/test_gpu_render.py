@@ -1,7 +1,6 @@
 """GPU parity of the fused render (FP64 and plain FP32) against the FP64 oracle.
 
-* Small plain scene (configs[0]: a=0.9, r_obs=1000, i=60 deg, 32x18, 64 fixed RK2 steps of h=20, 8x6x8
-  brick).  Steps of 20 M jump across the coordinate axis near the hole, so every ray is pole-flagged and
+* Small plain scene (a=0.9, r_obs=1000, i=60 deg, 32x18, 64 fixed RK2 steps of h=20, 8x6x8 brick).  Steps of 20 M jump across the coordinate axis near the hole, so every ray is pole-flagged and
   most of them amplify rounding errors enormously (a 1-ulp change of the camera moves I by 38 % at the
   90th percentile in the oracle itself).  Bars: statuses and step counts equal to the oracle on every
   pixel (FP64); I within 1e-9 relative on every pixel the oracle shows to be well conditioned at FP64
@@ -10,13 +9,12 @@
   1e-6 change; 64 %): statuses and steps on >= 45 % of them (measured 56 %: FP32 rounding inside a
   step that crosses the axis is not a perturbation of the camera) and I within 5e-4 on every one of
   those that agree (measured 4.7e-5).
-* Scale scene (configs[1]: a=0.94, i=17 deg, 1024^2, h = 0.05 (r - r_+), <= 4096 steps, 128x64x128 disk
-  brick): a seeded pixel subset of the FP64 render against the oracle (statuses and steps on >= 99 % of
+* Scale scene (a=0.94, i=17 deg, 1024^2, h = 0.05 (r - r_+), <= 4096 steps, 128x64x128 disk brick): a seeded pixel subset of the FP64 render against the oracle (statuses and steps on >= 99 % of
   unflagged pixels, I within 1e-9 on 99 % of those that agree); the full FP32 image against the FP64
   render: relative L1 error of I <= 1e-4, per-pixel relative I error and max(|dQ|, |dU|, |dV|) / |I|
   <= 1e-4 at the 99th percentile, max |dQUV| <= 3e-2 of the image's largest |I| (a few near-critical
   rays diverge: the oracle confirms the FP64 values of the worst pixels), at most 0.1 % of the pixels
-  flagged in one render only.  Mixed precision against the FP64 render: relative L1 error
+  flagged in one render only.  Mixed precision (bf16 / f16 bricks, FP64 islands) against the FP64 render: relative L1 error
   of I <= 1e-3 (bf16) / 2e-4 (f16), per-pixel p99 of I and Q/U/V <= 3e-3 / 5e-4, the same max and flag
   bars.
 * Rays flagged by the theta clamp (status bit 3) left the geodesic at a pole: on the scale scene their
