@@ -3172,6 +3172,31 @@ static int check_surrogate_desc(const bh_surrogate_desc *sd, bh_mlp *mlp, bh_ctx
     return BH_OK;
 }
 
+// Host floats into device memory through kernel arguments, which the launch copies: the host array need not
+// outlive the call and the stream is not synchronised, so a caller may capture the sequence in a graph.
+constexpr int kArgFloats = 896;   // 3.5 KB of values per launch, inside the kernel-argument limit
+struct FloatChunk {
+    float v[kArgFloats];
+    float *dst;
+    int n;
+};
+
+__global__ __launch_bounds__(256) void upload_floats_kernel(FloatChunk c) {
+    for (int i = threadIdx.x; i < c.n; i += blockDim.x) c.dst[i] = c.v[i];
+}
+
+static int upload_floats(hipStream_t stream, float *dst, const float *src, size_t n) {
+    for (size_t o = 0; o < n; o += kArgFloats) {
+        FloatChunk c;
+        c.dst = dst + o;
+        c.n = int(n - o < size_t(kArgFloats) ? n - o : size_t(kArgFloats));
+        std::memcpy(c.v, src + o, size_t(c.n) * sizeof(float));
+        hipLaunchKernelGGL(upload_floats_kernel, dim3(1), dim3(256), 0, stream, c);
+    }
+    BH_HIP(hipGetLastError());
+    return BH_OK;
+}
+
 static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id, bh_mlp *mlp, const float *rows_in,
                        int64_t ld_in, const bh_surrogate_desc *sd, float *d_work, size_t work_bytes, void *d_iquv,
                        int32_t *d_steps, uint8_t *d_status) {
@@ -3218,10 +3243,10 @@ static int knot_render(bh_ctx *ctx, const bh_scene_desc *scene, int32_t brick_id
     float *d_y = mlp ? reinterpret_cast<float *>(take(size_t(rows) * n_out * 4)) : nullptr;
     const std::vector<float> curve = thinned_curve(scene->spin, S.R.cam.th);
     S.n_curve = int(curve.size() / 2);
-    BH_HIP(hipMemcpyAsync(d_mul, sd->out_mul, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream));
-    BH_HIP(hipMemcpyAsync(d_add, sd->out_add, size_t(n_out) * 4, hipMemcpyHostToDevice, ctx->stream));
-    if (S.n_curve) BH_HIP(hipMemcpyAsync(d_curve, curve.data(), curve.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    BH_HIP(hipStreamSynchronize(ctx->stream));
+    if ((st = upload_floats(ctx->stream, d_mul, sd->out_mul, size_t(n_out))) ||
+        (st = upload_floats(ctx->stream, d_add, sd->out_add, size_t(n_out))) ||
+        (st = upload_floats(ctx->stream, d_curve, curve.data(), curve.size())))
+        return st;
     // the fused inference writes the knots as FP16 (half the bytes to write and to read back)
     const bool half = mlp && mlp_half_outputs(mlp);
     S.y = mlp ? static_cast<const void *>(d_y) : static_cast<const void *>(rows_in);
