@@ -31,8 +31,8 @@ constexpr int kKStep = 32;    // K per MFMA pass
 enum Epi { EPI_STORE = 0, EPI_BIAS_SILU = 1, EPI_BIAS = 2, EPI_DSILU = 3, EPI_MSE = 5 };
 
 struct GemmArgs {
-    const __bf16 *A;   // op(A): M x K; stored [M][K] (A_KC) or [K][M]
-    const __bf16 *B;   // op(B): K x N; stored [N][K] (B_KC) or [K][N]
+    const __bf16 *A;   // op(A): M x K, stored [M][K] (k contiguous)
+    const __bf16 *B;   // op(B): K x N, stored [N][K] (k contiguous)
     int M, N, K;
     int lda, ldb;
     int n_store;       // columns n < n_store are written
@@ -41,6 +41,8 @@ struct GemmArgs {
     int ldz;
     float *Cf;         // FP32 result (optional)
     __bf16 *Cb;        // BF16 result (optional)
+    __bf16 *Cbt;       // the same BF16 values transposed, [n][m] with row stride ldct (optional)
+    long long ldct;
     int ldc;
     long long z_stride;   // elements between the results of successive grid.z slices (split K) in Cf
     int c_trans;       // Cf stored [n][m] (ld = ldc) instead of [m][n]
@@ -61,49 +63,46 @@ __device__ __forceinline__ float dsilu(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
-// A 64 x kStageK (rows x k) operand tile moves through registers into LDS as [row][k]: every thread loads
-// kStageLoads 16-byte pieces (all in flight together), then stores them.  Zeros past the operand's rows
-// and past the slice's K range.
+// A 64 x kStageK (rows x k) operand tile, stored [row][k] with row stride ld, moves through registers into
+// LDS as [row][k]: every thread loads kStageLoads 16-byte pieces (all in flight together), then stores
+// them.  Zeros past the operand's rows and past the slice's K range.  Every operand is k-contiguous: the
+// weight-gradient GEMMs read the transposed copies of their inputs that the producing kernels write.
 constexpr int kStageK = 128;
 constexpr int kLdsStage = kStageK + 8;
 constexpr int kStageLoads = kStageK / kKStep;
 
-template <bool KC>
 __device__ __forceinline__ void load_stage(const __bf16 *src, int ld, int rows, int K, int r0, int k0, uint4 (&v)[kStageLoads],
                                            int tid) {
+    const int r = tid >> 2;
 #pragma unroll
     for (int s = 0; s < kStageLoads; ++s) {
         v[s] = uint4{0u, 0u, 0u, 0u};
-        if constexpr (KC) {   // stored [row][k]
-            const int r = tid >> 2, kk = s * kKStep + (tid & 3) * 8;
-            if (r0 + r < rows && k0 + kk < K) v[s] = *reinterpret_cast<const uint4 *>(src + size_t(r0 + r) * ld + k0 + kk);
-        } else {              // stored [k][row]
-            const int k = s * kKStep + (tid >> 3), rr = (tid & 7) * 8;
-            if (k0 + k < K && r0 + rr < rows) v[s] = *reinterpret_cast<const uint4 *>(src + size_t(k0 + k) * ld + r0 + rr);
+        const int kk = s * kKStep + (tid & 3) * 8;
+        if (r0 + r < rows && k0 + kk < K) {
+            const __bf16 *q = src + size_t(r0 + r) * ld + k0 + kk;
+            if (k0 + kk + 8 <= K) {
+                v[s] = *reinterpret_cast<const uint4 *>(q);
+            } else {   // the last, partial group of the K range: zeros past it
+                unsigned short e[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    e[i] = k0 + kk + i < K ? *reinterpret_cast<const unsigned short *>(q + i) : (unsigned short)0;
+                v[s] = __builtin_bit_cast(uint4, e);
+            }
         }
     }
 }
 
-template <bool KC>
 __device__ __forceinline__ void store_stage(const uint4 (&v)[kStageLoads], __bf16 (*dst)[kLdsStage], int tid) {
 #pragma unroll
-    for (int s = 0; s < kStageLoads; ++s) {
-        if constexpr (KC) {
-            *reinterpret_cast<uint4 *>(&dst[tid >> 2][s * kKStep + (tid & 3) * 8]) = v[s];
-        } else {
-            const int k = s * kKStep + (tid >> 3), rr = (tid & 7) * 8;
-            const unsigned short *e = reinterpret_cast<const unsigned short *>(&v[s]);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) *reinterpret_cast<unsigned short *>(&dst[rr + i][k]) = e[i];
-        }
-    }
+    for (int s = 0; s < kStageLoads; ++s) *reinterpret_cast<uint4 *>(&dst[tid >> 2][s * kKStep + (tid & 3) * 8]) = v[s];
 }
 
 // C tile (64 x 64) = op(A) op(B) over this slice's K range, in stages of kStageK: the next stage's loads
 // are in flight while the current stage's MFMAs run.  Four waves, each a 32 x 32 quadrant of 2 x 2 MFMA
 // 16x16x32 tiles, the k-steps in order (so the sums do not depend on the staging).  Lane l of an MFMA
 // holds A[l%16][(l/16)*8 .. +7] and B[(l/16)*8 .. +7][l%16] and receives D[(l/16)*4 + q][l%16], q = 0..3.
-template <bool A_KC, bool B_KC, int EPI>
+template <int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     __shared__ __align__(16) __bf16 sA[kTile][kLdsStage];
     __shared__ __align__(16) __bf16 sB[kTile][kLdsStage];
@@ -120,15 +119,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int kb = (lane >> 4) * 8, rl = lane & 15;
     uint4 va[kStageLoads], vb[kStageLoads];
-    load_stage<A_KC>(g.A, g.lda, g.M, kend, m0, kbeg, va, tid);
-    load_stage<B_KC>(g.B, g.ldb, g.N, kend, n0, kbeg, vb, tid);
+    load_stage(g.A, g.lda, g.M, kend, m0, kbeg, va, tid);
+    load_stage(g.B, g.ldb, g.N, kend, n0, kbeg, vb, tid);
     for (int k0 = kbeg; k0 < kend; k0 += kStageK) {
-        store_stage<A_KC>(va, sA, tid);
-        store_stage<B_KC>(vb, sB, tid);
+        store_stage(va, sA, tid);
+        store_stage(vb, sB, tid);
         __syncthreads();
         if (k0 + kStageK < kend) {
-            load_stage<A_KC>(g.A, g.lda, g.M, kend, m0, k0 + kStageK, va, tid);
-            load_stage<B_KC>(g.B, g.ldb, g.N, kend, n0, k0 + kStageK, vb, tid);
+            load_stage(g.A, g.lda, g.M, kend, m0, k0 + kStageK, va, tid);
+            load_stage(g.B, g.ldb, g.N, kend, n0, k0 + kStageK, vb, tid);
         }
 #pragma unroll
         for (int ks = 0; ks < kStageLoads; ++ks) {
@@ -151,11 +150,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) {
+            // the lane's four rows m of column n, for the transposed copy (one 8-byte store when all are rows)
+            union {
+                __bf16 h[4];
+                uint2 u;
+            } tv;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
                 const int n = n0 + wn + j * 16 + rl;
+                tv.h[q] = __bf16(0.f);
                 if (m >= g.M || n >= g.n_store) continue;
                 float v = acc[i][j][q];
                 if constexpr (EPI == EPI_BIAS_SILU) {
@@ -182,7 +187,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                 const size_t ci = g.c_trans ? size_t(n) * g.ldc + m : size_t(m) * g.ldc + n;
                 if (g.Cf) g.Cf[size_t(blockIdx.z) * size_t(g.z_stride) + ci] = v;
                 if (g.Cb) g.Cb[ci] = __bf16(v);
+                tv.h[q] = __bf16(v);
             }
+            if (g.Cbt) {
+                const int mb = m0 + wm + i * 16 + (lane >> 4) * 4, n = n0 + wn + j * 16 + rl;
+                if (n < g.n_store) {
+                    __bf16 *t = g.Cbt + size_t(n) * size_t(g.ldct) + mb;
+                    if (mb + 3 < g.M) {
+                        *reinterpret_cast<uint2 *>(t) = tv.u;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (mb + q < g.M) t[q] = tv.h[q];
+                    }
+                }
+            }
+        }
     if constexpr (EPI == EPI_DSILU || EPI == EPI_MSE) {
         // the four lanes of a column (lane / 16) hold its 4 x 2 x 4 values of this wave's 32 rows; the two
         // waves of a column range (wm = 0, 32) meet in LDS, and the tile's column sums are stored (no atomics,
@@ -282,12 +302,15 @@ __global__ __launch_bounds__(256) void sum_kernel(const float *x, int n, float *
 }
 
 // FP32 [rows][n] -> BF16 [rows][n_pad], zero padded
-__global__ void pack_input_kernel(const float *x, int rows, int n, int n_pad, __bf16 *y) {
+// BF16 inputs [rows][n_pad] (zero-padded columns), and when yt is set their transpose [n_pad][ldt].
+__global__ void pack_input_kernel(const float *x, int rows, int n, int n_pad, __bf16 *y, __bf16 *yt, long long ldt) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(rows) * n_pad) return;
     const int c = int(i % n_pad);
     const size_t r = i / n_pad;
-    y[i] = __bf16(c < n ? x[r * n + c] : 0.0f);
+    const __bf16 v = __bf16(c < n ? x[r * n + c] : 0.0f);
+    y[i] = v;
+    if (yt) yt[size_t(c) * size_t(ldt) + r] = v;
 }
 
 __global__ void cast_bf16_kernel(const float *x, __bf16 *y, size_t n) {
@@ -588,6 +611,11 @@ struct bh_mlp {
     std::vector<float *> zpre;                 // per hidden layer pre-activation [cap][width]
     __bf16 *dyb = nullptr;                     // [cap][n_out_pad] loss gradient of the outputs
     __bf16 *dab = nullptr, *dab2 = nullptr;    // [cap][widest] pre-activation gradients, alternating
+    // training: the same BF16 inputs, activations and gradients transposed ([columns][ldt]), written by the
+    // kernels that produce them, so the weight-gradient GEMMs read both operands k-contiguous
+    long long ldt = 0;
+    __bf16 *xbT = nullptr, *dybT = nullptr, *dabT = nullptr, *dab2T = nullptr;
+    std::vector<__bf16 *> actT;
     float *loss = nullptr;
     float *loss_part = nullptr;                // per-workgroup loss of the output GEMM (one per 64 x 64 tile)
     int loss_parts = 0;
@@ -601,14 +629,14 @@ struct bh_mlp {
 
 namespace bh {
 
-template <bool A_KC, bool B_KC, int EPI>
+template <int EPI>
 static int run_gemm(bh_ctx *ctx, const GemmArgs &g0, int splits = 1) {
     GemmArgs g = g0;
     if (g.n_store == 0) g.n_store = g.N;
     g.k_split = ((g.K + splits - 1) / splits + kKStep - 1) / kKStep * kKStep;
     const int z = (g.K + g.k_split - 1) / g.k_split;
     dim3 grid((g.N + kTile - 1) / kTile, (g.M + kTile - 1) / kTile, z);
-    hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI>), grid, dim3(256), 0, ctx->stream, g);
+    hipLaunchKernelGGL((gemm_kernel<EPI>), grid, dim3(256), 0, ctx->stream, g);
     BH_HIP(hipGetLastError());
     return BH_OK;
 }
@@ -633,8 +661,13 @@ static void free_acts(bh_mlp *m) {
     for (auto p : m->zpre)
         if (p) (void)hipFree(p);
     for (void *p : {(void *)m->dyb, (void *)m->dab, (void *)m->dab2, (void *)m->loss_part, (void *)m->bias_part,
-                    (void *)m->dw_part})
+                    (void *)m->dw_part, (void *)m->xbT, (void *)m->dybT, (void *)m->dabT, (void *)m->dab2T})
         if (p) (void)hipFree(p);
+    for (auto p : m->actT)
+        if (p) (void)hipFree(p);
+    m->actT.clear();
+    m->xbT = m->dybT = m->dabT = m->dab2T = nullptr;
+    m->ldt = 0;
     m->loss_part = m->bias_part = m->dw_part = nullptr;
     m->dw_splits = 0;
     m->loss_parts = 0;
@@ -671,6 +704,18 @@ static int ensure_cap(bh_mlp *m, int rows, bool train) {
              hipMalloc(&m->bias_part, ((R + kTile - 1) / kTile) * size_t(m->n_bias) * 4) == hipSuccess &&
              hipMalloc(&m->dw_part, size_t(dw_splits_for(rows)) * weight_count(m) * 4) == hipSuccess;
     if (ok && train) {
+        const size_t ldt = (R + 63) / 64 * 64;
+        ok = hipMalloc(&m->xbT, ldt * m->n_in_pad * 2) == hipSuccess &&
+             hipMalloc(&m->dybT, ldt * m->n_out_pad * 2) == hipSuccess &&
+             hipMalloc(&m->dabT, ldt * widest * 2) == hipSuccess && hipMalloc(&m->dab2T, ldt * widest * 2) == hipSuccess;
+        for (int l = 0; l < m->depth && ok; ++l) {
+            __bf16 *a = nullptr;
+            ok = hipMalloc(&a, ldt * m->width * 2) == hipSuccess;
+            m->actT.push_back(a);
+        }
+        m->ldt = (long long)ldt;
+    }
+    if (ok && train) {
         m->loss_parts = parts;
         m->dw_splits = dw_splits_for(rows);
     }
@@ -704,7 +749,7 @@ static int refresh_bf16(bh_mlp *m) {
 static int forward(bh_mlp *m, const float *d_x, int rows, bool train, float *y_out, int trans, int ld_out) {
     bh_ctx *ctx = m->ctx;
     hipLaunchKernelGGL(pack_input_kernel, dim3(blocks_for(size_t(rows) * m->n_in_pad)), dim3(256), 0, ctx->stream, d_x,
-                       rows, m->n_in, m->n_in_pad, m->xb);
+                       rows, m->n_in, m->n_in_pad, m->xb, train ? m->xbT : nullptr, m->ldt);
     const __bf16 *in = m->xb;
     int st;
     for (int l = 0; l <= (train ? m->depth - 1 : m->depth); ++l) {
@@ -722,14 +767,16 @@ static int forward(bh_mlp *m, const float *d_x, int rows, bool train, float *y_o
             g.ldz = g.N;
             g.Cb = m->act[l];
             g.ldc = g.N;
-            st = run_gemm<true, true, EPI_BIAS_SILU>(ctx, g);
+            g.Cbt = train ? m->actT[l] : nullptr;
+            g.ldct = m->ldt;
+            st = run_gemm<EPI_BIAS_SILU>(ctx, g);
             in = m->act[l];
         } else {
             g.Cf = y_out;
             g.n_store = m->n_out;
             g.c_trans = trans;
             g.ldc = ld_out;
-            st = run_gemm<true, true, EPI_BIAS>(ctx, g);
+            st = run_gemm<EPI_BIAS>(ctx, g);
         }
         if (st) return st;
     }
@@ -769,6 +816,8 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         g.bias = m->w32 + m->b_off[l];
         g.Cb = m->dyb;
         g.ldc = g.N;
+        g.Cbt = m->dybT;
+        g.ldct = m->ldt;
         g.colpart = m->bias_part + bias_part_base(m, l);
         g.t = d_t;
         g.w = d_w;
@@ -776,9 +825,10 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         g.n_real = m->n_out;
         g.masked = masked;
         g.loss_part = m->loss_part;
-        if ((st = run_gemm<true, true, EPI_MSE>(ctx, g))) return st;
+        if ((st = run_gemm<EPI_MSE>(ctx, g))) return st;
     }
     const __bf16 *dcur = m->dyb;   // gradient w.r.t. this layer's pre-activation, BF16
+    const __bf16 *dcurT = m->dybT;  // the same, transposed
     // weight gradients in split-K slices of the batch rows (as run_gemm cuts them), summed afterwards
     const int splits = dw_splits_for(rows);
     const int k_split = ((rows + splits - 1) / splits + kKStep - 1) / kKStep * kKStep;
@@ -795,23 +845,23 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
     }
     for (int l = m->depth; l >= 0; --l) {
         const int K = m->fi[l], N = m->fo[l];
-        const __bf16 *xin = l == 0 ? m->xb : m->act[l - 1];
-        // dW [K][N] = x^T dZ, split over rows
+        // dW [K][N] = x^T dZ, split over rows, from the transposed copies ([K][rows] and [N][rows])
         GemmArgs gw{};
-        gw.A = xin;
-        gw.lda = K;
-        gw.B = dcur;
-        gw.ldb = N;
+        gw.A = l == 0 ? m->xbT : m->actT[l - 1];
+        gw.lda = int(m->ldt);
+        gw.B = dcurT;
+        gw.ldb = int(m->ldt);
         gw.M = K;
         gw.N = N;
         gw.K = rows;
         gw.Cf = m->dw_part + W.part_off[l];
         gw.ldc = N;
         gw.z_stride = W.n[l];
-        if ((st = run_gemm<false, false, EPI_STORE>(ctx, gw, splits))) return st;
+        if ((st = run_gemm<EPI_STORE>(ctx, gw, splits))) return st;
         if (l == 0) break;
         // dZ_prev = (dZ W^T) * silu'(z_prev)
         __bf16 *dnext = (dcur == m->dab) ? m->dab2 : m->dab;
+        __bf16 *dnextT = (dcur == m->dab) ? m->dab2T : m->dabT;
         GemmArgs gd{};
         gd.A = dcur;
         gd.lda = N;
@@ -824,9 +874,12 @@ static int backward(bh_mlp *m, int rows, const float *d_t, const float *d_w, con
         gd.ldz = K;
         gd.Cb = dnext;
         gd.ldc = K;
+        gd.Cbt = dnextT;
+        gd.ldct = m->ldt;
         gd.colpart = m->bias_part + bias_part_base(m, l - 1);
-        if ((st = run_gemm<true, true, EPI_DSILU>(ctx, gd))) return st;
+        if ((st = run_gemm<EPI_DSILU>(ctx, gd))) return st;
         dcur = dnext;
+        dcurT = dnextT;
     }
     hipLaunchKernelGGL(weight_reduce_kernel, dim3(blocks_for(weight_count(m))), dim3(256), 0, ctx->stream, m->dw_part, W,
                        m->grad);
