@@ -189,20 +189,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
                 if (g.Cb) g.Cb[ci] = __bf16(v);
                 tv.h[q] = __bf16(v);
             }
-            if (g.Cbt) {
-                const int mb = m0 + wm + i * 16 + (lane >> 4) * 4, n = n0 + wn + j * 16 + rl;
-                if (n < g.n_store) {
-                    __bf16 *t = g.Cbt + size_t(n) * size_t(g.ldct) + mb;
-                    if (mb + 3 < g.M) {
-                        *reinterpret_cast<uint2 *>(t) = tv.u;
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (mb + q < g.M) t[q] = tv.h[q];
-                    }
-                }
-            }
+            // the transposed copy goes through LDS (the staging tile is free after the main loop's last
+            // barrier): column n of the tile becomes one 128-byte row of Cbt
+            if (g.Cbt) *reinterpret_cast<uint2 *>(&sA[wn + j * 16 + rl][wm + i * 16 + (lane >> 4) * 4]) = tv.u;
         }
+    if (g.Cbt) {   // wave-uniform
+        __syncthreads();
+#pragma unroll
+        for (int e = tid; e < kTile * 8; e += 256) {
+            const int nl = e >> 3, ml = (e & 7) * 8;
+            // rows of the tile past M hold zeros, and the copy's row stride reaches the tile's end
+            if (n0 + nl < g.n_store)
+                *reinterpret_cast<uint4 *>(g.Cbt + size_t(n0 + nl) * size_t(g.ldct) + m0 + ml) =
+                    *reinterpret_cast<const uint4 *>(&sA[nl][ml]);
+        }
+    }
     if constexpr (EPI == EPI_DSILU || EPI == EPI_MSE) {
         // the four lanes of a column (lane / 16) hold its 4 x 2 x 4 values of this wave's 32 rows; the two
         // waves of a column range (wm = 0, 32) meet in LDS, and the tile's column sums are stored (no atomics,
