@@ -42,7 +42,7 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed); scalar issue caps at
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the plain FP32 kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled x2.0 / x1.0 by a 256 MiB copy); profiles/render_kerr_stokes_1024.md
-TRAFFIC_PLAIN_FP32 = 1.833e9 + 27.7e6
+TRAFFIC_PLAIN_FP32 = 1.836e9 + 27.4e6
 
 
 def log(msg: str) -> None:
@@ -476,7 +476,7 @@ def main(argv=None) -> int:
                                 ("batched_mixed_f16", "f16", PREC_MIXED)):
         dbx = rr.upload(brick, storage, "corner")
         bt = Batch(rr, [(fs, dbx) for fs in frame_scenes], prec)
-        tuned = autotune(rr, prec, lambda: bt.launch(graph=False), space={"steps_per_sync": (8, 16, 32),
+        tuned = autotune(rr, prec, lambda: bt.launch(graph=False), space={"steps_per_sync": (8, 16, 32, 48),
                                                                           "block_threads": (64, 256)}, reps=2)
         for _ in range(args.warmup):
             bt.launch(graph=True)
@@ -513,7 +513,7 @@ def main(argv=None) -> int:
     bricks = [rr.upload(b, "bf16", "corner") for b in scenes.batch_bricks()]
     jobs = [(sc, bricks[k]) for sc, k in scenes.batch_scenes(64)]
     bt = Batch(rr, jobs, PREC_MIXED)
-    tuned = autotune(rr, PREC_MIXED, lambda: bt.launch(graph=False), space={"steps_per_sync": (8, 16, 32),
+    tuned = autotune(rr, PREC_MIXED, lambda: bt.launch(graph=False), space={"steps_per_sync": (8, 16, 32, 48),
                                                                              "block_threads": (64, 256)}, reps=1)
     bt.launch(graph=True)
     rr.synchronize()

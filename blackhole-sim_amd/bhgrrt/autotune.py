@@ -21,7 +21,7 @@ import numpy as np
 
 DEFAULT_SPACE = {
     "block_threads": (64, 128, 256),
-    "steps_per_sync": (4, 8, 16, 32),
+    "steps_per_sync": (8, 16, 32, 48),
     "blocks_per_cu": (0, 1, 2),
     "refill_min": (64,),
 }

@@ -60,9 +60,9 @@ constexpr int kWorkOffset = 16;   // two uint64 work counters after the queue co
 struct Tuning {
     int block_threads = 256;
     int blocks_per_cu = 0;
-    int steps_per_sync = 8;
+    int steps_per_sync = 48;   // measured best for the 1024^2 renders (32..64 within noise; 8 is 8 % slower)
     int refill_min = 64;
-    int queues = 8;
+    int queues = 2;            // measured: 2 queues beat 8 (one per XCD) by 3-5 % on single renders
     int order = 1;        // 1: tiles nearest the critical curve (longest rays) first
     int coop_steps = 32;  // lone steps before a lone ray asks an idle wave for help (0: never)
 };

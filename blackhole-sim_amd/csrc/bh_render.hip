@@ -20,7 +20,7 @@
 //  * The r/theta/phi grids are staged in LDS.  Brackets take an O(1) index guess
 //    (log2 for geometric r grids, affine otherwise) and correct it against the
 //    grid with compares, which reproduces searchsorted(side='right') - 1 exactly.
-//  * Ray scheduling: persistent waves pull rays from per-XCD queues (block b
+//  * Ray scheduling: persistent waves pull rays from a few ray queues (block b
 //    serves queue b % queues and steals from the others when it runs dry).  Ray
 //    ids are ordered in 8x8 pixel tiles so a wave's lanes trace neighbouring
 //    rays and share cache lines.  Every steps_per_sync steps a wave ballots its

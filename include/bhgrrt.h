@@ -190,7 +190,7 @@ typedef struct bh_tuning {
     int32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu (0: occupancy) */
     int32_t steps_per_sync;   /* steps between wave refill checks */
     int32_t refill_min;       /* refill when at least this many lanes are idle (1..64) */
-    int32_t queues;           /* 1, or 8 for one ray queue per XCD */
+    int32_t queues;           /* ray queues, 1..8; workgroup b serves queue b % queues first */
     int32_t order;            /* 0: row-major tiles; 1: tiles nearest the critical curve first */
     int32_t coop_steps;       /* render kernels: a ray alone in its wave for this many steps asks an idle
                                  wave of its workgroup to take its geodesic and brick samples (0: never;
